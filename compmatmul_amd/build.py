@@ -1,0 +1,73 @@
+"""Builds the native parts of compmatmul_amd in-tree.
+
+* ``libcompmatmul_amd.so``: HIP kernels for gfx950 plus the C-ABI
+  (hipcc, one object per source, compiled in parallel).
+* ``compmatmul<ext>``: the pybind11 drop-in module ``compmatmul``, linked
+  against the C-ABI library with an ``$ORIGIN`` rpath.
+
+Usage: ``python -m compmatmul_amd.build`` (or ``build()`` from Python).
+"""
+import concurrent.futures as cf
+import os
+import subprocess
+import sys
+import sysconfig
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+OBJ = os.path.join(HERE, "_build")
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
+HIPCC = os.path.join(ROCM, "bin", "hipcc")
+LIB = os.path.join(HERE, "libcompmatmul_amd.so")
+EXT = os.path.join(HERE, "compmatmul" + sysconfig.get_config_var("EXT_SUFFIX"))
+
+HIP_FLAGS = [
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
+    # packed FP32 VALU instructions stay disabled for all device code
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wno-unused-command-line-argument",
+]
+
+SOURCES = ["hash.hip", "compress.hip", "compress_reg.hip", "decompress.hip",
+           "plain.hip", "api.cpp", "host_hash.cpp"]
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("build failed:\n" + " ".join(cmd) + "\n" + r.stdout + r.stderr)
+    return r
+
+
+def _compile(src):
+    out = os.path.join(OBJ, os.path.splitext(src)[0] + ".o")
+    path = os.path.join(CSRC, src)
+    lang = ["-x", "hip"] if src.endswith(".hip") else []
+    _run([HIPCC] + HIP_FLAGS + lang + ["-c", path, "-o", out])
+    return out
+
+
+def build(verbose=False):
+    os.makedirs(OBJ, exist_ok=True)
+    srcs = [s for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
+    with cf.ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
+        objs = list(ex.map(_compile, srcs))
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", LIB + ".tmp"] + objs)
+    os.replace(LIB + ".tmp", LIB)
+    import pybind11
+    py_inc = sysconfig.get_paths()["include"]
+    _run(["g++", "-O2", "-std=c++17", "-shared", "-fPIC",
+          "-I", pybind11.get_include(), "-I", py_inc,
+          os.path.join(CSRC, "pybind_compmatmul.cpp"),
+          "-L", HERE, "-lcompmatmul_amd", "-Wl,-rpath,$ORIGIN",
+          "-o", EXT + ".tmp"])
+    os.replace(EXT + ".tmp", EXT)
+    if verbose:
+        print("built", LIB, "and", EXT)
+    return LIB
+
+
+if __name__ == "__main__":
+    build(verbose=True)
+    sys.exit(0)

@@ -1,0 +1,781 @@
+// C-ABI of the MI355X compressed matrix product (declared in
+// include/compmatmul_amd.h).  Host orchestration: argument checks, hash
+// parameter drawing, workspace management, kernel sequencing on one HIP
+// stream, timing.  All device arithmetic lives in the .hip files.
+#include "../../include/compmatmul_amd.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "host_hash.hpp"
+#include "kernels.hpp"
+
+struct cm_sketch {
+  int fwht, f64, d, b, lb, m, p, family;
+  uint64_t seed;
+  uint32_t* hA;  // device, d x m packed bucket/sign
+  uint32_t* hB;  // device, d x p
+  void* P;       // device, d x b sketch values (element type per f64)
+};
+
+namespace {
+
+thread_local std::string g_err;
+std::recursive_mutex g_mu;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define CM_HIP(expr)                                                              \
+  do {                                                                            \
+    hipError_t e_ = (expr);                                                       \
+    if (e_ != hipSuccess)                                                         \
+      return fail(e_ == hipErrorOutOfMemory ? CM_ENOMEM : CM_EDEVICE,             \
+                  std::string(#expr) + ": " + hipGetErrorString(e_));             \
+  } while (0)
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t n = 0;
+};
+
+struct State {
+  int device = -1;
+  hipStream_t stream = nullptr;
+  DevBuf partial, V, params, tables, AT, B, C, counter, oi, oj, ov, plainA, plainB;
+  void* pinned = nullptr;
+  size_t pinned_n = 0;
+  std::map<std::pair<int, int>, void*> twiddles;  // (log2 b, f64) -> table
+  bool timing = false;
+  struct Rec {
+    std::string name;
+    hipEvent_t a, b;
+  };
+  std::vector<Rec> pending;
+  std::vector<hipEvent_t> event_pool;
+  std::map<std::string, std::pair<double, int64_t>> totals;
+};
+State g;
+
+int ensure_buf(DevBuf& b, size_t bytes) {
+  if (b.n >= bytes && b.p) return CM_OK;
+  if (b.p) {
+    CM_HIP(hipStreamSynchronize(g.stream));
+    CM_HIP(hipFree(b.p));
+    b.p = nullptr;
+    b.n = 0;
+  }
+  const size_t want = std::max<size_t>(bytes, 256);
+  hipError_t e = hipMalloc(&b.p, want);
+  if (e != hipSuccess) {
+    b.p = nullptr;
+    return fail(CM_ENOMEM, "device allocation of " + std::to_string(want) + " bytes failed");
+  }
+  b.n = want;
+  return CM_OK;
+}
+
+int ensure_pinned(size_t bytes) {
+  if (g.pinned_n >= bytes) return CM_OK;
+  if (g.pinned) {
+    CM_HIP(hipStreamSynchronize(g.stream));
+    CM_HIP(hipHostFree(g.pinned));
+    g.pinned = nullptr;
+    g.pinned_n = 0;
+  }
+  CM_HIP(hipHostMalloc(&g.pinned, bytes, hipHostMallocDefault));
+  g.pinned_n = bytes;
+  return CM_OK;
+}
+
+int init_locked(int device) {
+  if (g.stream && (device < 0 || device == g.device)) return CM_OK;
+  if (device < 0) device = 0;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count == 0)
+    return fail(CM_EDEVICE, "no HIP device is available (compmatmul_amd needs an MI355X)");
+  if (device >= count) return fail(CM_EINVAL, "device index out of range");
+  hipDeviceProp_t prop;
+  CM_HIP(hipGetDeviceProperties(&prop, device));
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+    return fail(CM_EDEVICE, std::string("compmatmul_amd is built for gfx950, found ") +
+                                prop.gcnArchName);
+  CM_HIP(hipSetDevice(device));
+  if (g.stream) CM_HIP(hipStreamDestroy(g.stream));
+  CM_HIP(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
+  g.device = device;
+  return CM_OK;
+}
+
+hipEvent_t get_event() {
+  if (!g.event_pool.empty()) {
+    hipEvent_t e = g.event_pool.back();
+    g.event_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  hipEventCreate(&e);
+  return e;
+}
+
+// launch wrapper: brackets the launch with events when timing is on
+template <class F>
+int timed(const char* name, F&& launch) {
+  hipEvent_t a = nullptr, b = nullptr;
+  if (g.timing) {
+    a = get_event();
+    b = get_event();
+    hipEventRecord(a, g.stream);
+  }
+  const hipError_t e = launch();
+  if (g.timing) {
+    hipEventRecord(b, g.stream);
+    g.pending.push_back({name, a, b});
+  }
+  if (e != hipSuccess) return fail(CM_EDEVICE, std::string(name) + " launch failed: " + hipGetErrorString(e));
+  return CM_OK;
+}
+
+void resolve_timing() {
+  if (g.pending.empty()) return;
+  hipStreamSynchronize(g.stream);
+  for (auto& r : g.pending) {
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, r.a, r.b);
+    auto& t = g.totals[r.name];
+    t.first += ms;
+    t.second += 1;
+    g.event_pool.push_back(r.a);
+    g.event_pool.push_back(r.b);
+  }
+  g.pending.clear();
+}
+
+bool is_pow2(int64_t x) { return x > 0 && (x & (x - 1)) == 0; }
+int ilog2(int64_t x) {
+  int l = 0;
+  while ((int64_t(1) << (l + 1)) <= x) ++l;
+  return l;
+}
+
+int check_db(int d, int b) {
+  if (b < 2 || (b & (b - 1)) != 0)
+    return fail(CM_EINVAL, "b must be power of 2 greater than 1 (got " + std::to_string(b) + ")");
+  if (d < 1) return fail(CM_EINVAL, "d must be positive (got " + std::to_string(d) + ")");
+  return CM_OK;
+}
+
+int check_family(int family) {
+  if (family < 0 || family > 3) return fail(CM_EHASH, "Tried to use an invalid hash");
+  return CM_OK;
+}
+
+// W_b^j = exp(-2 pi i j / b) for j < b, built in FP64 from one octant
+int twiddles(int lb, int f64, const void** out) {
+  auto key = std::make_pair(lb, f64);
+  auto it = g.twiddles.find(key);
+  if (it != g.twiddles.end()) {
+    *out = it->second;
+    return CM_OK;
+  }
+  const int64_t b = int64_t(1) << lb;
+  std::vector<double> re(b), im(b);
+  for (int64_t j = 0; j < b; ++j) {
+    // reduce j to the first octant with exact integer arithmetic
+    const int64_t q = (j * 8) / b;  // octant index 0..7
+    const int64_t r = j - q * b / 8;
+    double c, s;
+    auto cs = [&](int64_t num, double& cc, double& ss) {  // angle 2 pi num / b
+      const double ang = 2.0 * M_PI * double(num) / double(b);
+      cc = std::cos(ang);
+      ss = std::sin(ang);
+    };
+    if (b < 8) {
+      cs(j, c, s);
+    } else {
+      double c0, s0, c1, s1;
+      cs(r, c0, s0);                  // angle in [0, pi/4)
+      cs(b / 8 - r, c1, s1);          // complementary angle
+      switch (q) {                    // cos/sin of q*pi/4 + r-angle
+        case 0: c = c0; s = s0; break;
+        case 1: c = s1; s = c1; break;   // pi/2 - (b/8 - r)
+        case 2: c = -s0; s = c0; break;
+        case 3: c = -c1; s = s1; break;
+        case 4: c = -c0; s = -s0; break;
+        case 5: c = -s1; s = -c1; break;
+        case 6: c = s0; s = -c0; break;
+        default: c = c1; s = -s1; break;
+      }
+    }
+    re[j] = c;
+    im[j] = -s;  // e^{-i angle}
+  }
+  const size_t esz = f64 ? sizeof(double) : sizeof(float);
+  std::vector<unsigned char> host(size_t(b) * 2 * esz);
+  for (int64_t j = 0; j < b; ++j) {
+    if (f64) {
+      reinterpret_cast<double*>(host.data())[2 * j] = re[j];
+      reinterpret_cast<double*>(host.data())[2 * j + 1] = im[j];
+    } else {
+      reinterpret_cast<float*>(host.data())[2 * j] = float(re[j]);
+      reinterpret_cast<float*>(host.data())[2 * j + 1] = float(im[j]);
+    }
+  }
+  void* dev = nullptr;
+  CM_HIP(hipMalloc(&dev, host.size()));
+  CM_HIP(hipMemcpy(dev, host.data(), host.size(), hipMemcpyHostToDevice));
+  g.twiddles[key] = dev;
+  *out = dev;
+  return CM_OK;
+}
+
+// Draw h1, h2, s1, s2 and evaluate the packed arrays on the device.
+int build_hashes(int family, int d, int lb, uint64_t seed, int m, int p, uint32_t* outA,
+                 uint32_t* outB) {
+  cm::HashArgs ha{};
+  ha.family = family;
+  ha.d = d;
+  ha.lb = lb;
+  ha.m = m;
+  ha.p = p;
+  ha.outA = outA;
+  ha.outB = outB;
+  if (family == cm::kMultiplyAddShift || family == cm::kMultiplyShift) {
+    const int w = family == cm::kMultiplyAddShift ? 2 : 1;
+    std::vector<uint64_t> all, one;
+    for (int f = 0; f < 4; ++f) {
+      cm::draw_multiply(family, d, seed + uint64_t(f), one);
+      all.insert(all.end(), one.begin(), one.end());
+    }
+    if (all.size() <= 256) {
+      ha.use_inline = 1;
+      std::copy(all.begin(), all.end(), ha.inline_params);
+    } else {
+      int rc = ensure_buf(g.params, all.size() * 8);
+      if (rc) return rc;
+      rc = ensure_pinned(all.size() * 8);
+      if (rc) return rc;
+      CM_HIP(hipStreamSynchronize(g.stream));
+      std::memcpy(g.pinned, all.data(), all.size() * 8);
+      CM_HIP(hipMemcpyAsync(g.params.p, g.pinned, all.size() * 8, hipMemcpyHostToDevice, g.stream));
+      ha.mul_params = static_cast<const uint64_t*>(g.params.p);
+    }
+    (void)w;
+  } else {
+    const int chars = family == cm::kTab8 ? 4 : 2;
+    const int64_t full = family == cm::kTab8 ? 256 : 65536;
+    const int64_t maxkey = std::max(1, std::max(m, p)) - 1;
+    const int64_t tab_len = family == cm::kTab8 ? 256 : std::min<int64_t>(full, maxkey + 1);
+    const size_t bytes = size_t(4) * d * chars * tab_len * 4;
+    int rc = ensure_buf(g.tables, bytes);
+    if (rc) return rc;
+    rc = ensure_pinned(bytes);
+    if (rc) return rc;
+    CM_HIP(hipStreamSynchronize(g.stream));
+    uint32_t* dst = static_cast<uint32_t*>(g.pinned);
+    std::vector<uint32_t> tabs;
+    for (int f = 0; f < 4; ++f) {
+      cm::draw_tables(family, d, seed + uint64_t(f), tabs);
+      for (int t = 0; t < d; ++t)
+        for (int c = 0; c < chars; ++c) {
+          const uint32_t* src = tabs.data() + (int64_t(t) * chars + c) * full;
+          std::memcpy(dst, src, tab_len * 4);
+          dst += tab_len;
+        }
+    }
+    CM_HIP(hipMemcpyAsync(g.tables.p, g.pinned, bytes, hipMemcpyHostToDevice, g.stream));
+    ha.tables = static_cast<const uint32_t*>(g.tables.p);
+    ha.tab_len = int(tab_len);
+  }
+  return timed("hash", [&] { return cm::launch_hash(ha, g.stream); });
+}
+
+int alloc_sketch(int fwht, int f64, int d, int b, int m, int p, cm_sketch** out) {
+  cm_sketch* sk = new cm_sketch{};
+  sk->fwht = fwht;
+  sk->f64 = f64;
+  sk->d = d;
+  sk->b = b;
+  sk->lb = ilog2(b);
+  sk->m = m;
+  sk->p = p;
+  const size_t esz = f64 ? 8 : 4;
+  hipError_t e1 = hipMalloc(&sk->hA, std::max<size_t>(4, size_t(d) * m * 4));
+  hipError_t e2 = hipMalloc(&sk->hB, std::max<size_t>(4, size_t(d) * p * 4));
+  hipError_t e3 = hipMalloc(&sk->P, size_t(d) * b * esz);
+  if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) {
+    cm_sketch_free(sk);
+    return fail(CM_ENOMEM, "device allocation for the sketch failed");
+  }
+  *out = sk;
+  return CM_OK;
+}
+
+// stage a host matrix (rows x cols, leading dimension ld) into a device buffer
+int stage_in(DevBuf& buf, const void* host, int rows, int cols, int64_t ld, size_t esz,
+             const void** dev) {
+  if (rows == 0 || cols == 0) {
+    *dev = nullptr;
+    return CM_OK;
+  }
+  const size_t elems = size_t(rows - 1) * size_t(ld) + size_t(cols);
+  int rc = ensure_buf(buf, elems * esz);
+  if (rc) return rc;
+  CM_HIP(hipMemcpyAsync(buf.p, host, elems * esz, hipMemcpyHostToDevice, g.stream));
+  *dev = buf.p;
+  return CM_OK;
+}
+
+int compress_device(cm_sketch* sk, const void* AT, int64_t ld_at, const void* B, int64_t ld_b,
+                    int n, unsigned flags) {
+  const int f64 = sk->f64, fwht = sk->fwht, d = sk->d, lb = sk->lb, m = sk->m, p = sk->p;
+  const int64_t b = int64_t(1) << lb;
+  const size_t esz = f64 ? 8 : 4;
+  const size_t cplx = fwht ? 1 : 2;
+  const int max_lb2 = cm::compress_lds_max_lb2(fwht, f64);
+  const void* tw = nullptr;
+  if (!fwht) {
+    int rc = twiddles(lb, f64, &tw);
+    if (rc) return rc;
+  }
+  if (flags & CM_FLAG_PLAIN) {
+    if (lb > max_lb2) return fail(CM_ENOTSUP, "the plain path handles b <= 8192 (float) / 4096 (double)");
+    int rc = ensure_buf(g.plainA, std::max<size_t>(1, size_t(n) * b * cplx * esz));
+    if (rc) return rc;
+    rc = ensure_buf(g.plainB, std::max<size_t>(1, size_t(n) * b * esz));
+    if (rc) return rc;
+    rc = ensure_buf(g.partial, size_t(d) * b * cplx * esz);
+    if (rc) return rc;
+    for (int t = 0; t < d; ++t) {
+      cm::PlainArgs pa{};
+      pa.fwht = fwht;
+      pa.f64 = f64;
+      pa.AT = AT;
+      pa.ld_at = ld_at;
+      pa.B = B;
+      pa.ld_b = ld_b;
+      pa.hA = sk->hA;
+      pa.hB = sk->hB;
+      pa.m = m;
+      pa.n = n;
+      pa.p = p;
+      pa.d = d;
+      pa.lb = lb;
+      pa.t = t;
+      pa.tw = tw;
+      pa.SA = g.plainA.p;
+      pa.SB = g.plainB.p;
+      pa.acc = static_cast<char*>(g.partial.p) + size_t(t) * b * cplx * esz;
+      if ((rc = timed("plain_sketch", [&] { return cm::launch_plain_sketch(pa, g.stream); }))) return rc;
+      if ((rc = timed("plain_transform", [&] { return cm::launch_plain_transform(pa, g.stream); }))) return rc;
+      if ((rc = timed("plain_mac", [&] { return cm::launch_plain_mac(pa, g.stream); }))) return rc;
+    }
+    cm::ReduceArgs ra{fwht, f64, int((flags & CM_FLAG_F64_REDUCE) != 0), g.partial.p, 1, d, lb, lb,
+                      tw, nullptr};
+    rc = ensure_buf(g.V, size_t(d) * b * cplx * esz);
+    if (rc) return rc;
+    ra.V = g.V.p;
+    if ((rc = timed("reduce", [&] { return cm::launch_reduce_inverse(ra, g.stream); }))) return rc;
+    cm::CombineArgs ca{fwht, f64, g.V.p, d, lb, lb, tw, sk->P};
+    return timed("combine", [&] { return cm::launch_combine(ca, g.stream); });
+  }
+
+  const int lb2 = std::min(lb, max_lb2);
+  const int b1 = 1 << (lb - lb2);
+  // aim at ~8 workgroups per CU (256 CUs) over all (t, slice, chunk) units
+  const int64_t per_chunk = int64_t(d) * b1;
+  int64_t nchunks = std::max<int64_t>(1, (2048 + per_chunk - 1) / per_chunk);
+  nchunks = std::min<int64_t>(nchunks, std::max(1, n));
+  const size_t chunk_bytes = size_t(d) * b * cplx * esz;
+  const size_t cap = size_t(1) << 30;  // partial spectra stay under 1 GiB
+  nchunks = std::max<int64_t>(1, std::min<int64_t>(nchunks, int64_t(cap / chunk_bytes)));
+  const int kper = n > 0 ? int((n + nchunks - 1) / nchunks) : 0;
+  nchunks = n > 0 ? (n + kper - 1) / kper : 0;
+  int rc = ensure_buf(g.partial, std::max<size_t>(1, size_t(nchunks) * chunk_bytes));
+  if (rc) return rc;
+  rc = ensure_buf(g.V, chunk_bytes);
+  if (rc) return rc;
+  cm::CompressArgs caa{};
+  caa.fwht = fwht;
+  caa.f64 = f64;
+  caa.AT = AT;
+  caa.ld_at = ld_at;
+  caa.B = B;
+  caa.ld_b = ld_b;
+  caa.hA = sk->hA;
+  caa.hB = sk->hB;
+  caa.m = m;
+  caa.n = n;
+  caa.p = p;
+  caa.d = d;
+  caa.lb = lb;
+  caa.lb2 = lb2;
+  caa.nchunks = int(nchunks);
+  caa.kper = kper;
+  caa.tw = tw;
+  caa.partial = g.partial.p;
+  if ((rc = timed("compress", [&] { return cm::launch_compress_lds(caa, g.stream); }))) return rc;
+  const int f64_reduce = (flags & CM_FLAG_F64_REDUCE) || n > 16384;
+  cm::ReduceArgs ra{fwht, f64, f64_reduce, g.partial.p, int(nchunks), d, lb, lb2, tw, g.V.p};
+  if ((rc = timed("reduce", [&] { return cm::launch_reduce_inverse(ra, g.stream); }))) return rc;
+  cm::CombineArgs ca{fwht, f64, g.V.p, d, lb, lb2, tw, sk->P};
+  return timed("combine", [&] { return cm::launch_combine(ca, g.stream); });
+}
+
+int compress_locked(int variant, int dtype, int m, int n, int p, const void* AT, int64_t ld_at,
+                    const void* B, int64_t ld_b, int d, int b, uint64_t seed, int family,
+                    int mem, unsigned flags, cm_sketch** inout) {
+  int rc = init_locked(-1);
+  if (rc) return rc;
+  if (variant != CM_FFT && variant != CM_FWHT) return fail(CM_EINVAL, "variant must be CM_FFT or CM_FWHT");
+  if (dtype != CM_F64 && dtype != CM_F32) return fail(CM_EINVAL, "dtype must be CM_F64 or CM_F32");
+  if ((rc = check_db(d, b))) return rc;
+  if ((rc = check_family(family))) return rc;
+  if (m < 0 || n < 0 || p < 0) return fail(CM_EINVAL, "matrix dimensions must be non-negative");
+  if (b > (1 << 30)) return fail(CM_ENOTSUP, "b above 2^30 is not supported");
+  if (ld_at < m || ld_b < p) return fail(CM_EINVAL, "leading dimension smaller than the row length");
+  if ((int64_t(n) * m > 0 && !AT) || (int64_t(n) * p > 0 && !B))
+    return fail(CM_EINVAL, "null input pointer");
+  if (mem != CM_HOST && mem != CM_DEVICE) return fail(CM_EINVAL, "mem must be CM_HOST or CM_DEVICE");
+  const int fwht = variant == CM_FWHT, f64 = dtype == CM_F64;
+  cm_sketch* sk = *inout;
+  if (sk && (sk->fwht != fwht || sk->f64 != f64 || sk->d != d || sk->b != b || sk->m != m ||
+             sk->p != p)) {
+    return fail(CM_EINVAL, "sketch handle does not match (variant, dtype, d, b, m, p)");
+  }
+  bool fresh = false;
+  if (!sk) {
+    if ((rc = alloc_sketch(fwht, f64, d, b, m, p, &sk))) return rc;
+    fresh = true;
+  }
+  sk->family = family;
+  sk->seed = seed;
+  const size_t esz = f64 ? 8 : 4;
+  const void* dAT = AT;
+  const void* dB = B;
+  if (mem == CM_HOST) {
+    if ((rc = stage_in(g.AT, AT, n, m, ld_at, esz, &dAT)) ||
+        (rc = stage_in(g.B, B, n, p, ld_b, esz, &dB))) {
+      if (fresh) cm_sketch_free(sk);
+      return rc;
+    }
+  }
+  if ((rc = build_hashes(family, d, sk->lb, seed, m, p, sk->hA, sk->hB)) ||
+      (rc = compress_device(sk, dAT, ld_at, dB, ld_b, n, flags))) {
+    if (fresh) cm_sketch_free(sk);
+    return rc;
+  }
+  *inout = sk;
+  return CM_OK;
+}
+
+int decompress_locked(const cm_sketch* sk, void* C, int64_t ldc, int mem, int recover, double tau,
+                      int64_t max_out) {
+  cm::DecompressArgs da{};
+  da.fwht = sk->fwht;
+  da.f64 = sk->f64;
+  da.P = sk->P;
+  da.hA = sk->hA;
+  da.hB = sk->hB;
+  da.m = sk->m;
+  da.p = sk->p;
+  da.d = sk->d;
+  da.b = sk->b;
+  da.ldc = ldc;
+  const size_t esz = sk->f64 ? 8 : 4;
+  int rc;
+  if (C && mem == CM_HOST) {
+    if ((rc = ensure_buf(g.C, std::max<size_t>(1, size_t(sk->m) * ldc * esz)))) return rc;
+    da.C = g.C.p;
+  } else {
+    da.C = C;
+  }
+  if (recover) {
+    if ((rc = ensure_buf(g.counter, 8)) || (rc = ensure_buf(g.oi, std::max<int64_t>(1, max_out) * 4)) ||
+        (rc = ensure_buf(g.oj, std::max<int64_t>(1, max_out) * 4)) ||
+        (rc = ensure_buf(g.ov, std::max<int64_t>(1, max_out) * esz)))
+      return rc;
+    CM_HIP(hipMemsetAsync(g.counter.p, 0, 8, g.stream));
+    da.recover = 1;
+    da.tau = tau;
+    da.counter = static_cast<unsigned long long*>(g.counter.p);
+    da.max_out = max_out;
+    da.oi = static_cast<int32_t*>(g.oi.p);
+    da.oj = static_cast<int32_t*>(g.oj.p);
+    da.ov = g.ov.p;
+  }
+  if ((rc = timed("decompress", [&] { return cm::launch_decompress(da, g.stream); }))) return rc;
+  if (C && mem == CM_HOST && sk->m > 0) {
+    const size_t bytes = (size_t(sk->m - 1) * ldc + sk->p) * esz;
+    CM_HIP(hipMemcpyAsync(C, g.C.p, bytes, hipMemcpyDeviceToHost, g.stream));
+    CM_HIP(hipStreamSynchronize(g.stream));
+  }
+  return CM_OK;
+}
+
+int matmul_dropin(int variant, int n_a, int n_inner, int n_b, double* C, const double* AT,
+                  const double* B, int d, int b, uint64_t seed, int seed_given, int family) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  int rc = check_db(d, b);
+  if (rc) return rc;
+  if ((rc = check_family(family))) return rc;
+  if (!seed_given) seed = cm::time_seed();
+  cm_sketch* sk = nullptr;
+  rc = compress_locked(variant, CM_F64, n_a, n_inner, n_b, AT, n_a, B, n_b, d, b, seed, family,
+                       CM_HOST, 0, &sk);
+  if (rc) return rc;
+  rc = decompress_locked(sk, C, n_b, CM_HOST, 0, 0.0, 0);
+  cm_sketch_free(sk);
+  return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* cm_last_error(void) { return g_err.c_str(); }
+
+int cm_init(int device) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  return init_locked(device);
+}
+
+uint64_t cm_default_seed(void) { return cm::time_seed(); }
+
+int cm_matmul_fft(int n_a, int n_inner, int n_b, double* C, const double* AT, const double* B,
+                  int d, int b, uint64_t seed, int seed_given, int family) {
+  return matmul_dropin(CM_FFT, n_a, n_inner, n_b, C, AT, B, d, b, seed, seed_given, family);
+}
+
+int cm_matmul_fwht(int n_a, int n_inner, int n_b, double* C, const double* AT, const double* B,
+                   int d, int b, uint64_t seed, int seed_given, int family) {
+  return matmul_dropin(CM_FWHT, n_a, n_inner, n_b, C, AT, B, d, b, seed, seed_given, family);
+}
+
+int cm_gemm(int n_a, int n_inner, int n_b, double* C, const double* AT, const double* B) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  int rc = init_locked(-1);
+  if (rc) return rc;
+  if (n_a < 0 || n_inner < 0 || n_b < 0) return fail(CM_EINVAL, "matrix dimensions must be non-negative");
+  const void *dAT, *dB;
+  if ((rc = stage_in(g.AT, AT, n_inner, n_a, n_a, 8, &dAT)) || (rc = stage_in(g.B, B, n_inner, n_b, n_b, 8, &dB)))
+    return rc;
+  if ((rc = ensure_buf(g.C, std::max<size_t>(8, size_t(n_a) * n_b * 8)))) return rc;
+  if (n_inner == 0) {
+    CM_HIP(hipMemsetAsync(g.C.p, 0, size_t(n_a) * n_b * 8, g.stream));
+  } else if ((rc = timed("gemm", [&] {
+                return cm::launch_gemm_f64(n_a, n_inner, n_b, static_cast<const double*>(dAT),
+                                           static_cast<const double*>(dB),
+                                           static_cast<double*>(g.C.p), g.stream);
+              }))) {
+    return rc;
+  }
+  if (size_t(n_a) * n_b)
+    CM_HIP(hipMemcpyAsync(C, g.C.p, size_t(n_a) * n_b * 8, hipMemcpyDeviceToHost, g.stream));
+  CM_HIP(hipStreamSynchronize(g.stream));
+  return CM_OK;
+}
+
+int cm_hash_arrays(int family, int d, int b, uint64_t seed, int m, int p, uint32_t* outA,
+                   uint32_t* outB, int mem) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  int rc = init_locked(-1);
+  if (rc) return rc;
+  if ((rc = check_db(d, b)) || (rc = check_family(family))) return rc;
+  if (m < 0 || p < 0) return fail(CM_EINVAL, "key counts must be non-negative");
+  uint32_t *dA = outA, *dB = outB;
+  DevBuf tA, tB;
+  if (mem == CM_HOST) {
+    CM_HIP(hipMalloc(&tA.p, std::max<size_t>(4, size_t(d) * m * 4)));
+    CM_HIP(hipMalloc(&tB.p, std::max<size_t>(4, size_t(d) * p * 4)));
+    dA = static_cast<uint32_t*>(tA.p);
+    dB = static_cast<uint32_t*>(tB.p);
+  }
+  rc = build_hashes(family, d, ilog2(b), seed, m, p, dA, dB);
+  if (!rc && mem == CM_HOST) {
+    if (size_t(d) * m) CM_HIP(hipMemcpyAsync(outA, dA, size_t(d) * m * 4, hipMemcpyDeviceToHost, g.stream));
+    if (size_t(d) * p) CM_HIP(hipMemcpyAsync(outB, dB, size_t(d) * p * 4, hipMemcpyDeviceToHost, g.stream));
+  }
+  hipStreamSynchronize(g.stream);
+  if (tA.p) hipFree(tA.p);
+  if (tB.p) hipFree(tB.p);
+  return rc;
+}
+
+int cm_compress(int variant, int dtype, int n_a, int n_inner, int n_b, const void* AT,
+                int64_t ld_at, const void* B, int64_t ld_b, int d, int b, uint64_t seed,
+                int family, int mem, cm_sketch** out) {
+  if (!out) return fail(CM_EINVAL, "null output handle");
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  *out = nullptr;
+  int rc = compress_locked(variant, dtype, n_a, n_inner, n_b, AT, ld_at, B, ld_b, d, b, seed,
+                           family, mem, 0, out);
+  if (!rc && mem == CM_HOST) CM_HIP(hipStreamSynchronize(g.stream));
+  return rc;
+}
+
+int cm_compress_ex(int variant, int dtype, int n_a, int n_inner, int n_b, const void* AT,
+                   int64_t ld_at, const void* B, int64_t ld_b, int d, int b, uint64_t seed,
+                   int family, int mem, unsigned flags, cm_sketch** inout) {
+  if (!inout) return fail(CM_EINVAL, "null handle pointer");
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  int rc = compress_locked(variant, dtype, n_a, n_inner, n_b, AT, ld_at, B, ld_b, d, b, seed,
+                           family, mem, flags, inout);
+  if (!rc && mem == CM_HOST) CM_HIP(hipStreamSynchronize(g.stream));
+  return rc;
+}
+
+int cm_decompress(const cm_sketch* sk, void* C, int64_t ldc, int mem) {
+  if (!sk) return fail(CM_EINVAL, "null sketch");
+  if (!C && int64_t(sk->m) * sk->p > 0) return fail(CM_EINVAL, "null output pointer");
+  if (ldc < sk->p) return fail(CM_EINVAL, "ldc smaller than the number of columns");
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  return decompress_locked(sk, C, ldc, mem, 0, 0.0, 0);
+}
+
+int cm_recover(const cm_sketch* sk, double tau, int64_t max_out, int32_t* out_i, int32_t* out_j,
+               double* out_val, int64_t* count) {
+  if (!sk || !count) return fail(CM_EINVAL, "null argument");
+  if (max_out < 0) return fail(CM_EINVAL, "max_out must be non-negative");
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  int rc = decompress_locked(sk, nullptr, sk->p, CM_DEVICE, 1, tau, max_out);
+  if (rc) return rc;
+  unsigned long long found = 0;
+  CM_HIP(hipMemcpyAsync(&found, g.counter.p, 8, hipMemcpyDeviceToHost, g.stream));
+  CM_HIP(hipStreamSynchronize(g.stream));
+  *count = int64_t(found);
+  const int64_t k = std::min<int64_t>(int64_t(found), max_out);
+  if (k == 0) return CM_OK;
+  std::vector<int32_t> vi(k), vj(k);
+  std::vector<double> vv(k);
+  CM_HIP(hipMemcpy(vi.data(), g.oi.p, k * 4, hipMemcpyDeviceToHost));
+  CM_HIP(hipMemcpy(vj.data(), g.oj.p, k * 4, hipMemcpyDeviceToHost));
+  if (sk->f64) {
+    CM_HIP(hipMemcpy(vv.data(), g.ov.p, k * 8, hipMemcpyDeviceToHost));
+  } else {
+    std::vector<float> vf(k);
+    CM_HIP(hipMemcpy(vf.data(), g.ov.p, k * 4, hipMemcpyDeviceToHost));
+    for (int64_t q = 0; q < k; ++q) vv[q] = vf[q];
+  }
+  std::vector<int64_t> order(k);
+  std::iota(order.begin(), order.end(), 0);
+  std::sort(order.begin(), order.end(), [&](int64_t x, int64_t y) {
+    return vi[x] != vi[y] ? vi[x] < vi[y] : vj[x] < vj[y];
+  });
+  for (int64_t q = 0; q < k; ++q) {
+    if (out_i) out_i[q] = vi[order[q]];
+    if (out_j) out_j[q] = vj[order[q]];
+    if (out_val) out_val[q] = vv[order[q]];
+  }
+  return CM_OK;
+}
+
+int cm_sketch_values(const cm_sketch* sk, double* host_out) {
+  if (!sk || !host_out) return fail(CM_EINVAL, "null argument");
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  const size_t n = size_t(sk->d) * sk->b;
+  CM_HIP(hipStreamSynchronize(g.stream));
+  if (sk->f64) {
+    CM_HIP(hipMemcpy(host_out, sk->P, n * 8, hipMemcpyDeviceToHost));
+  } else {
+    std::vector<float> tmp(n);
+    CM_HIP(hipMemcpy(tmp.data(), sk->P, n * 4, hipMemcpyDeviceToHost));
+    for (size_t q = 0; q < n; ++q) host_out[q] = tmp[q];
+  }
+  return CM_OK;
+}
+
+int cm_sketch_hashes(const cm_sketch* sk, uint32_t* outA, uint32_t* outB) {
+  if (!sk) return fail(CM_EINVAL, "null sketch");
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  CM_HIP(hipStreamSynchronize(g.stream));
+  if (outA && size_t(sk->d) * sk->m)
+    CM_HIP(hipMemcpy(outA, sk->hA, size_t(sk->d) * sk->m * 4, hipMemcpyDeviceToHost));
+  if (outB && size_t(sk->d) * sk->p)
+    CM_HIP(hipMemcpy(outB, sk->hB, size_t(sk->d) * sk->p * 4, hipMemcpyDeviceToHost));
+  return CM_OK;
+}
+
+void cm_sketch_free(cm_sketch* sk) {
+  if (!sk) return;
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  if (g.stream) hipStreamSynchronize(g.stream);
+  if (sk->hA) hipFree(sk->hA);
+  if (sk->hB) hipFree(sk->hB);
+  if (sk->P) hipFree(sk->P);
+  delete sk;
+}
+
+int cm_dev_alloc(size_t bytes, void** ptr) {
+  if (!ptr) return fail(CM_EINVAL, "null pointer");
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  int rc = init_locked(-1);
+  if (rc) return rc;
+  CM_HIP(hipMalloc(ptr, std::max<size_t>(bytes, 1)));
+  return CM_OK;
+}
+
+int cm_dev_free(void* ptr) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  if (g.stream) hipStreamSynchronize(g.stream);
+  if (ptr) CM_HIP(hipFree(ptr));
+  return CM_OK;
+}
+
+int cm_memcpy(void* dst, const void* src, size_t bytes, int dst_mem, int src_mem) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  int rc = init_locked(-1);
+  if (rc) return rc;
+  if (bytes == 0) return CM_OK;
+  hipMemcpyKind kind = dst_mem == CM_DEVICE ? (src_mem == CM_DEVICE ? hipMemcpyDeviceToDevice
+                                                                    : hipMemcpyHostToDevice)
+                                            : (src_mem == CM_DEVICE ? hipMemcpyDeviceToHost
+                                                                    : hipMemcpyHostToHost);
+  CM_HIP(hipMemcpyAsync(dst, src, bytes, kind, g.stream));
+  CM_HIP(hipStreamSynchronize(g.stream));
+  return CM_OK;
+}
+
+int cm_synchronize(void) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  int rc = init_locked(-1);
+  if (rc) return rc;
+  CM_HIP(hipStreamSynchronize(g.stream));
+  return CM_OK;
+}
+
+int cm_timing_enable(int on) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  resolve_timing();
+  g.timing = on != 0;
+  return CM_OK;
+}
+
+int cm_timing_reset(void) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  resolve_timing();
+  g.totals.clear();
+  return CM_OK;
+}
+
+int cm_timing_get(const char* name, double* total_ms, int64_t* launches) {
+  if (!name) return fail(CM_EINVAL, "null name");
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  resolve_timing();
+  auto it = g.totals.find(name);
+  if (total_ms) *total_ms = it == g.totals.end() ? 0.0 : it->second.first;
+  if (launches) *launches = it == g.totals.end() ? 0 : it->second.second;
+  return CM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,314 @@
+// Sliced fused compress, spectral reduction/inverse and cross-slice combine.
+//
+// Algorithm (reference: matmul_fft.hpp:57-122, matmul_fwht.hpp:53-108):
+//   p_t = (1/b) T^-1( sum_k T(S_t a_k) . T(S'_t b_k) )
+// where a_k is row k of AT (column k of A), b_k is row k of B, S_t/S'_t the
+// count-sketch with (h1_t, s1_t) / (h2_t, s2_t), and T the real DFT (cyclic
+// convolution) or the natural-order Walsh-Hadamard transform (XOR
+// convolution).  Differences from the reference's loop structure:
+//
+// * Slicing.  The spectrum of length b = b1*b2 is split into b1 slices of b2
+//   frequencies.  A workgroup owns (t, slice, chunk of k).  Its slice of the
+//   transform is obtained from a "modulated" count-sketch into b2 buckets
+//   followed by a length-b2 transform:
+//     FWHT  u[h mod b2] += (-1)^popc((h >> log2 b2) & slice) s(i) a_i,
+//           X[slice*b2 + f] = WHT_b2(u)[f]
+//     FFT   u[h mod b2] += W_b^(h*slice) s(i) a_i,
+//           X[slice + b1*f] = DFT_b2(u)[f]
+//   so a product with b up to 2^30 runs through on-chip transforms of at
+//   most 8192 points, and the sparse sketch (<= m nonzeros) replaces the
+//   first log2(b1) butterfly stages.
+// * Two real transforms per complex FFT.  z = S a + i S' b has spectrum
+//   Z = A + iB, and A[f]B[f] = (Z[f]^2 - conj(Z[-f])^2)/(4i).  Summed over k
+//   this is linear in Q[f] = sum_k Z_k[f]^2, so only Q is accumulated and the
+//   inverse reduces to p[h] = Im(IDFT_b(Q)[h]) / (2b).
+// * The spectral accumulator lives in registers of the owning workgroup for
+//   its whole chunk of k; chunks are summed afterwards in a fixed order
+//   (reduce kernel), replacing the reference's per-t omp locks.
+#include "device_common.hpp"
+#include "kernels.hpp"
+
+namespace cm {
+
+namespace {
+
+constexpr int NT = 256;
+
+template <typename T>
+struct LdsCfg {
+  static constexpr int LB2MAX = sizeof(T) == 4 ? 13 : 12;  // 64 KB of LDS
+  static constexpr int MAXR = (1 << LB2MAX) / NT;
+};
+
+// in-LDS natural-order (Sylvester) Walsh-Hadamard transform of two vectors
+template <typename T>
+__device__ void lds_wht2(T* sa, T* sb, int b2, int tid) {
+  for (int len = 1; len < b2; len <<= 1) {
+    for (int q = tid; q < (b2 >> 1); q += NT) {
+      const int j = ((q & ~(len - 1)) << 1) | (q & (len - 1));
+      const T x0 = sa[j], x1 = sa[j + len];
+      sa[j] = x0 + x1;
+      sa[j + len] = x0 - x1;
+      if (sb) {
+        const T y0 = sb[j], y1 = sb[j + len];
+        sb[j] = y0 + y1;
+        sb[j + len] = y0 - y1;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// in-LDS radix-2 decimation-in-frequency DFT of length b2 (natural order in,
+// bit-reversed order out).  tw holds W_b^j for j < b; conj selects W^-1.
+template <typename T, bool CONJ>
+__device__ void lds_dif(cpx<T>* z, int b2, int lb, const cpx<T>* __restrict__ tw, int tid) {
+  for (int len = b2 >> 1; len >= 1; len >>= 1) {
+    const int lstride = lb - __ffs(len);  // log2(b / (2 len))
+    for (int q = tid; q < (b2 >> 1); q += NT) {
+      const int pos = q & (len - 1);
+      const int j = ((q & ~(len - 1)) << 1) | pos;
+      const cpx<T> x0 = z[j], x1 = z[j + len];
+      const cpx<T> w = tw[int64_t(pos) << lstride];
+      z[j] = cadd(x0, x1);
+      z[j + len] = CONJ ? cmulc(csub(x0, x1), w) : cmul(csub(x0, x1), w);
+    }
+    __syncthreads();
+  }
+}
+
+template <typename T, bool FWHT>
+__global__ void __launch_bounds__(NT) k_compress_lds(CompressArgs a) {
+  constexpr int MAXR = LdsCfg<T>::MAXR;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int b2 = 1 << a.lb2;
+  const int lbs = a.lb - a.lb2;
+  const int b1 = 1 << lbs;
+  int g = blockIdx.x;
+  const int slice = g & (b1 - 1);
+  g >>= lbs;
+  const int t = g % a.d;
+  const int chunk = g / a.d;
+  const int k0 = chunk * a.kper;
+  const int k1 = min(a.n, k0 + a.kper);
+  const int tid = threadIdx.x;
+  const T* __restrict__ AT = static_cast<const T*>(a.AT);
+  const T* __restrict__ B = static_cast<const T*>(a.B);
+  const uint32_t* __restrict__ hA = a.hA + int64_t(t) * a.m;
+  const uint32_t* __restrict__ hB = a.hB + int64_t(t) * a.p;
+  const int64_t out_off = ((int64_t(chunk) * a.d + t) * b1 + slice) * b2;
+
+  if constexpr (FWHT) {
+    T* sa = reinterpret_cast<T*>(smem_raw);
+    T* sb = sa + b2;
+    T acc[MAXR];
+#pragma unroll
+    for (int r = 0; r < MAXR; ++r) acc[r] = T(0);
+    for (int k = k0; k < k1; ++k) {
+      for (int e = tid; e < b2; e += NT) {
+        sa[e] = T(0);
+        sb[e] = T(0);
+      }
+      __syncthreads();
+      const T* __restrict__ ar = AT + int64_t(k) * a.ld_at;
+      const T* __restrict__ br = B + int64_t(k) * a.ld_b;
+      for (int i = tid; i < a.m; i += NT) {
+        const uint32_t e = hA[i], h = hbucket(e);
+        const uint32_t neg = hneg(e) ^ (__popc((h >> a.lb2) & slice) & 1);
+        atomicAdd(&sa[h & (b2 - 1)], apply_sign(ar[i], neg));
+      }
+      for (int j = tid; j < a.p; j += NT) {
+        const uint32_t e = hB[j], h = hbucket(e);
+        const uint32_t neg = hneg(e) ^ (__popc((h >> a.lb2) & slice) & 1);
+        atomicAdd(&sb[h & (b2 - 1)], apply_sign(br[j], neg));
+      }
+      __syncthreads();
+      lds_wht2(sa, sb, b2, tid);
+#pragma unroll
+      for (int r = 0; r < MAXR; ++r) {
+        const int e = tid + r * NT;
+        if (e < b2) acc[r] += sa[e] * sb[e];
+      }
+      __syncthreads();
+    }
+    T* out = static_cast<T*>(a.partial) + out_off;
+#pragma unroll
+    for (int r = 0; r < MAXR; ++r) {
+      const int e = tid + r * NT;
+      if (e < b2) out[e] = acc[r];
+    }
+  } else {
+    cpx<T>* z = reinterpret_cast<cpx<T>*>(smem_raw);
+    const cpx<T>* __restrict__ tw = static_cast<const cpx<T>*>(a.tw);
+    const uint64_t bmask = (uint64_t(1) << a.lb) - 1;
+    cpx<T> acc[MAXR];
+#pragma unroll
+    for (int r = 0; r < MAXR; ++r) acc[r] = {T(0), T(0)};
+    for (int k = k0; k < k1; ++k) {
+      for (int e = tid; e < b2; e += NT) z[e] = {T(0), T(0)};
+      __syncthreads();
+      const T* __restrict__ ar = AT + int64_t(k) * a.ld_at;
+      const T* __restrict__ br = B + int64_t(k) * a.ld_b;
+      for (int i = tid; i < a.m; i += NT) {
+        const uint32_t e = hA[i], h = hbucket(e);
+        const T v = apply_sign(ar[i], hneg(e));
+        const cpx<T> w = tw[(uint64_t(h) * uint64_t(slice)) & bmask];
+        cpx<T>* dst = &z[h & (b2 - 1)];
+        atomicAdd(&dst->x, v * w.x);
+        atomicAdd(&dst->y, v * w.y);
+      }
+      for (int j = tid; j < a.p; j += NT) {
+        const uint32_t e = hB[j], h = hbucket(e);
+        const T v = apply_sign(br[j], hneg(e));
+        const cpx<T> w = tw[(uint64_t(h) * uint64_t(slice)) & bmask];
+        cpx<T>* dst = &z[h & (b2 - 1)];
+        atomicAdd(&dst->x, -v * w.y);  // i * v * w
+        atomicAdd(&dst->y, v * w.x);
+      }
+      __syncthreads();
+      lds_dif<T, false>(z, b2, a.lb, tw, tid);
+#pragma unroll
+      for (int r = 0; r < MAXR; ++r) {
+        const int e = tid + r * NT;
+        if (e < b2) acc[r] = cadd(acc[r], csqr(z[e]));
+      }
+      __syncthreads();
+    }
+    cpx<T>* out = static_cast<cpx<T>*>(a.partial) + out_off;
+#pragma unroll
+    for (int r = 0; r < MAXR; ++r) {
+      const int e = tid + r * NT;
+      if (e < b2) out[bitrev(e, a.lb2)] = acc[r];
+    }
+  }
+}
+
+template <typename T, typename ACC, bool FWHT>
+__global__ void __launch_bounds__(NT) k_reduce_inverse(ReduceArgs a) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int b2 = 1 << a.lb2;
+  const int lbs = a.lb - a.lb2;
+  const int b1 = 1 << lbs;
+  const int slice = blockIdx.x & (b1 - 1);
+  const int t = blockIdx.x >> lbs;
+  const int tid = threadIdx.x;
+  const int64_t chunk_stride = int64_t(a.d) * b1 * b2;
+  const int64_t off = (int64_t(t) * b1 + slice) * b2;
+  if constexpr (FWHT) {
+    T* s = reinterpret_cast<T*>(smem_raw);
+    const T* __restrict__ P = static_cast<const T*>(a.partial);
+    for (int e = tid; e < b2; e += NT) {
+      ACC sum = ACC(0);
+      for (int c = 0; c < a.nchunks; ++c) sum += ACC(P[c * chunk_stride + off + e]);
+      s[e] = T(sum);
+    }
+    __syncthreads();
+    lds_wht2<T>(s, nullptr, b2, tid);
+    T* V = static_cast<T*>(a.V) + off;
+    for (int e = tid; e < b2; e += NT) V[e] = s[e];
+  } else {
+    cpx<T>* z = reinterpret_cast<cpx<T>*>(smem_raw);
+    const cpx<T>* __restrict__ P = static_cast<const cpx<T>*>(a.partial);
+    const cpx<T>* __restrict__ tw = static_cast<const cpx<T>*>(a.tw);
+    for (int e = tid; e < b2; e += NT) {
+      ACC sx = ACC(0), sy = ACC(0);
+      for (int c = 0; c < a.nchunks; ++c) {
+        const cpx<T> v = P[c * chunk_stride + off + e];
+        sx += ACC(v.x);
+        sy += ACC(v.y);
+      }
+      z[e] = {T(sx), T(sy)};
+    }
+    __syncthreads();
+    lds_dif<T, true>(z, b2, a.lb, tw, tid);
+    cpx<T>* V = static_cast<cpx<T>*>(a.V) + off;
+    for (int e = tid; e < b2; e += NT) V[bitrev(e, a.lb2)] = z[e];
+  }
+}
+
+// p[t][h] from the per-slice inverse transforms V[t][slice][h mod b2]
+template <typename T, bool FWHT>
+__global__ void __launch_bounds__(NT) k_combine(CombineArgs a) {
+  const int64_t b = int64_t(1) << a.lb;
+  const int64_t idx = int64_t(blockIdx.x) * NT + threadIdx.x;
+  if (idx >= int64_t(a.d) * b) return;
+  const int t = int(idx >> a.lb);
+  const uint32_t h = uint32_t(idx & (b - 1));
+  const int b2 = 1 << a.lb2;
+  const int b1 = 1 << (a.lb - a.lb2);
+  const uint32_t lo = h & uint32_t(b2 - 1), hi = h >> a.lb2;
+  T* P = static_cast<T*>(a.P);
+  if constexpr (FWHT) {
+    const T* V = static_cast<const T*>(a.V) + int64_t(t) * b;
+    T s = T(0);
+    for (int sl = 0; sl < b1; ++sl) {
+      const T v = V[int64_t(sl) * b2 + lo];
+      s += (__popc(uint32_t(sl) & hi) & 1) ? -v : v;
+    }
+    P[idx] = s * (T(1) / T(b));
+  } else {
+    const cpx<T>* V = static_cast<const cpx<T>*>(a.V) + int64_t(t) * b;
+    const cpx<T>* tw = static_cast<const cpx<T>*>(a.tw);
+    T im = T(0);
+    for (int sl = 0; sl < b1; ++sl) {
+      const cpx<T> v = V[int64_t(sl) * b2 + lo];
+      const cpx<T> w = tw[(uint64_t(sl) * h) & uint64_t(b - 1)];
+      im += v.y * w.x - v.x * w.y;  // Im(v * conj(w))
+    }
+    P[idx] = im * (T(1) / T(2 * b));
+  }
+}
+
+template <typename T, bool FWHT>
+hipError_t compress_lds_t(const CompressArgs& a, hipStream_t s) {
+  const int b1 = 1 << (a.lb - a.lb2);
+  const int64_t grid = int64_t(a.nchunks) * a.d * b1;
+  const size_t lds = FWHT ? 2 * sizeof(T) * (size_t(1) << a.lb2)
+                          : 2 * sizeof(T) * (size_t(1) << a.lb2);
+  k_compress_lds<T, FWHT><<<dim3(unsigned(grid)), dim3(NT), lds, s>>>(a);
+  return hipGetLastError();
+}
+
+template <typename T, bool FWHT>
+hipError_t reduce_t(const ReduceArgs& a, hipStream_t s) {
+  const int b1 = 1 << (a.lb - a.lb2);
+  const size_t lds = (FWHT ? 1 : 2) * sizeof(T) * (size_t(1) << a.lb2);
+  const unsigned grid = unsigned(a.d * b1);
+  if (a.f64_reduce || sizeof(T) == 8)
+    k_reduce_inverse<T, double, FWHT><<<dim3(grid), dim3(NT), lds, s>>>(a);
+  else
+    k_reduce_inverse<T, T, FWHT><<<dim3(grid), dim3(NT), lds, s>>>(a);
+  return hipGetLastError();
+}
+
+template <typename T, bool FWHT>
+hipError_t combine_t(const CombineArgs& a, hipStream_t s) {
+  const int64_t total = int64_t(a.d) << a.lb;
+  k_combine<T, FWHT><<<dim3(unsigned((total + NT - 1) / NT)), dim3(NT), 0, s>>>(a);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+int compress_lds_max_lb2(int fwht, int f64) {
+  (void)fwht;
+  return f64 ? LdsCfg<double>::LB2MAX : LdsCfg<float>::LB2MAX;
+}
+
+hipError_t launch_compress_lds(const CompressArgs& a, hipStream_t s) {
+  if (a.nchunks == 0) return hipSuccess;
+  if (a.f64) return a.fwht ? compress_lds_t<double, true>(a, s) : compress_lds_t<double, false>(a, s);
+  return a.fwht ? compress_lds_t<float, true>(a, s) : compress_lds_t<float, false>(a, s);
+}
+
+hipError_t launch_reduce_inverse(const ReduceArgs& a, hipStream_t s) {
+  if (a.f64) return a.fwht ? reduce_t<double, true>(a, s) : reduce_t<double, false>(a, s);
+  return a.fwht ? reduce_t<float, true>(a, s) : reduce_t<float, false>(a, s);
+}
+
+hipError_t launch_combine(const CombineArgs& a, hipStream_t s) {
+  if (a.f64) return a.fwht ? combine_t<double, true>(a, s) : combine_t<double, false>(a, s);
+  return a.fwht ? combine_t<float, true>(a, s) : combine_t<float, false>(a, s);
+}
+
+}  // namespace cm

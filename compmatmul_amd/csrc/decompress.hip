@@ -1,0 +1,140 @@
+// Median-of-d decompression and large-entry recovery.
+//
+// Reference: matmul_decompress_fft (matmul_fft.hpp:135-147) and
+// matmul_decompress_fwht (matmul_fwht.hpp:122-134) evaluated for every
+// (i, j) (matmul_fft.hpp:165-177), with array::median = nth_element at d/2
+// (array.hpp:323-328; the upper median for even d).
+//   Chat[i,j] = median_t( s1_t(i) s2_t(j) p[t, h] ),
+//   h = (h1_t(i) + h2_t(j)) & (b-1)  (FFT)   or   h1_t(i) ^ h2_t(j)  (FWHT).
+// One thread owns a column j for a group of rows: the d packed hash entries
+// of column j stay in registers, the row entries are wave-uniform loads,
+// the sketch (d x b) is gathered from L2, and the median comes from a
+// register sorting network (d <= 16) or a rank count (larger d).
+// Recovery appends (i, j, value) for every estimate > tau through a global
+// counter; the host sorts the list by (i, j).
+#include "device_common.hpp"
+#include "kernels.hpp"
+
+namespace cm {
+
+namespace {
+
+constexpr int NT = 256;
+constexpr int ROWS = 8;
+
+__device__ __forceinline__ float vmin(float a, float b) { return fminf(a, b); }
+__device__ __forceinline__ float vmax(float a, float b) { return fmaxf(a, b); }
+__device__ __forceinline__ double vmin(double a, double b) { return fmin(a, b); }
+__device__ __forceinline__ double vmax(double a, double b) { return fmax(a, b); }
+
+// Odd-even transposition sort: D rounds sort any D values.
+template <typename T, int D>
+__device__ __forceinline__ T median_network(T (&v)[D]) {
+#pragma unroll
+  for (int r = 0; r < D; ++r) {
+#pragma unroll
+    for (int j = r & 1; j + 1 < D; j += 2) {
+      const T lo = vmin(v[j], v[j + 1]);
+      const T hi = vmax(v[j], v[j + 1]);
+      v[j] = lo;
+      v[j + 1] = hi;
+    }
+  }
+  return v[D / 2];
+}
+
+template <typename T, bool FWHT>
+__device__ __forceinline__ T gather(const T* __restrict__ P, uint32_t ea, uint32_t eb,
+                                    uint32_t mask, int64_t toff) {
+  const uint32_t h = FWHT ? (hbucket(ea) ^ hbucket(eb)) : ((hbucket(ea) + hbucket(eb)) & mask);
+  const T v = P[toff + h];
+  return (hneg(ea) ^ hneg(eb)) ? -v : v;
+}
+
+template <typename T>
+__device__ __forceinline__ void emit(const DecompressArgs& a, int i, int j, T med) {
+  if (a.C) static_cast<T*>(a.C)[int64_t(i) * a.ldc + j] = med;
+  if (a.recover && double(med) > a.tau) {
+    const unsigned long long slot = atomicAdd(a.counter, 1ull);
+    if (int64_t(slot) < a.max_out) {
+      a.oi[slot] = i;
+      a.oj[slot] = j;
+      static_cast<T*>(a.ov)[slot] = med;
+    }
+  }
+}
+
+template <typename T, bool FWHT, int D>
+__global__ void __launch_bounds__(NT) k_decompress(DecompressArgs a) {
+  const int tiles = (a.p + NT - 1) / NT;
+  const int j = int(blockIdx.x % tiles) * NT + threadIdx.x;
+  const int i0 = int(blockIdx.x / tiles) * ROWS;
+  if (j >= a.p) return;
+  const T* __restrict__ P = static_cast<const T*>(a.P);
+  const uint32_t mask = uint32_t(a.b - 1);
+  uint32_t eb[D];
+#pragma unroll
+  for (int t = 0; t < D; ++t) eb[t] = a.hB[int64_t(t) * a.p + j];
+  const int i1 = min(a.m, i0 + ROWS);
+  for (int i = i0; i < i1; ++i) {
+    T v[D];
+#pragma unroll
+    for (int t = 0; t < D; ++t)
+      v[t] = gather<T, FWHT>(P, a.hA[int64_t(t) * a.m + i], eb[t], mask, int64_t(t) * a.b);
+    emit<T>(a, i, j, median_network<T, D>(v));
+  }
+}
+
+// any d: the value of rank d/2 by counting, O(d^2) gathers per entry
+template <typename T, bool FWHT>
+__global__ void __launch_bounds__(NT) k_decompress_any(DecompressArgs a) {
+  const int tiles = (a.p + NT - 1) / NT;
+  const int j = int(blockIdx.x % tiles) * NT + threadIdx.x;
+  const int i0 = int(blockIdx.x / tiles) * ROWS;
+  if (j >= a.p) return;
+  const T* __restrict__ P = static_cast<const T*>(a.P);
+  const uint32_t mask = uint32_t(a.b - 1);
+  const int r = a.d / 2;
+  const int i1 = min(a.m, i0 + ROWS);
+  for (int i = i0; i < i1; ++i) {
+    T med = T(0);
+    for (int u = 0; u < a.d; ++u) {
+      const T vu = gather<T, FWHT>(P, a.hA[int64_t(u) * a.m + i], a.hB[int64_t(u) * a.p + j],
+                                   mask, int64_t(u) * a.b);
+      int less = 0, leq = 0;
+      for (int t = 0; t < a.d; ++t) {
+        const T vt = gather<T, FWHT>(P, a.hA[int64_t(t) * a.m + i], a.hB[int64_t(t) * a.p + j],
+                                     mask, int64_t(t) * a.b);
+        less += vt < vu;
+        leq += vt <= vu;
+      }
+      if (less <= r && r < leq) med = vu;
+    }
+    emit<T>(a, i, j, med);
+  }
+}
+
+template <typename T, bool FWHT>
+hipError_t launch_t(const DecompressArgs& a, hipStream_t s) {
+  const dim3 grid(unsigned(int64_t((a.p + NT - 1) / NT) * ((a.m + ROWS - 1) / ROWS)));
+  const dim3 block(NT);
+  switch (a.d) {
+#define CM_D(N) \
+  case N: k_decompress<T, FWHT, N><<<grid, block, 0, s>>>(a); break;
+    CM_D(1) CM_D(2) CM_D(3) CM_D(4) CM_D(5) CM_D(6) CM_D(7) CM_D(8)
+    CM_D(9) CM_D(10) CM_D(11) CM_D(12) CM_D(13) CM_D(14) CM_D(15) CM_D(16)
+#undef CM_D
+    default: k_decompress_any<T, FWHT><<<grid, block, 0, s>>>(a); break;
+  }
+  return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_decompress(const DecompressArgs& a, hipStream_t s) {
+  if (a.m == 0 || a.p == 0) return hipSuccess;
+  if (a.f64) return a.fwht ? launch_t<double, true>(a, s) : launch_t<double, false>(a, s);
+  return a.fwht ? launch_t<float, true>(a, s) : launch_t<float, false>(a, s);
+}
+
+}  // namespace cm

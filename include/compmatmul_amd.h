@@ -1,0 +1,159 @@
+/*
+ * compmatmul_amd -- C-ABI of the MI355X-native compressed matrix product.
+ *
+ * Plain pointers and sizes only; no exceptions cross this boundary.  Every
+ * function returns CM_OK (0) or a negative status, and cm_last_error()
+ * gives the message of the last failure on the calling thread.
+ *
+ * Notation follows the reference: m = n_a (rows of C), n = n_inner,
+ * p = n_b (columns of C).  AT is n x m row-major (A passed transposed),
+ * B is n x p row-major, C is m x p row-major (compmatmul.cpp:55-63).
+ * The library runs everything on one HIP stream per device; calls are
+ * synchronous with respect to the host unless stated otherwise.
+ */
+#ifndef COMPMATMUL_AMD_H
+#define COMPMATMUL_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define CM_OK 0
+#define CM_EINVAL (-1)    /* invalid argument (ValueError at the Python layer) */
+#define CM_ENOMEM (-2)    /* device or host allocation failed (MemoryError) */
+#define CM_EDEVICE (-3)   /* HIP runtime error or no usable gfx950 device */
+#define CM_ENOTSUP (-4)   /* argument combination not supported */
+#define CM_EHASH (-5)     /* unknown hash family (RuntimeError) */
+
+/* hash families; the numbering is this library's own */
+#define CM_MULTIPLY_ADD_SHIFT 0 /* hashing.hpp:111-183, the default */
+#define CM_MULTIPLY_SHIFT 1     /* hashing.hpp:27-102 */
+#define CM_TABULATION8 2        /* hashing.hpp:232-320, 8-bit characters */
+#define CM_TABULATION16 3       /* hashing.hpp:232-320, 16-bit characters */
+
+/* transform variant */
+#define CM_FFT 0  /* cyclic convolution, matmul_fft.hpp */
+#define CM_FWHT 1 /* XOR convolution, matmul_fwht.hpp */
+
+/* element type of AT, B, C and of the sketch arithmetic */
+#define CM_F64 0
+#define CM_F32 1
+
+/* where a pointer lives */
+#define CM_HOST 0
+#define CM_DEVICE 1
+
+/* flags for cm_compress_ex */
+#define CM_FLAG_F64_REDUCE 1u /* FP32 path: sum per-tile spectra in FP64 */
+#define CM_FLAG_PLAIN 2u      /* unfused stage-by-stage path (sketches and
+                                 spectra materialised in HBM) */
+
+typedef struct cm_sketch cm_sketch; /* device-resident compressed product */
+
+/* Message of the last failing call on this thread ("" if none). */
+const char* cm_last_error(void);
+
+/* Select the HIP device used by later calls (default 0) and create its
+ * stream.  Optional: every entry point initialises device 0 on first use. */
+int cm_init(int device);
+
+/* Seed used when the caller gives none: the first xoshiro256** output of a
+ * generator seeded with the wall-clock milliseconds.
+ * Replaces: matmul_fft.hpp:30-32 / matmul_fwht.hpp:30-32 (rng.hpp:40-57). */
+uint64_t cm_default_seed(void);
+
+/* ---------------------------------------------------------------- drop-in
+ * C = A^T B estimated by the compressed product, FP64, host pointers,
+ * synchronous; C is fully overwritten.  seed_given = 0 draws cm_default_seed().
+ * Replaces: compmatmul::matmul_fft<Hash>(n_a, n_inner, n_b, C, AT, B, d, b,
+ * seed) (matmul_fft.hpp:155-178), dispatched by compmatmul.cpp:107-146. */
+int cm_matmul_fft(int n_a, int n_inner, int n_b, double* C, const double* AT,
+                  const double* B, int d, int b, uint64_t seed, int seed_given,
+                  int family);
+
+/* Replaces: compmatmul::matmul_fwht<Hash> (matmul_fwht.hpp:144-167),
+ * dispatched by compmatmul.cpp:149-185. */
+int cm_matmul_fwht(int n_a, int n_inner, int n_b, double* C, const double* AT,
+                   const double* B, int d, int b, uint64_t seed, int seed_given,
+                   int family);
+
+/* Exact FP64 product C = AT^T B on the device, host pointers.
+ * Replaces: compmatmul::array::matmul_gemm<double> (array.hpp:81-95) behind
+ * compmatmul.cpp:76-85. */
+int cm_gemm(int n_a, int n_inner, int n_b, double* C, const double* AT,
+            const double* B);
+
+/* ---------------------------------------------------------------- hashing
+ * Packed bucket/sign arrays of a sketch: outA[t*m+i] = h1_t(i) | signbit,
+ * outB[t*p+j] = h2_t(j) | signbit, where signbit = 1u<<31 when s1_t(i)
+ * (resp. s2_t(j)) is -1.  Functions h1,h2,s1,s2 are drawn from seed,
+ * seed+1, seed+2, seed+3.  Evaluated on the device; `mem` says where
+ * outA/outB live.
+ * Replaces: matmul_sketch<Hash>(d, b, seed) (matmul_sketch.hpp:11-20) and
+ * the per-key Hash::operator()/sign (hashing.hpp:91-97, 163-169, 298-314). */
+int cm_hash_arrays(int family, int d, int b, uint64_t seed, int m, int p,
+                   uint32_t* outA, uint32_t* outB, int mem);
+
+/* ---------------------------------------------------------------- sketches
+ * Compressed product of AT (n x m, leading dimension ld_at) and B (n x p,
+ * leading dimension ld_b) into a device-resident sketch of d x b values.
+ * dtype CM_F64 or CM_F32 (inputs and arithmetic); mem CM_HOST or CM_DEVICE.
+ * *out receives a new handle (free with cm_sketch_free).
+ * Replaces: matmul_compressed_product_fft<Hash> (matmul_fft.hpp:25-129) and
+ * matmul_compressed_product_fwht<Hash> (matmul_fwht.hpp:25-114). */
+int cm_compress(int variant, int dtype, int n_a, int n_inner, int n_b,
+                const void* AT, int64_t ld_at, const void* B, int64_t ld_b,
+                int d, int b, uint64_t seed, int family, int mem, cm_sketch** out);
+
+/* As cm_compress, reusing an existing handle when its (variant, dtype, d,
+ * b, m, p) match (no device allocation after the first call), with flags. */
+int cm_compress_ex(int variant, int dtype, int n_a, int n_inner, int n_b,
+                   const void* AT, int64_t ld_at, const void* B, int64_t ld_b,
+                   int d, int b, uint64_t seed, int family, int mem,
+                   unsigned flags, cm_sketch** inout);
+
+/* Full median-of-d decompression into C (m x p, leading dimension ldc, the
+ * sketch's dtype).  Replaces: the decompress loops of matmul_fft.hpp:165-177
+ * and matmul_fwht.hpp:154-166 (matmul_decompress_fft/_fwht, array::median). */
+int cm_decompress(const cm_sketch* sk, void* C, int64_t ldc, int mem);
+
+/* Large-entry recovery: every (i, j) whose median estimate is > tau, sorted
+ * by (i, j), written to host arrays of capacity max_out; *count receives the
+ * total number found (which may exceed max_out).  No counterpart function in
+ * the reference, which leaves thresholding of the full C to the caller. */
+int cm_recover(const cm_sketch* sk, double tau, int64_t max_out, int32_t* out_i,
+               int32_t* out_j, double* out_val, int64_t* count);
+
+/* Copy the d x b sketch values (as FP64) to host memory.
+ * Replaces: reading matmul_sketch<Hash>::p (matmul_sketch.hpp:80). */
+int cm_sketch_values(const cm_sketch* sk, double* host_out);
+
+/* Copy the packed hash arrays of a sketch to host memory (d*m and d*p). */
+int cm_sketch_hashes(const cm_sketch* sk, uint32_t* outA, uint32_t* outB);
+
+void cm_sketch_free(cm_sketch* sk);
+
+/* ---------------------------------------------------------------- memory
+ * Device buffers for callers that keep inputs resident in HBM. */
+int cm_dev_alloc(size_t bytes, void** ptr);
+int cm_dev_free(void* ptr);
+int cm_memcpy(void* dst, const void* src, size_t bytes, int dst_mem, int src_mem);
+int cm_synchronize(void);
+
+/* ---------------------------------------------------------------- timing
+ * When enabled, the library brackets every kernel launch with HIP events on
+ * its own stream; cm_timing_get returns the summed device time (ms) and the
+ * launch count of the kernel class `name` ("compress", "reduce", "combine",
+ * "decompress", "hash", "plain_sketch", "plain_transform", "plain_mac"). */
+int cm_timing_enable(int on);
+int cm_timing_reset(void);
+int cm_timing_get(const char* name, double* total_ms, int64_t* launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* COMPMATMUL_AMD_H */
