@@ -349,7 +349,8 @@ int compress_device(cm_sketch* sk, const void* AT, int64_t ld_at, const void* B,
     if (rc) return rc;
   }
   if (flags & CM_FLAG_PLAIN) {
-    if (lb > max_lb2) return fail(CM_ENOTSUP, "the plain path handles b <= 8192 (float) / 4096 (double)");
+    if (lb > max_lb2 + (fwht ? 1 : 0))
+      return fail(CM_ENOTSUP, "b too large for the plain path (64 KB of LDS per transform)");
     int rc = ensure_buf(g.plainA, std::max<size_t>(1, size_t(n) * b * cplx * esz));
     if (rc) return rc;
     rc = ensure_buf(g.plainB, std::max<size_t>(1, size_t(n) * b * esz));

@@ -7,7 +7,8 @@
 // HBM and is the per-product baseline the fused path is measured against.
 // It uses the same pair packing (z = S a + i S' b, accumulate z^2) and the
 // same reduce/combine kernels as the fused path, so the two differ only in
-// fusion.  It handles b <= 8192 (one on-chip transform per vector).
+// fusion.  It handles transforms that fit 64 KB of LDS: b <= 2^14 (FWHT, float),
+// 2^13 (FWHT double, FFT float), 2^12 (FFT double).
 //
 // cm::launch_gemm_f64 is the exact FP64 product behind compmatmul.gemm
 // (array.hpp:81-95: C = AT^T B, all row-major).
@@ -31,19 +32,18 @@ __global__ void __launch_bounds__(NT) k_plain_sketch(PlainArgs a) {
   const uint32_t* __restrict__ hA = a.hA + int64_t(a.t) * a.m;
   const uint32_t* __restrict__ hB = a.hB + int64_t(a.t) * a.p;
   if constexpr (FWHT) {
-    T* sa = reinterpret_cast<T*>(smem_raw);
-    T* sb = sa + b;
-    for (int e = tid; e < b; e += NT) sa[e] = sb[e] = T(0);
+    // blockIdx.y selects the A side (0) or the B side (1)
+    T* s = reinterpret_cast<T*>(smem_raw);
+    const bool side_b = blockIdx.y != 0;
+    const int keys = side_b ? a.p : a.m;
+    const T* __restrict__ row = side_b ? br : ar;
+    const uint32_t* __restrict__ hh = side_b ? hB : hA;
+    for (int e = tid; e < b; e += NT) s[e] = T(0);
     __syncthreads();
-    for (int i = tid; i < a.m; i += NT) atomicAdd(&sa[hbucket(hA[i])], apply_sign(ar[i], hneg(hA[i])));
-    for (int j = tid; j < a.p; j += NT) atomicAdd(&sb[hbucket(hB[j])], apply_sign(br[j], hneg(hB[j])));
+    for (int i = tid; i < keys; i += NT) atomicAdd(&s[hbucket(hh[i])], apply_sign(row[i], hneg(hh[i])));
     __syncthreads();
-    T* SA = static_cast<T*>(a.SA) + int64_t(k) * b;
-    T* SB = static_cast<T*>(a.SB) + int64_t(k) * b;
-    for (int e = tid; e < b; e += NT) {
-      SA[e] = sa[e];
-      SB[e] = sb[e];
-    }
+    T* S = static_cast<T*>(side_b ? a.SB : a.SA) + int64_t(k) * b;
+    for (int e = tid; e < b; e += NT) S[e] = s[e];
   } else {
     cpx<T>* z = reinterpret_cast<cpx<T>*>(smem_raw);
     for (int e = tid; e < b; e += NT) z[e] = {T(0), T(0)};
@@ -121,8 +121,8 @@ __global__ void __launch_bounds__(NT) k_plain_mac(PlainArgs a) {
 
 template <typename T, bool FWHT>
 hipError_t plain_sketch_t(const PlainArgs& a, hipStream_t s) {
-  const size_t lds = 2 * sizeof(T) * (size_t(1) << a.lb);
-  k_plain_sketch<T, FWHT><<<dim3(unsigned(a.n)), dim3(NT), lds, s>>>(a);
+  const size_t lds = (FWHT ? 1 : 2) * sizeof(T) * (size_t(1) << a.lb);
+  k_plain_sketch<T, FWHT><<<dim3(unsigned(a.n), FWHT ? 2 : 1), dim3(NT), lds, s>>>(a);
   return hipGetLastError();
 }
 

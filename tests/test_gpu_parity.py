@@ -1,0 +1,177 @@
+"""Parity of the HIP path with the CPU oracle, through the C-ABI.
+
+Bars: hash arrays and recovered sets bit-exact; FP64 sketches within 1e-10
+relative (different summation order only) and exact on integer FWHT inputs;
+FP32 sketches within 1e-5 relative L2 error of the oracle's FP64 sketch;
+decompression of a given sketch bit-exact (selection of gathered values).
+"""
+import numpy as np
+import pytest
+
+import compmatmul_amd as cma
+from compmatmul_amd import workloads
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+FAMS = [None, "multiply shift", "tabulation8", "tabulation16"]
+
+
+def rel_l2(x, ref):
+    return float(np.linalg.norm(x - ref) / max(np.linalg.norm(ref), 1e-300))
+
+
+# ---------------------------------------------------------------- hashing
+@pytest.mark.parametrize("hash", FAMS)
+@pytest.mark.parametrize("d,b,seed,m,p", [(3, 2, 5, 7, 9), (9, 4096, 13794, 512, 512),
+                                          (15, 1 << 14, 13794, 2048, 2048),
+                                          (5, 1 << 30, 99, 300, 70000)])
+def test_hash_arrays_bit_exact(hash, d, b, seed, m, p):
+    print(f"hash={hash} d={d} b={b} m={m} p={p}", flush=True)
+    fam = orc.FAMILIES[hash]
+    hA, hB = cma.hash_arrays(hash, d, b, seed, m, p)
+    oA, oB = orc.hash_arrays(fam, d, b, seed, m, p)
+    assert np.array_equal(hA, oA)
+    assert np.array_equal(hB, oB)
+
+
+# ---------------------------------------------------------------- sketches
+@pytest.mark.parametrize("variant", ["fft", "fwht"])
+@pytest.mark.parametrize("hash", FAMS)
+@pytest.mark.parametrize("m,n,p,d,b", [(37, 19, 23, 3, 64), (64, 40, 50, 5, 1 << 13),
+                                       (100, 33, 80, 2, 1 << 15), (5, 0, 4, 3, 32),
+                                       (1, 300, 1, 1, 2)])
+def test_sketch_f64_matches_oracle(variant, hash, m, n, p, d, b):
+    print(f"{variant} {hash} m={m} n={n} p={p} d={d} b={b}", flush=True)
+    rng = np.random.default_rng(m * 7 + n)
+    AT = rng.standard_normal((n, m))
+    B = rng.standard_normal((n, p))
+    sk = cma.compress(variant, AT, B, d, b, seed=4242, hash=hash)
+    got = sk.values()
+    hA, hB = orc.hash_arrays(orc.FAMILIES[hash], d, b, 4242, m, p)
+    assert np.array_equal(sk.hashes()[0], hA) and np.array_equal(sk.hashes()[1], hB)
+    want = orc.sketch_from_product(variant, AT.T @ B, hA, hB, b)
+    scale = max(np.abs(want).max(), 1.0)
+    assert np.abs(got - want).max() <= 1e-10 * scale
+    # decompression of this sketch is a selection: bit-exact vs the oracle's
+    C = sk.decompress()
+    assert np.array_equal(C, orc.decompress(variant, got, hA, hB))
+    sk.free()
+
+
+@pytest.mark.parametrize("variant", ["fft", "fwht"])
+def test_sketch_f64_exact_on_integers(variant):
+    rng = np.random.default_rng(11)
+    AT = rng.integers(-9, 10, (40, 30)).astype(float)
+    B = rng.integers(-9, 10, (40, 20)).astype(float)
+    sk = cma.compress(variant, AT, B, 4, 1 << 14, seed=7)
+    hA, hB = sk.hashes()
+    want = orc.sketch_from_product(variant, AT.T @ B, hA, hB, 1 << 14)
+    got = sk.values()
+    if variant == "fwht":
+        assert np.array_equal(got, want)
+    else:
+        assert np.abs(got - np.round(got)).max() < 1e-9
+        assert np.array_equal(np.round(got), want)
+
+
+@pytest.mark.parametrize("variant", ["fft", "fwht"])
+@pytest.mark.parametrize("b", [1 << 10, 1 << 14, 1 << 16])
+def test_sketch_f32_relative_error(variant, b):
+    m, n, p, d = 256, 512, 192, 5
+    AT, B, _ = workloads.planted(m, n, p, 16, 5.0, 77, np.float32)
+    sk = cma.compress(variant, AT, B, d, b, seed=13794, dtype=np.float32)
+    hA, hB = sk.hashes()
+    want = orc.sketch_from_product(variant, AT.astype(np.float64).T @ B.astype(np.float64),
+                                   hA, hB, b)
+    err = rel_l2(sk.values(), want)
+    print(f"{variant} b={b} fp32 sketch rel err {err:.3e}")
+    assert err <= 1e-5
+    sk.free()
+
+
+@pytest.mark.parametrize("variant", ["fft", "fwht"])
+def test_plain_path_matches_fused(variant):
+    AT, B, _ = workloads.planted(200, 300, 150, 8, 5.0, 5, np.float32)
+    fused = cma.compress(variant, AT, B, 5, 4096, seed=1, dtype=np.float32)
+    plain = cma.compress(variant, AT, B, 5, 4096, seed=1, dtype=np.float32,
+                         flags=cma.FLAG_PLAIN)
+    v1, v2 = fused.values(), plain.values()
+    assert rel_l2(v2, v1) <= 2e-6
+
+
+# ---------------------------------------------------------------- decompress
+@pytest.mark.parametrize("d", [1, 2, 4, 7, 16, 17, 20])
+def test_median_rank_any_d(d):
+    # array.hpp:323-328: nth_element at d/2 (upper median for even d)
+    rng = np.random.default_rng(d)
+    AT = rng.standard_normal((20, 33))
+    B = rng.standard_normal((20, 29))
+    for variant in ("fft", "fwht"):
+        sk = cma.compress(variant, AT, B, d, 64, seed=d)
+        vals = sk.values()
+        hA, hB = sk.hashes()
+        assert np.array_equal(sk.decompress(), orc.decompress(variant, vals, hA, hB))
+
+
+def test_decompress_into_device_buffer():
+    rng = np.random.default_rng(1)
+    AT = rng.standard_normal((30, 40)).astype(np.float32)
+    B = rng.standard_normal((30, 50)).astype(np.float32)
+    dAT, dB = cma.DeviceArray.from_numpy(AT), cma.DeviceArray.from_numpy(B)
+    sk = cma.compress("fwht", dAT, dB, 5, 256, seed=3)
+    hsk = cma.compress("fwht", AT, B, 5, 256, seed=3)
+    out = cma.DeviceArray((40, 50), np.float32)
+    sk.decompress(out)
+    assert np.array_equal(out.numpy(), hsk.decompress())
+
+
+# ---------------------------------------------------------------- end to end
+@pytest.mark.parametrize("name", ["fft512_fp64", "fwht2048_fp32"])
+def test_recovered_set_planted(name):
+    c = workloads.CONFIGS[name]
+    AT, B, plants = workloads.make(name)
+    C_true = orc.gemm(AT.astype(np.float64), B.astype(np.float64))
+    truth = set(zip(*np.nonzero(C_true > c["tau"])))
+    sk = cma.compress(c["variant"], AT, B, c["d"], c["b"], seed=13794)
+    ii, jj, vv = sk.recover(c["tau"])
+    got = set(zip(ii.tolist(), jj.tolist()))
+    assert got == truth
+    assert len(truth) >= c["plants"]
+    # the oracle's own median estimate from the GPU sketch gives the same set
+    hA, hB = sk.hashes()
+    C_orc = orc.decompress(c["variant"], sk.values(), hA, hB)
+    assert set(zip(*np.nonzero(C_orc > c["tau"]))) == got
+    # and the sketch agrees with the oracle's FP64 sketch
+    want = orc.sketch_from_product(c["variant"], C_true, hA, hB, c["b"])
+    err = rel_l2(sk.values(), want)
+    print(f"{name}: |set|={len(got)} sketch rel err {err:.3e}")
+    assert err <= (1e-10 if c["dtype"] == np.float64 else 1e-5)
+
+
+def test_repeatable_and_padded_leading_dimension():
+    rng = np.random.default_rng(2)
+    AT = rng.integers(-5, 6, (64, 48)).astype(np.float64)
+    B = rng.integers(-5, 6, (64, 40)).astype(np.float64)
+    a = cma.compress("fwht", AT, B, 7, 512, seed=9).values()
+    b = cma.compress("fwht", AT, B, 7, 512, seed=9).values()
+    assert np.array_equal(a, b)
+    ATp = np.zeros((64, 64))
+    ATp[:, :48] = AT
+    dp = cma.DeviceArray.from_numpy(ATp)
+    dp.shape = (64, 64)
+    # padded leading dimension through the C-ABI directly
+    sk = cma.Sketch(cma.FWHT, np.dtype(np.float64), 7, 512, 48, 40)
+    dB = cma.DeviceArray.from_numpy(B)
+    import ctypes
+    cma.check(cma.lib().cm_compress_ex(cma.FWHT, cma.F64, 48, 64, 40, dp.ptr, 64, dB.ptr, 40,
+                                       7, 512, 9, 0, cma.DEVICE, 0, ctypes.byref(sk.handle)))
+    assert np.array_equal(sk.values(), a)
+
+
+def test_errors_raise():
+    AT = np.ones((4, 4))
+    with pytest.raises(ValueError, match="b must be power of 2"):
+        cma.compress("fft", AT, AT, 3, 48, seed=1)
+    with pytest.raises(ValueError, match="d must be positive"):
+        cma.compress("fft", AT, AT, 0, 32, seed=1)
