@@ -52,7 +52,7 @@ struct DevBuf {
 struct State {
   int device = -1;
   hipStream_t stream = nullptr;
-  DevBuf partial, V, params, tables, AT, B, C, counter, oi, oj, ov, plainA, plainB;
+  DevBuf partial, V, params, tables, AT, B, C, counter, oi, oj, ov, plainA, plainB, entries;
   void* pinned = nullptr;
   size_t pinned_n = 0;
   std::map<std::pair<int, int>, void*> twiddles;  // (log2 b, f64) -> table
@@ -388,6 +388,49 @@ int compress_device(cm_sketch* sk, const void* AT, int64_t ld_at, const void* B,
     ra.V = g.V.p;
     if ((rc = timed("reduce", [&] { return cm::launch_reduce_inverse(ra, g.stream); }))) return rc;
     cm::CombineArgs ca{fwht, f64, g.V.p, d, lb, lb, tw, sk->P};
+    return timed("combine", [&] { return cm::launch_combine(ca, g.stream); });
+  }
+
+  // fast path: FP32 FWHT with b >= 4096 runs the register-resident kernel
+  const bool reg = fwht && !f64 && !(flags & CM_FLAG_LDS_KERNEL) && cm::compress_fwht_reg_supported(lb);
+  if (reg) {
+    const int lb2 = cm::compress_fwht_reg_lb2();
+    const int64_t per_chunk = int64_t(d) << (lb - lb2);
+    const int64_t slots = 2048;  // resident single-wave workgroups (8 per CU)
+    const int64_t rounds = std::max<int64_t>(1, (int64_t(n) * per_chunk + slots * 64 - 1) / (slots * 64));
+    int64_t nchunks = std::max<int64_t>(1, slots * rounds / per_chunk);
+    nchunks = std::min<int64_t>(nchunks, std::max(1, n));
+    const size_t chunk_bytes = size_t(d) * b * esz;
+    nchunks = std::max<int64_t>(1, std::min<int64_t>(nchunks, int64_t((size_t(1) << 30) / chunk_bytes)));
+    const int kper = n > 0 ? int((n + nchunks - 1) / nchunks) : 0;
+    nchunks = n > 0 ? (n + kper - 1) / kper : 0;
+    int rc = ensure_buf(g.partial, std::max<size_t>(1, size_t(nchunks) * chunk_bytes));
+    if (rc) return rc;
+    if ((rc = ensure_buf(g.V, chunk_bytes))) return rc;
+    if ((rc = ensure_buf(g.entries, cm::compress_fwht_reg_entries_bytes(d, lb, m, p)))) return rc;
+    cm::CompressArgs caa{};
+    caa.fwht = 1;
+    caa.AT = AT;
+    caa.ld_at = ld_at;
+    caa.B = B;
+    caa.ld_b = ld_b;
+    caa.hA = sk->hA;
+    caa.hB = sk->hB;
+    caa.m = m;
+    caa.n = n;
+    caa.p = p;
+    caa.d = d;
+    caa.lb = lb;
+    caa.lb2 = lb2;
+    caa.nchunks = int(nchunks);
+    caa.kper = kper;
+    caa.partial = g.partial.p;
+    if ((rc = timed("compress", [&] { return cm::launch_compress_fwht_reg(caa, g.entries.p, g.stream); })))
+      return rc;
+    const int f64_reduce = (flags & CM_FLAG_F64_REDUCE) || n > 16384;
+    cm::ReduceArgs ra{1, 0, f64_reduce, g.partial.p, int(nchunks), d, lb, lb2, nullptr, g.V.p};
+    if ((rc = timed("reduce", [&] { return cm::launch_reduce_inverse(ra, g.stream); }))) return rc;
+    cm::CombineArgs ca{1, 0, g.V.p, d, lb, lb2, nullptr, sk->P};
     return timed("combine", [&] { return cm::launch_combine(ca, g.stream); });
   }
 

@@ -183,6 +183,17 @@ __global__ void __launch_bounds__(NT) k_compress_lds(CompressArgs a) {
   }
 }
 
+// sum of the per-chunk partial spectra, written over chunk 0 (each thread
+// owns one element and reads every chunk in order: deterministic)
+template <typename T, typename ACC>
+__global__ void __launch_bounds__(NT) k_chunk_sum(T* __restrict__ P, int64_t elems, int nchunks) {
+  const int64_t idx = int64_t(blockIdx.x) * NT + threadIdx.x;
+  if (idx >= elems) return;
+  ACC s = ACC(0);
+  for (int c = 0; c < nchunks; ++c) s += ACC(P[c * elems + idx]);
+  P[idx] = T(s);
+}
+
 template <typename T, typename ACC, bool FWHT>
 __global__ void __launch_bounds__(NT) k_reduce_inverse(ReduceArgs a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -270,11 +281,23 @@ hipError_t compress_lds_t(const CompressArgs& a, hipStream_t s) {
 }
 
 template <typename T, bool FWHT>
-hipError_t reduce_t(const ReduceArgs& a, hipStream_t s) {
+hipError_t reduce_t(const ReduceArgs& a0, hipStream_t s) {
+  ReduceArgs a = a0;
+  const bool wide = a.f64_reduce || sizeof(T) == 8;
+  if (a.nchunks > 1) {
+    const int64_t elems = (int64_t(a.d) << a.lb) * (FWHT ? 1 : 2);
+    const unsigned blocks = unsigned((elems + NT - 1) / NT);
+    T* P = static_cast<T*>(const_cast<void*>(a.partial));
+    if (wide)
+      k_chunk_sum<T, double><<<dim3(blocks), dim3(NT), 0, s>>>(P, elems, a.nchunks);
+    else
+      k_chunk_sum<T, T><<<dim3(blocks), dim3(NT), 0, s>>>(P, elems, a.nchunks);
+    a.nchunks = 1;
+  }
   const int b1 = 1 << (a.lb - a.lb2);
   const size_t lds = (FWHT ? 1 : 2) * sizeof(T) * (size_t(1) << a.lb2);
   const unsigned grid = unsigned(a.d * b1);
-  if (a.f64_reduce || sizeof(T) == 8)
+  if (wide)
     k_reduce_inverse<T, double, FWHT><<<dim3(grid), dim3(NT), lds, s>>>(a);
   else
     k_reduce_inverse<T, T, FWHT><<<dim3(grid), dim3(NT), lds, s>>>(a);

@@ -85,6 +85,71 @@ __global__ void __launch_bounds__(NT) k_decompress(DecompressArgs a) {
   }
 }
 
+// LDS-staged variant for sketch rows of at most 64 KB: the workgroup (512
+// threads, one column each, 6 (float) or 3 (double) rows per thread) streams the d rows of the
+// sketch through two LDS buffers (row t+1 is fetched while row t is
+// gathered), so the random gathers hit LDS instead of 128-byte L2 lines.
+constexpr int LNT = 512;
+template <typename T>
+constexpr int rows_per_thread() { return sizeof(T) == 8 ? 3 : 6; }
+
+template <typename T, bool FWHT, int D>
+__global__ void __launch_bounds__(LNT) k_decompress_lds(DecompressArgs a) {
+  constexpr int ER = rows_per_thread<T>();
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int b = a.b;
+  T* const buf0 = reinterpret_cast<T*>(smem_raw);
+  const int tiles = (a.p + LNT - 1) / LNT;
+  const int tid = threadIdx.x;
+  const int j = int(blockIdx.x % tiles) * LNT + tid;
+  const int i0 = int(blockIdx.x / tiles) * ER;
+  const bool jok = j < a.p;
+  const uint32_t mask = uint32_t(b - 1);
+  const T* __restrict__ P = static_cast<const T*>(a.P);
+  const int vec = b * int(sizeof(T)) / 16;  // float4 per row
+  constexpr int MAXV = 65536 / 16 / LNT;    // float4 per thread per row (<= 8)
+  uint32_t eb[D];
+#pragma unroll
+  for (int t = 0; t < D; ++t) eb[t] = jok ? a.hB[int64_t(t) * a.p + j] : 0u;
+  {
+    const float4* src = reinterpret_cast<const float4*>(P);
+    float4* dst = reinterpret_cast<float4*>(buf0);
+    for (int q = tid; q < vec; q += LNT) dst[q] = src[q];
+  }
+  T v[ER][D];
+#pragma unroll
+  for (int t = 0; t < D; ++t) {
+    __syncthreads();
+    float4 nxt[MAXV];
+    if (t + 1 < D) {
+      const float4* src = reinterpret_cast<const float4*>(P + int64_t(t + 1) * b);
+#pragma unroll
+      for (int q = 0; q < MAXV; ++q) nxt[q] = src[min(q * LNT + tid, vec - 1)];
+    }
+    const T* cur = buf0 + (t & 1) * b;
+#pragma unroll
+    for (int e = 0; e < ER; ++e) {
+      const int i = min(i0 + e, a.m - 1);
+      const uint32_t ea = a.hA[int64_t(t) * a.m + i];
+      const uint32_t h = FWHT ? (hbucket(ea) ^ hbucket(eb[t])) : ((hbucket(ea) + hbucket(eb[t])) & mask);
+      const T val = cur[h];
+      v[e][t] = (hneg(ea) ^ hneg(eb[t])) ? -val : val;
+    }
+    if (t + 1 < D) {
+      float4* dst = reinterpret_cast<float4*>(buf0 + ((t + 1) & 1) * b);
+#pragma unroll
+      for (int q = 0; q < MAXV; ++q)
+        if (q * LNT + tid < vec) dst[q * LNT + tid] = nxt[q];
+    }
+  }
+  if (!jok) return;
+#pragma unroll
+  for (int e = 0; e < ER; ++e) {
+    if (i0 + e >= a.m) break;
+    emit<T>(a, i0 + e, j, median_network<T, D>(v[e]));
+  }
+}
+
 // any d: the value of rank d/2 by counting, O(d^2) gathers per entry
 template <typename T, bool FWHT>
 __global__ void __launch_bounds__(NT) k_decompress_any(DecompressArgs a) {
@@ -116,6 +181,25 @@ __global__ void __launch_bounds__(NT) k_decompress_any(DecompressArgs a) {
 
 template <typename T, bool FWHT>
 hipError_t launch_t(const DecompressArgs& a, hipStream_t s) {
+  if (a.d <= 16 && size_t(a.b) * sizeof(T) <= 65536 && size_t(a.b) * sizeof(T) >= 16) {
+    constexpr int ER = rows_per_thread<T>();
+    const dim3 lgrid(unsigned(int64_t((a.p + LNT - 1) / LNT) * ((a.m + ER - 1) / ER)));
+    const size_t lds = 2 * size_t(a.b) * sizeof(T);
+    switch (a.d) {
+#define CM_D(N)                                                                       \
+  case N:                                                                           \
+    if (lds > 65536 && hipFuncSetAttribute(reinterpret_cast<const void*>(&k_decompress_lds<T, FWHT, N>), \
+                                           hipFuncAttributeMaxDynamicSharedMemorySize,  \
+                                           int(lds)) != hipSuccess)                     \
+      break;                                                                        \
+    k_decompress_lds<T, FWHT, N><<<lgrid, dim3(LNT), lds, s>>>(a);                  \
+    return hipGetLastError();
+      CM_D(1) CM_D(2) CM_D(3) CM_D(4) CM_D(5) CM_D(6) CM_D(7) CM_D(8)
+      CM_D(9) CM_D(10) CM_D(11) CM_D(12) CM_D(13) CM_D(14) CM_D(15) CM_D(16)
+#undef CM_D
+    }
+    (void)hipGetLastError();  // attribute refused: use the L2-gather kernel below
+  }
   const dim3 grid(unsigned(int64_t((a.p + NT - 1) / NT) * ((a.m + ROWS - 1) / ROWS)));
   const dim3 block(NT);
   switch (a.d) {

@@ -1,6 +1,7 @@
 // Host-side launchers of the HIP kernels (implemented in the .hip files).
 #pragma once
 #include <hip/hip_runtime.h>
+#include <stddef.h>
 #include <stdint.h>
 
 namespace cm {
@@ -41,9 +42,13 @@ struct CompressArgs {
 int compress_lds_max_lb2(int fwht, int f64);
 hipError_t launch_compress_lds(const CompressArgs& a, hipStream_t s);
 
-// Register-resident single-wave FP32 FWHT compress (fast path).
-int compress_fwht_reg_supported(int lb2);
-hipError_t launch_compress_fwht_reg(const CompressArgs& a, hipStream_t s);
+// Register-resident single-wave FP32 FWHT compress (fast path): slices of
+// 2^compress_fwht_reg_lb2() frequencies; `entries` is a scratch buffer of
+// compress_fwht_reg_entries_bytes() bytes for the packed scatter entries.
+int compress_fwht_reg_supported(int lb);
+int compress_fwht_reg_lb2();
+size_t compress_fwht_reg_entries_bytes(int d, int lb, int m, int p);
+hipError_t launch_compress_fwht_reg(const CompressArgs& a, void* entries, hipStream_t s);
 
 // Sum of the per-chunk partial spectra, then the per-slice inverse transform.
 struct ReduceArgs {

@@ -175,3 +175,26 @@ def test_errors_raise():
         cma.compress("fft", AT, AT, 3, 48, seed=1)
     with pytest.raises(ValueError, match="d must be positive"):
         cma.compress("fft", AT, AT, 0, 32, seed=1)
+
+
+@pytest.mark.parametrize("m,n,p,b", [(2048, 96, 2048, 1 << 14), (100, 70, 37, 1 << 12),
+                                     (1, 5, 3, 1 << 13), (300, 33, 0, 1 << 12)])
+def test_fwht_register_kernel_matches_lds_kernel(m, n, p, b):
+    print(f"reg vs lds m={m} n={n} p={p} b={b}", flush=True)
+    AT, B, _ = workloads.planted(m, n, max(p, 1), 4, 5.0, m + n, np.float32)
+    B = np.ascontiguousarray(B[:, :p])
+    d = 7
+    reg = cma.compress("fwht", AT, B, d, b, seed=3, dtype=np.float32)
+    lds = cma.compress("fwht", AT, B, d, b, seed=3, dtype=np.float32,
+                       flags=cma.FLAG_LDS_KERNEL)
+    again = cma.compress("fwht", AT, B, d, b, seed=3, dtype=np.float32)
+    v1, v2 = reg.values(), lds.values()
+    assert np.array_equal(v1, again.values())  # single-wave scatter: deterministic
+    if p == 0 or m == 0:
+        assert not v1.any() and not v2.any()
+        return
+    assert rel_l2(v1, v2) <= 2e-6
+    hA, hB = reg.hashes()
+    want = orc.sketch_from_product("fwht", AT.astype(np.float64).T @ B.astype(np.float64),
+                                   hA, hB, b)
+    assert rel_l2(v1, want) <= 1e-5
