@@ -392,7 +392,7 @@ int compress_device(cm_sketch* sk, const void* AT, int64_t ld_at, const void* B,
   }
 
   // fast path: FP32 FWHT with b >= 4096 runs the register-resident kernel
-  const bool reg = fwht && !f64 && !(flags & CM_FLAG_LDS_KERNEL) && cm::compress_fwht_reg_supported(lb);
+  const bool reg = fwht && !f64 && !(flags & CM_FLAG_LDS_KERNEL) && cm::compress_fwht_reg_supported(lb, m, p);
   if (reg) {
     const int lb2 = cm::compress_fwht_reg_lb2();
     const int64_t per_chunk = int64_t(d) << (lb - lb2);

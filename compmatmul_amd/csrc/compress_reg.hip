@@ -16,12 +16,11 @@
 //    per-lane base plus an immediate offset.
 //  * The spectral accumulator (64 VGPRs) stays in registers for the whole
 //    chunk of k; only the final sum is written.
-//  * The scatter uses per-(t, slice) precomputed entries (LDS byte offset |
-//    combined sign bit), so each nonzero costs two ANDs, one XOR and one LDS
-//    float add.  One wave owns its LDS buffer, so the scatter order -- and
-//    the result -- is deterministic.
-//  * One 16.25 KB LDS buffer per wave serves the scatter, the read-back and
-//    both transposes; eight waves fit a CU by LDS.
+//  * The scatter follows a per-(t, slice, operand) schedule of conflict-free
+//    plain LDS writes built once per product (k_reg_lists, below); the row
+//    is staged in LDS so each scheduled item is an LDS gather plus a write.
+//  * LDS per wave: 16.25 KB for the sketch/transposes + 8 KB row stage
+//    (six waves per CU).
 //  * Workgroups are numbered so that the (t, slice) units of one chunk of k
 //    run on the same XCD and share its L2 copy of the rows of AT and B.
 #include "device_common.hpp"
@@ -41,29 +40,103 @@ __device__ __forceinline__ float flip(float v, uint32_t sgn) {
   return __int_as_float(__float_as_int(v) ^ sgn);
 }
 
-// packed scatter entries: entry = (lo * 4) | (sign << 31), where lo is the
-// bucket modulo b2 and sign combines s(i) with the slice modulation
-// (-1)^popc((h >> log2 b2) & slice)
-__global__ void __launch_bounds__(256) k_reg_entries(const uint32_t* __restrict__ hA,
-                                                     const uint32_t* __restrict__ hB, int m,
-                                                     int p, int d, int lbs,
-                                                     uint32_t* __restrict__ EA,
-                                                     uint32_t* __restrict__ EB) {
+// ---------------------------------------------------------------- schedules
+// LDS float atomics are slow on gfx950 (~190 clk per wave instruction), so
+// the scatter is a precomputed schedule of plain LDS writes.  For each
+// (t, slice, operand) the keys of a row are processed in segments of 2048;
+// within a segment, items are grouped by their occurrence index in their
+// bucket ("round"): round 0 (first occurrence in the whole row) writes its
+// value, later rounds read-add-write, and ranks >= RCAP-1 fall back to LDS
+// atomics.  Items of one group (64 lanes) never share a bucket, and every
+// group runs after the groups holding earlier items of its buckets, so the
+// result is exact FP32 in a fixed order (deterministic).
+// Entry: dst (13 bits: bucket mod 4096, or a per-lane dummy slot 4096+lane
+// for padding) | src (16 bits: key index) << 13 | sign << 29 | RMW << 30 |
+// ATOMIC << 31.  Flags are uniform per group.
+constexpr int SEG = 2048;
+constexpr int RCAP = 8;
+constexpr uint32_t F_SIGN = 1u << 29, F_RMW = 1u << 30, F_ATOMIC = 1u << 31;
+
+__device__ __forceinline__ uint32_t round_flags(int r) {
+  return r == 0 ? 0u : (r < RCAP - 1 ? F_RMW : F_ATOMIC);
+}
+
+__global__ void __launch_bounds__(64) k_reg_lists(const uint32_t* __restrict__ hA,
+                                                  const uint32_t* __restrict__ hB, int m, int p,
+                                                  int d, int lbs, int64_t cap_a, int64_t cap_b,
+                                                  uint32_t* __restrict__ LA,
+                                                  uint32_t* __restrict__ LB,
+                                                  int* __restrict__ ngroups) {
+  __shared__ uint32_t cnt[B2];
+  __shared__ uint8_t rnd[SEG];
+  __shared__ int nr[RCAP];
   const int b1 = 1 << lbs;
-  const int64_t na = int64_t(d) * b1 * m;
-  const int64_t idx = int64_t(blockIdx.x) * 256 + threadIdx.x;
-  if (idx >= na + int64_t(d) * b1 * p) return;
-  const bool sb = idx >= na;
-  const int64_t r = sb ? idx - na : idx;
-  const int keys = sb ? p : m;
-  const int key = int(r % keys);
-  const int64_t ts = r / keys;  // t * b1 + slice
-  const int slice = int(ts & (b1 - 1));
-  const int t = int(ts >> lbs);
-  const uint32_t e = (sb ? hB : hA)[int64_t(t) * keys + key];
-  const uint32_t h = hbucket(e);
-  const uint32_t neg = hneg(e) ^ (__popc((h >> LB2) & uint32_t(slice)) & 1u);
-  (sb ? EB : EA)[r] = ((h & (B2 - 1)) << 2) | (neg << 31);
+  const int lists = d * b1;
+  const int g = blockIdx.x;
+  const bool side_b = g >= lists;
+  const int ts = side_b ? g - lists : g;
+  const int t = ts >> lbs;
+  const uint32_t slice = uint32_t(ts & (b1 - 1));
+  const int keys = side_b ? p : m;
+  const uint32_t* __restrict__ hh = (side_b ? hB : hA) + int64_t(t) * keys;
+  uint32_t* __restrict__ L = side_b ? LB + ts * cap_b : LA + ts * cap_a;
+  const int lane = threadIdx.x;
+  for (int e = lane; e < B2; e += 64) cnt[e] = 0;
+  int out = 0;  // groups written so far
+  for (int s0 = 0; s0 < keys; s0 += SEG) {
+    const int sl = min(SEG, keys - s0);
+    if (lane < RCAP) nr[lane] = 0;
+    __syncthreads();
+    for (int c = 0; c < sl; c += 64) {
+      if (c + lane < sl) {
+        const uint32_t h = hbucket(hh[s0 + c + lane]);
+        const int r = min(int(atomicAdd(&cnt[h & (B2 - 1)], 1u)), RCAP - 1);
+        rnd[c + lane] = uint8_t(r);
+        atomicAdd(&nr[r], 1);
+      }
+      __syncthreads();
+    }
+    int base[RCAP], run[RCAP];
+    int acc = out;
+#pragma unroll
+    for (int r = 0; r < RCAP; ++r) {
+      base[r] = acc;
+      acc += (nr[r] + 63) / 64;
+      run[r] = 0;
+    }
+    for (int c = 0; c < sl; c += 64) {
+      const bool ok = c + lane < sl;
+      const int r = ok ? int(rnd[c + lane]) : -1;
+      uint32_t ent = 0;
+      if (ok) {
+        const uint32_t e = hh[s0 + c + lane];
+        const uint32_t h = hbucket(e);
+        const uint32_t neg = hneg(e) ^ (__popc((h >> LB2) & slice) & 1u);
+        ent = (h & (B2 - 1)) | (uint32_t(s0 + c + lane) << 13) | (neg ? F_SIGN : 0u) |
+              round_flags(r);
+      }
+#pragma unroll
+      for (int rr = 0; rr < RCAP; ++rr) {
+        const uint64_t mask = __ballot(r == rr);
+        if (r == rr) {
+          const int pos = run[rr] + __popcll(mask & ((1ull << lane) - 1ull));
+          L[int64_t(base[rr]) * 64 + pos] = ent;
+        }
+        run[rr] += __popcll(mask);
+      }
+    }
+    // pad every round to whole groups with writes to per-lane dummy slots
+#pragma unroll
+    for (int rr = 0; rr < RCAP; ++rr) {
+      const int full = (nr[rr] + 63) / 64 * 64;
+      for (int slot = nr[rr] + lane; slot < full; slot += 64)
+        L[int64_t(base[rr]) * 64 + slot] =
+            uint32_t(B2 + (slot & 63)) | (uint32_t(s0) << 13) | round_flags(rr);
+    }
+    out = acc;
+    __syncthreads();
+  }
+  if (lane == 0) ngroups[g] = out;
 }
 
 __device__ __forceinline__ void wht_reg(float (&x)[R]) {
@@ -79,41 +152,54 @@ __device__ __forceinline__ void wht_reg(float (&x)[R]) {
   }
 }
 
-__device__ __forceinline__ void zero_lds(float* S, int lane) {
-  float4* S4 = reinterpret_cast<float4*>(S);
+// stage keys [seg*SEG, seg*SEG + SEG) of a row into Rb
+__device__ __forceinline__ void stage_row(const float* __restrict__ row, int keys, int seg,
+                                          float* Rb, int lane) {
+  const int s0 = seg * SEG;
+  const int sl = min(SEG, keys - s0);
+  float v[SEG / 64];
 #pragma unroll
-  for (int j = 0; j < SLEN / 4 / 64; ++j) S4[j * 64 + lane] = make_float4(0.f, 0.f, 0.f, 0.f);
-  if (lane < (SLEN / 4) % 64) S4[(SLEN / 4 / 64) * 64 + lane] = make_float4(0.f, 0.f, 0.f, 0.f);
+  for (int q = 0; q < SEG / 64; ++q) v[q] = row[s0 + min(q * 64 + lane, sl - 1)];
+#pragma unroll
+  for (int q = 0; q < SEG / 64; ++q) Rb[q * 64 + lane] = v[q];
 }
 
-// sketch one row into S (zeroed), read it into x, zero S, full WHT of x.
-// The row and its entries are loaded in batches of 32 per lane so that the
-// loads are in flight together before the LDS adds consume them.
-constexpr int BATCH = 32;
+// scatter one row into S following its schedule, read the sketch into x,
+// full WHT of x (the transpose leaves S dirty; the next scatter zeroes it)
+constexpr int GB = 16;  // schedule entries prefetched per batch
 
-__device__ __forceinline__ void sketch_transform(const float* __restrict__ row,
-                                                 const uint32_t* __restrict__ ent, int keys,
-                                                 float* S, int lane, float (&x)[R]) {
-  char* Sb = reinterpret_cast<char*>(S);
-  for (int base = 0; base < keys; base += 64 * BATCH) {
-    float v[BATCH];
-    uint32_t e[BATCH];
+__device__ __forceinline__ void sketch_transform(const float* __restrict__ row, int keys,
+                                                 const uint32_t* __restrict__ list, int ng,
+                                                 float* S, float* Rb, int lane, float (&x)[R]) {
+  float4* S4 = reinterpret_cast<float4*>(S);
 #pragma unroll
-    for (int q = 0; q < BATCH; ++q) {
-      // lanes past the end re-read the last key and add -0.0 to S[0], an
-      // exact no-op, so the loop needs no branches
-      const int i = base + q * 64 + lane;
-      const int ic = min(i, keys - 1);
-      const float vq = row[ic];
-      const uint32_t eq = ent[ic];
-      v[q] = i < keys ? vq : -0.f;
-      e[q] = i < keys ? eq : 0u;
+  for (int j = 0; j < B2 / 4 / 64; ++j) S4[j * 64 + lane] = make_float4(0.f, 0.f, 0.f, 0.f);
+  int cur = -1;
+  for (int g0 = 0; g0 < ng; g0 += GB) {
+    uint32_t ent[GB];
+#pragma unroll
+    for (int q = 0; q < GB; ++q) ent[q] = list[int64_t(min(g0 + q, ng - 1)) * 64 + lane];
+#pragma unroll
+    for (int q = 0; q < GB; ++q) {
+      if (g0 + q >= ng) break;
+      const uint32_t e = ent[q];
+      const uint32_t first = __builtin_amdgcn_readfirstlane(e);
+      const int seg = int(((first >> 13) & 0xffffu) / SEG);
+      if (seg != cur) {
+        __syncthreads();
+        stage_row(row, keys, seg, Rb, lane);
+        __syncthreads();
+        cur = seg;
+      }
+      const uint32_t dst = e & 0x1fffu;
+      float v = flip(Rb[((e >> 13) & 0xffffu) % SEG], (e & F_SIGN) << 2);
+      if (first & F_ATOMIC) {
+        __hip_atomic_fetch_add(&S[dst], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      } else {
+        if (first & F_RMW) v += S[dst];
+        S[dst] = v;
+      }
     }
-#pragma unroll
-    for (int q = 0; q < BATCH; ++q)
-      __hip_atomic_fetch_add(reinterpret_cast<float*>(Sb + (e[q] & 0xffffu)),
-                             flip(v[q], e[q] & 0x80000000u), __ATOMIC_RELAXED,
-                             __HIP_MEMORY_SCOPE_WORKGROUP);
   }
   __syncthreads();
 #pragma unroll
@@ -127,15 +213,17 @@ __device__ __forceinline__ void sketch_transform(const float* __restrict__ row,
 #pragma unroll
   for (int r = 0; r < R; ++r) x[r] = S[lane * SROW + r];
   __syncthreads();
-  zero_lds(S, lane);
   wht_reg(x);  // former lane bits (index bits 0..5)
 }
 
 __global__ void __launch_bounds__(64) k_compress_fwht_reg(CompressArgs a,
-                                                          const uint32_t* __restrict__ EA,
-                                                          const uint32_t* __restrict__ EB,
+                                                          const uint32_t* __restrict__ LA,
+                                                          const uint32_t* __restrict__ LB,
+                                                          int64_t cap_a, int64_t cap_b,
+                                                          const int* __restrict__ ngroups,
                                                           int units) {
   __shared__ __align__(16) float S[SLEN];
+  __shared__ __align__(16) float Rb[SEG];
   const int lbs = a.lb - LB2;
   const int b1 = 1 << lbs;
   // XCD-aware numbering: workgroup g runs on XCD g % 8; give each XCD a
@@ -147,30 +235,26 @@ __global__ void __launch_bounds__(64) k_compress_fwht_reg(CompressArgs a,
   const int per_chunk = a.d * b1;
   const int chunk = u / per_chunk;
   const int ts = u - chunk * per_chunk;  // t * b1 + slice
-  const int t = ts >> lbs;
   const int lane = threadIdx.x;
   const int k0 = chunk * a.kper;
   const int k1 = min(a.n, k0 + a.kper);
   const float* __restrict__ AT = static_cast<const float*>(a.AT);
   const float* __restrict__ B = static_cast<const float*>(a.B);
-  const uint32_t* __restrict__ ea = EA + int64_t(ts) * a.m;
-  const uint32_t* __restrict__ eb = EB + int64_t(ts) * a.p;
+  const uint32_t* __restrict__ la = LA + ts * cap_a;
+  const uint32_t* __restrict__ lb = LB + ts * cap_b;
+  const int nga = ngroups[ts];
+  const int ngb = ngroups[per_chunk + ts];
 
-  zero_lds(S, lane);
-  __syncthreads();
   float acc[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) acc[r] = 0.f;
   for (int k = k0; k < k1; ++k) {
     float xa[R], xb[R];
-    sketch_transform(AT + int64_t(k) * a.ld_at, ea, a.m, S, lane, xa);
-    __syncthreads();
-    sketch_transform(B + int64_t(k) * a.ld_b, eb, a.p, S, lane, xb);
-    __syncthreads();
+    sketch_transform(AT + int64_t(k) * a.ld_at, a.m, la, nga, S, Rb, lane, xa);
+    sketch_transform(B + int64_t(k) * a.ld_b, a.p, lb, ngb, S, Rb, lane, xb);
 #pragma unroll
     for (int r = 0; r < R; ++r) acc[r] = fmaf(xa[r], xb[r], acc[r]);
   }
-  (void)t;
   // lane L, register r holds frequency L*64 + r of this slice
   float4* out = reinterpret_cast<float4*>(static_cast<float*>(a.partial) +
                                           (int64_t(chunk) * per_chunk + ts) * B2 + lane * R);
@@ -178,29 +262,37 @@ __global__ void __launch_bounds__(64) k_compress_fwht_reg(CompressArgs a,
   for (int r = 0; r < R; r += 4) out[r / 4] = make_float4(acc[r], acc[r + 1], acc[r + 2], acc[r + 3]);
 }
 
+int64_t list_cap(int keys) {
+  const int64_t segs = (int64_t(keys) + SEG - 1) / SEG;
+  return int64_t(keys) + segs * 64 * RCAP;
+}
+
 }  // namespace
 
-int compress_fwht_reg_supported(int lb) { return lb >= LB2 && lb <= 30; }
+int compress_fwht_reg_supported(int lb, int m, int p) {
+  return lb >= LB2 && lb <= 30 && m <= 65536 && p <= 65536;
+}
 
 int compress_fwht_reg_lb2() { return LB2; }
 
 size_t compress_fwht_reg_entries_bytes(int d, int lb, int m, int p) {
-  return size_t(d) * (size_t(1) << (lb - LB2)) * (size_t(m) + size_t(p)) * 4;
+  const size_t lists = size_t(d) << (lb - LB2);
+  return lists * size_t(list_cap(m) + list_cap(p)) * 4 + 2 * lists * sizeof(int) + 256;
 }
 
 hipError_t launch_compress_fwht_reg(const CompressArgs& a, void* entries, hipStream_t s) {
   if (a.nchunks == 0) return hipSuccess;
   const int lbs = a.lb - LB2;
-  const int b1 = 1 << lbs;
-  uint32_t* EA = static_cast<uint32_t*>(entries);
-  uint32_t* EB = EA + int64_t(a.d) * b1 * a.m;
-  const int64_t total = int64_t(a.d) * b1 * (int64_t(a.m) + a.p);
-  if (total > 0)
-    k_reg_entries<<<dim3(unsigned((total + 255) / 256)), dim3(256), 0, s>>>(a.hA, a.hB, a.m, a.p,
-                                                                            a.d, lbs, EA, EB);
-  const int units = a.nchunks * a.d * b1;
+  const int lists = a.d << lbs;
+  const int64_t cap_a = list_cap(a.m), cap_b = list_cap(a.p);
+  uint32_t* LA = static_cast<uint32_t*>(entries);
+  uint32_t* LB = LA + lists * cap_a;
+  int* ng = reinterpret_cast<int*>(LB + lists * cap_b);
+  k_reg_lists<<<dim3(unsigned(2 * lists)), dim3(64), 0, s>>>(a.hA, a.hB, a.m, a.p, a.d, lbs, cap_a,
+                                                             cap_b, LA, LB, ng);
+  const int units = a.nchunks * lists;
   const int grid = (units + 7) / 8 * 8;
-  k_compress_fwht_reg<<<dim3(unsigned(grid)), dim3(64), 0, s>>>(a, EA, EB, units);
+  k_compress_fwht_reg<<<dim3(unsigned(grid)), dim3(64), 0, s>>>(a, LA, LB, cap_a, cap_b, ng, units);
   return hipGetLastError();
 }
 

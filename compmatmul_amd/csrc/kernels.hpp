@@ -45,7 +45,7 @@ hipError_t launch_compress_lds(const CompressArgs& a, hipStream_t s);
 // Register-resident single-wave FP32 FWHT compress (fast path): slices of
 // 2^compress_fwht_reg_lb2() frequencies; `entries` is a scratch buffer of
 // compress_fwht_reg_entries_bytes() bytes for the packed scatter entries.
-int compress_fwht_reg_supported(int lb);
+int compress_fwht_reg_supported(int lb, int m, int p);
 int compress_fwht_reg_lb2();
 size_t compress_fwht_reg_entries_bytes(int d, int lb, int m, int p);
 hipError_t launch_compress_fwht_reg(const CompressArgs& a, void* entries, hipStream_t s);

@@ -178,7 +178,8 @@ def test_errors_raise():
 
 
 @pytest.mark.parametrize("m,n,p,b", [(2048, 96, 2048, 1 << 14), (100, 70, 37, 1 << 12),
-                                     (1, 5, 3, 1 << 13), (300, 33, 0, 1 << 12)])
+                                     (1, 5, 3, 1 << 13), (300, 33, 0, 1 << 12),
+                                     (60000, 3, 50, 1 << 12), (5000, 20, 4100, 1 << 15)])
 def test_fwht_register_kernel_matches_lds_kernel(m, n, p, b):
     print(f"reg vs lds m={m} n={n} p={p} b={b}", flush=True)
     AT, B, _ = workloads.planted(m, n, max(p, 1), 4, 5.0, m + n, np.float32)
