@@ -396,7 +396,7 @@ int compress_device(cm_sketch* sk, const void* AT, int64_t ld_at, const void* B,
   if (reg) {
     const int lb2 = cm::compress_fwht_reg_lb2();
     const int64_t per_chunk = int64_t(d) << (lb - lb2);
-    const int64_t slots = 2048;  // resident single-wave workgroups (8 per CU)
+    const int64_t slots = cm::compress_fwht_reg_slots();  // resident single-wave workgroups
     const int64_t rounds = std::max<int64_t>(1, (int64_t(n) * per_chunk + slots * 64 - 1) / (slots * 64));
     int64_t nchunks = std::max<int64_t>(1, slots * rounds / per_chunk);
     nchunks = std::min<int64_t>(nchunks, std::max(1, n));

@@ -1,26 +1,27 @@
 // Register-resident single-wave FP32 FWHT compress (the fast path).
 //
 // Same mathematics as the LDS kernel in compress.hip (sliced, modulated
-// count-sketch into b2 = 4096 buckets, natural-order Walsh-Hadamard
+// count-sketch into b2 = 2048 buckets, natural-order Walsh-Hadamard
 // transform, pointwise product, accumulation over a chunk of k), laid out
 // for one 64-lane wavefront per workgroup:
 //
-//  * A slice of 4096 values lives in 64 VGPRs per lane: element
-//    e = r*64 + lane after the sketch is read from LDS.  The WHT is a tensor
-//    product of 2-point butterflies over the 12 index bits, so the six
-//    register bits are transformed in registers, the 64x64 block is
-//    transposed through LDS, and the six lane
-//    bits -- now register bits -- are transformed in registers.  After that
-//    lane L, register r holds frequency f = L*64 + r.  The transpose uses a
-//    row stride of 65 floats: conflict-free, and every LDS address is one
-//    per-lane base plus an immediate offset.
-//  * The spectral accumulator (64 VGPRs) stays in registers for the whole
-//    chunk of k; only the final sum is written.
-//  * The scatter follows a per-(t, slice, operand) schedule of conflict-free
-//    plain LDS writes built once per product (k_reg_lists, below); the row
-//    is staged in LDS so each scheduled item is an LDS gather plus a write.
-//  * LDS per wave: 16.25 KB for the sketch/transposes + 8 KB row stage
-//    (six waves per CU).
+//  * A slice of 2048 values lives in 32 VGPRs per lane.  The WHT is a tensor
+//    product of 2-point butterflies over the 11 index bits: after the sketch
+//    is read from LDS (lane L, register r = element r*64 + L) the five
+//    register bits are transformed in registers; a transpose through LDS
+//    (rows of 33 floats: conflict-free, one base address plus immediates)
+//    moves index bits 0..4 into registers and bit 5 into lane bit 5; those
+//    five are transformed in registers and the last bit with
+//    v_permlane32_swap (lanes L and L+32) and one butterfly.
+//  * The spectral accumulator (32 VGPRs) stays in registers for the whole
+//    chunk of k; only the final sum is written (reordered through LDS).
+//  * The scatter follows a per-(t, slice, operand) schedule of
+//    conflict-free plain LDS writes built once per product (k_reg_lists):
+//    LDS float atomics cost ~190 clk per wave instruction on gfx950, plain
+//    random LDS writes ~8.  The row is staged in LDS 1024 keys at a time, so
+//    each scheduled item is an LDS gather plus a write.
+//  * LDS per wave: 8.25 KB sketch/transpose + 4 KB row stage; about 160
+//    VGPRs: three waves per SIMD.
 //  * Workgroups are numbered so that the (t, slice) units of one chunk of k
 //    run on the same XCD and share its L2 copy of the rows of AT and B.
 #include "device_common.hpp"
@@ -30,11 +31,11 @@ namespace cm {
 
 namespace {
 
-constexpr int LB2 = 12;           // slice length 4096
+constexpr int LB2 = 11;           // slice length 2048
 constexpr int B2 = 1 << LB2;
-constexpr int R = B2 / 64;        // values per lane
-constexpr int SROW = 65;          // padded row stride of the transpose
-constexpr int SLEN = 64 * SROW;   // LDS floats per wave (4160)
+constexpr int R = B2 / 64;        // values per lane (32)
+constexpr int TROW = 33;          // padded row (32 floats) of the transpose
+constexpr int SLEN = 64 * TROW;   // LDS floats per wave for sketch/transpose
 
 __device__ __forceinline__ float flip(float v, uint32_t sgn) {
   return __int_as_float(__float_as_int(v) ^ sgn);
@@ -43,18 +44,19 @@ __device__ __forceinline__ float flip(float v, uint32_t sgn) {
 // ---------------------------------------------------------------- schedules
 // LDS float atomics are slow on gfx950 (~190 clk per wave instruction), so
 // the scatter is a precomputed schedule of plain LDS writes.  For each
-// (t, slice, operand) the keys of a row are processed in segments of 2048;
+// (t, slice, operand) the keys of a row are processed in segments of 1024;
 // within a segment, items are grouped by their occurrence index in their
 // bucket ("round"): round 0 (first occurrence in the whole row) writes its
 // value, later rounds read-add-write, and ranks >= RCAP-1 fall back to LDS
 // atomics.  Items of one group (64 lanes) never share a bucket, and every
 // group runs after the groups holding earlier items of its buckets, so the
 // result is exact FP32 in a fixed order (deterministic).
-// Entry: dst (13 bits: bucket mod 4096, or a per-lane dummy slot 4096+lane
+// Entry: dst (13 bits: bucket mod 2048, or a per-lane dummy slot 2048+lane
 // for padding) | src (16 bits: key index) << 13 | sign << 29 | RMW << 30 |
 // ATOMIC << 31.  Flags are uniform per group.
-constexpr int SEG = 2048;
+constexpr int SEG = 1024;
 constexpr int RCAP = 8;
+constexpr int MAXSEG = 64;  // keys <= MAXSEG * SEG = 65536
 constexpr uint32_t F_SIGN = 1u << 29, F_RMW = 1u << 30, F_ATOMIC = 1u << 31;
 
 __device__ __forceinline__ uint32_t round_flags(int r) {
@@ -133,10 +135,10 @@ __global__ void __launch_bounds__(64) k_reg_lists(const uint32_t* __restrict__ h
         L[int64_t(base[rr]) * 64 + slot] =
             uint32_t(B2 + (slot & 63)) | (uint32_t(s0) << 13) | round_flags(rr);
     }
+    if (lane == 0) ngroups[g * MAXSEG + s0 / SEG] = acc - out;
     out = acc;
     __syncthreads();
   }
-  if (lane == 0) ngroups[g] = out;
 }
 
 __device__ __forceinline__ void wht_reg(float (&x)[R]) {
@@ -165,55 +167,87 @@ __device__ __forceinline__ void stage_row(const float* __restrict__ row, int key
 }
 
 // scatter one row into S following its schedule, read the sketch into x,
-// full WHT of x (the transpose leaves S dirty; the next scatter zeroes it)
-constexpr int GB = 16;  // schedule entries prefetched per batch
+// full WHT of x (the transpose leaves S dirty; the next scatter zeroes it).
+// Groups run in batches of GB: the batch's schedule entries and staged
+// values are all loaded before its LDS writes, so the gathers overlap.
+// On return lane L, register r holds frequency f_of(L, r).
+constexpr int GB = 16;
 
 __device__ __forceinline__ void sketch_transform(const float* __restrict__ row, int keys,
-                                                 const uint32_t* __restrict__ list, int ng,
-                                                 float* S, float* Rb, int lane, float (&x)[R]) {
+                                                 const uint32_t* __restrict__ list,
+                                                 const int* __restrict__ segcnt, float* S,
+                                                 float* Rb, int lane, float (&x)[R]) {
   float4* S4 = reinterpret_cast<float4*>(S);
 #pragma unroll
   for (int j = 0; j < B2 / 4 / 64; ++j) S4[j * 64 + lane] = make_float4(0.f, 0.f, 0.f, 0.f);
-  int cur = -1;
-  for (int g0 = 0; g0 < ng; g0 += GB) {
-    uint32_t ent[GB];
+  const int nseg = (keys + SEG - 1) / SEG;
+  int gbase = 0;
+  for (int seg = 0; seg < nseg; ++seg) {
+    const int ngs = segcnt[seg];
+    __syncthreads();
+    stage_row(row, keys, seg, Rb, lane);
+    __syncthreads();
+    for (int g0 = 0; g0 < ngs; g0 += GB) {
+      const int nb = min(GB, ngs - g0);
+      uint32_t ent[GB];
+      float v[GB];
 #pragma unroll
-    for (int q = 0; q < GB; ++q) ent[q] = list[int64_t(min(g0 + q, ng - 1)) * 64 + lane];
+      for (int q = 0; q < GB; ++q) ent[q] = list[int64_t(gbase + g0 + min(q, nb - 1)) * 64 + lane];
 #pragma unroll
-    for (int q = 0; q < GB; ++q) {
-      if (g0 + q >= ng) break;
-      const uint32_t e = ent[q];
-      const uint32_t first = __builtin_amdgcn_readfirstlane(e);
-      const int seg = int(((first >> 13) & 0xffffu) / SEG);
-      if (seg != cur) {
-        __syncthreads();
-        stage_row(row, keys, seg, Rb, lane);
-        __syncthreads();
-        cur = seg;
-      }
-      const uint32_t dst = e & 0x1fffu;
-      float v = flip(Rb[((e >> 13) & 0xffffu) % SEG], (e & F_SIGN) << 2);
-      if (first & F_ATOMIC) {
-        __hip_atomic_fetch_add(&S[dst], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-      } else {
-        if (first & F_RMW) v += S[dst];
-        S[dst] = v;
+      for (int q = 0; q < GB; ++q) v[q] = flip(Rb[((ent[q] >> 13) & 0xffffu) % SEG], (ent[q] & F_SIGN) << 2);
+#pragma unroll
+      for (int q = 0; q < GB; ++q) {
+        if (q < nb) {
+          const uint32_t first = __builtin_amdgcn_readfirstlane(ent[q]);
+          const uint32_t dst = ent[q] & 0x1fffu;
+          if (first & F_ATOMIC) {
+            __hip_atomic_fetch_add(&S[dst], v[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          } else {
+            float w = v[q];
+            if (first & F_RMW) w += S[dst];
+            S[dst] = w;
+          }
+        }
       }
     }
+    gbase += ngs;
   }
   __syncthreads();
 #pragma unroll
   for (int r = 0; r < R; ++r) x[r] = S[r * 64 + lane];
   __syncthreads();
-  wht_reg(x);  // register bits (index bits 6..11)
-  // transpose: element (reg r, lane L) -> (reg L, lane r)
+  wht_reg(x);  // register bits: index bits 6..10
+  // transpose: element e = r*64 + L goes to row e >> 5, column e & 31; lane
+  // L then reads row ((L & 31) << 1) | (L >> 5), i.e. index bits 0..4 land in
+  // registers, bits 6..10 in lane bits 0..4 and bit 5 in lane bit 5
+  {
+    float* wbase = S + (lane >> 5) * TROW + (lane & 31);
 #pragma unroll
-  for (int r = 0; r < R; ++r) S[r * SROW + lane] = x[r];
+    for (int r = 0; r < R; ++r) wbase[r * 2 * TROW] = x[r];
+  }
   __syncthreads();
+  {
+    const float* rbase = S + (((lane & 31) << 1) | (lane >> 5)) * TROW;
 #pragma unroll
-  for (int r = 0; r < R; ++r) x[r] = S[lane * SROW + r];
+    for (int r = 0; r < R; ++r) x[r] = rbase[r];
+  }
   __syncthreads();
-  wht_reg(x);  // former lane bits (index bits 0..5)
+  wht_reg(x);  // index bits 0..4
+  // index bit 5 sits in lane bit 5: swap halves between register pairs and
+  // butterfly (lanes < 32 finish register 2j's pairs, lanes >= 32 2j+1's)
+#pragma unroll
+  for (int j = 0; j < R / 2; ++j) {
+    const auto pq = __builtin_amdgcn_permlane32_swap(__float_as_uint(x[2 * j]),
+                                                     __float_as_uint(x[2 * j + 1]), false, false);
+    const float p = __uint_as_float(pq[0]), q = __uint_as_float(pq[1]);
+    x[2 * j] = p + q;
+    x[2 * j + 1] = p - q;
+  }
+}
+
+// frequency held by lane L, register r after sketch_transform
+__device__ __forceinline__ int f_of(int L, int r) {
+  return ((L & 31) << 6) | ((r & 1) << 5) | ((r & ~1) + (L >> 5));
 }
 
 __global__ void __launch_bounds__(64) k_compress_fwht_reg(CompressArgs a,
@@ -242,24 +276,27 @@ __global__ void __launch_bounds__(64) k_compress_fwht_reg(CompressArgs a,
   const float* __restrict__ B = static_cast<const float*>(a.B);
   const uint32_t* __restrict__ la = LA + ts * cap_a;
   const uint32_t* __restrict__ lb = LB + ts * cap_b;
-  const int nga = ngroups[ts];
-  const int ngb = ngroups[per_chunk + ts];
+  const int* __restrict__ sca = ngroups + int64_t(ts) * MAXSEG;
+  const int* __restrict__ scb = ngroups + int64_t(per_chunk + ts) * MAXSEG;
 
   float acc[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) acc[r] = 0.f;
   for (int k = k0; k < k1; ++k) {
     float xa[R], xb[R];
-    sketch_transform(AT + int64_t(k) * a.ld_at, a.m, la, nga, S, Rb, lane, xa);
-    sketch_transform(B + int64_t(k) * a.ld_b, a.p, lb, ngb, S, Rb, lane, xb);
+    sketch_transform(AT + int64_t(k) * a.ld_at, a.m, la, sca, S, Rb, lane, xa);
+    sketch_transform(B + int64_t(k) * a.ld_b, a.p, lb, scb, S, Rb, lane, xb);
 #pragma unroll
     for (int r = 0; r < R; ++r) acc[r] = fmaf(xa[r], xb[r], acc[r]);
   }
-  // lane L, register r holds frequency L*64 + r of this slice
-  float4* out = reinterpret_cast<float4*>(static_cast<float*>(a.partial) +
-                                          (int64_t(chunk) * per_chunk + ts) * B2 + lane * R);
+  // reorder through LDS to natural frequency order, then coalesced stores
+  __syncthreads();
 #pragma unroll
-  for (int r = 0; r < R; r += 4) out[r / 4] = make_float4(acc[r], acc[r + 1], acc[r + 2], acc[r + 3]);
+  for (int r = 0; r < R; ++r) S[f_of(lane, r)] = acc[r];
+  __syncthreads();
+  float* out = static_cast<float*>(a.partial) + (int64_t(chunk) * per_chunk + ts) * B2;
+#pragma unroll
+  for (int r = 0; r < R; ++r) out[r * 64 + lane] = S[r * 64 + lane];
 }
 
 int64_t list_cap(int keys) {
@@ -270,14 +307,24 @@ int64_t list_cap(int keys) {
 }  // namespace
 
 int compress_fwht_reg_supported(int lb, int m, int p) {
-  return lb >= LB2 && lb <= 30 && m <= 65536 && p <= 65536;
+  return lb >= LB2 && lb <= 30 && m <= MAXSEG * SEG && p <= MAXSEG * SEG;
 }
 
 int compress_fwht_reg_lb2() { return LB2; }
 
+int compress_fwht_reg_slots() {
+  int dev = 0, cus = 0, per_cu = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_compress_fwht_reg, 64, 0) != hipSuccess ||
+      cus <= 0 || per_cu <= 0)
+    return 2048;
+  return cus * per_cu;
+}
+
 size_t compress_fwht_reg_entries_bytes(int d, int lb, int m, int p) {
   const size_t lists = size_t(d) << (lb - LB2);
-  return lists * size_t(list_cap(m) + list_cap(p)) * 4 + 2 * lists * sizeof(int) + 256;
+  return lists * size_t(list_cap(m) + list_cap(p)) * 4 + 2 * lists * MAXSEG * sizeof(int) + 256;
 }
 
 hipError_t launch_compress_fwht_reg(const CompressArgs& a, void* entries, hipStream_t s) {

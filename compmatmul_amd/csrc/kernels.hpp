@@ -47,6 +47,7 @@ hipError_t launch_compress_lds(const CompressArgs& a, hipStream_t s);
 // compress_fwht_reg_entries_bytes() bytes for the packed scatter entries.
 int compress_fwht_reg_supported(int lb, int m, int p);
 int compress_fwht_reg_lb2();
+int compress_fwht_reg_slots();  // resident workgroups on the device
 size_t compress_fwht_reg_entries_bytes(int d, int lb, int m, int p);
 hipError_t launch_compress_fwht_reg(const CompressArgs& a, void* entries, hipStream_t s);
 
