@@ -28,17 +28,23 @@ sys.path.insert(0, HERE)
 import compmatmul_amd as cma  # noqa: E402
 from compmatmul_amd import workloads  # noqa: E402
 
-WORKLOAD = "fwht2048_fp32"
+DEFAULT_WORKLOAD = "fwht2048_fp32"
 HASH_SEED = 13794
 PEAK_FP32_TFLOPS = 157.3
+PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
 
 
-def fwht_flops(m, n, p, d, b):
-    """Algorithmic FLOPs of the compress kernel (sketch + 2 forward WHTs per
-    (t, k) + spectral MAC); the d inverse transforms run in the reduce kernel."""
+def compress_flops(variant, m, n, p, d, b):
+    """Algorithmic FLOPs of the compress kernel: sketch + forward transforms
+    per (t, k) + spectral MAC (the d inverse transforms run in the reduce
+    kernel).  FWHT: 2dn(m+p) + 2dn*b*log2(b) + 2dn*b.  FFT (two real
+    transforms of length b per (t, k), 2.5 b log2 b flops each, complex MAC
+    over b/2+1 bins): 2dn(m+p) + 2dn*2.5*b*log2(b) + 8dn(b/2+1)."""
     lb = int(np.log2(b))
-    return 2 * d * n * (m + p) + 2 * d * n * b * lb + 2 * d * n * b
+    if variant == "fwht":
+        return 2 * d * n * (m + p) + 2 * d * n * b * lb + 2 * d * n * b
+    return 2 * d * n * (m + p) + 2 * d * n * 2.5 * b * lb + 8 * d * n * (b // 2 + 1)
 
 
 def cpu_baseline(c, AT, B):
@@ -74,6 +80,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--plain", action="store_true", help="time the unfused path instead")
+    ap.add_argument("--workload", default=DEFAULT_WORKLOAD, choices=sorted(workloads.CONFIGS),
+                    help="named workload (default: the metric's configuration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -81,20 +89,22 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
-    c = workloads.CONFIGS[WORKLOAD]
+    name = args.workload
+    c = workloads.CONFIGS[name]
     m, n, p, d, b = c["m"], c["n"], c["p"], c["d"], c["b"]
-    AT, B, _ = workloads.make(WORKLOAD)
-    print(f"[bench] workload {WORKLOAD} m={m} n={n} p={p} d={d} b={b}", file=sys.stderr,
+    dtype = np.dtype(c["dtype"])
+    AT, B, _ = workloads.make(name)
+    print(f"[bench] workload {name} m={m} n={n} p={p} d={d} b={b}", file=sys.stderr,
           flush=True)
     dAT = cma.DeviceArray.from_numpy(AT)
     dB = cma.DeviceArray.from_numpy(B)
-    dC = cma.DeviceArray((m, p), np.float32)
+    dC = cma.DeviceArray((m, p), dtype)
     flags = cma.FLAG_PLAIN if args.plain else 0
     sk = None
 
     def step():
         nonlocal sk
-        sk = cma.compress(c["variant"], dAT, dB, d, b, seed=HASH_SEED, dtype=np.float32,
+        sk = cma.compress(c["variant"], dAT, dB, d, b, seed=HASH_SEED, dtype=dtype,
                           flags=flags, sketch=sk)
         sk.decompress(dC)
 
@@ -129,7 +139,8 @@ def main():
     precision = len(got & truth) / max(1, len(got))
 
     ms_per_step = 1e3 * elapsed / args.steps
-    flops = fwht_flops(m, n, p, d, b)
+    flops = compress_flops(c["variant"], m, n, p, d, b)
+    peak = PEAK_FP32_TFLOPS if dtype == np.float32 else PEAK_FP64_TFLOPS
     k_avg_s = (k_ms / max(1, k_n)) * 1e-3
     achieved = flops / k_avg_s / 1e12 if k_avg_s > 0 else 0.0
     out = {
@@ -143,21 +154,23 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic: N(0,1/2048) noise + 128 planted entries of 6.0 (seeded)",
-        "config": {"workload": "fwht 2048x2048x2048 fp32, b=2^14, d=15, 128 plants, tau=18",
-                   "m": m, "n": n, "p": p, "b": b, "d": d, "variant": "fwht",
+        "dtype": "f32" if dtype == np.float32 else "f64",
+        "data": f"synthetic: N(0,1/{n}) noise + {c['plants']} planted entries of "
+                f"{c['magnitude']} (seeded)",
+        "config": {"workload": f"{c['variant']} {m}x{n}x{p} {dtype.name}, b={b}, d={d}, "
+                               f"{c['plants']} plants, tau={c['tau']}",
+                   "m": m, "n": n, "p": p, "b": b, "d": d, "variant": c["variant"],
                    "hash": "multiply add shift", "path": "plain" if args.plain else "fused",
                    "parallelism": "single GPU"},
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak,
+                     "unit": "TFLOP/s", "frac": achieved / peak,
                      "traffic": None, "kernel": kname,
                      "kernel_ms": k_avg_s * 1e3, "flops_per_launch": flops},
         "detail": {"recall": recall, "precision": precision, "recovered": len(got),
                    "truth": len(truth), "C_threshold_set_equal": est_set == got,
                    "kernel_ms": {k: (v[0] / v[1] if v[1] else None) for k, v in totals.items()}},
     }
-    if not args.no_cpu_baseline:
+    if not args.no_cpu_baseline and name == DEFAULT_WORKLOAD:
         print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(c, AT, B)
     print(json.dumps(out), flush=True)

@@ -29,7 +29,7 @@ HIP_FLAGS = [
     "-Wall", "-Wno-unused-command-line-argument",
 ]
 
-SOURCES = ["hash.hip", "compress.hip", "compress_reg.hip", "decompress.hip",
+SOURCES = ["hash.hip", "compress.hip", "compress_reg.hip", "compress_fft_reg.hip", "decompress.hip",
            "plain.hip", "api.cpp", "host_hash.cpp"]
 
 

@@ -53,6 +53,7 @@ struct State {
   int device = -1;
   hipStream_t stream = nullptr;
   DevBuf partial, V, params, tables, AT, B, C, counter, oi, oj, ov, plainA, plainB, entries;
+  void* fft_t2 = nullptr;  // W_2048^(L bitrev5(q)) table of the FFT register kernel
   void* pinned = nullptr;
   size_t pinned_n = 0;
   std::map<std::pair<int, int>, void*> twiddles;  // (log2 b, f64) -> table
@@ -236,6 +237,27 @@ int twiddles(int lb, int f64, const void** out) {
   CM_HIP(hipMemcpy(dev, host.data(), host.size(), hipMemcpyHostToDevice));
   g.twiddles[key] = dev;
   *out = dev;
+  return CM_OK;
+}
+
+// Table of the FFT register kernel: W_2048^(L * bitrev5(q)) at [q*64 + L].
+int fft_t2_table(const void** out) {
+  if (!g.fft_t2) {
+    std::vector<float> host(2 * 2048);
+    for (int q = 0; q < 32; ++q) {
+      int br = 0;
+      for (int bit = 0; bit < 5; ++bit) br |= ((q >> bit) & 1) << (4 - bit);
+      for (int L = 0; L < 64; ++L) {
+        const int j = (L * br) % 2048;
+        const double ang = -2.0 * M_PI * double(j) / 2048.0;
+        host[2 * (q * 64 + L)] = float(std::cos(ang));
+        host[2 * (q * 64 + L) + 1] = float(std::sin(ang));
+      }
+    }
+    CM_HIP(hipMalloc(&g.fft_t2, host.size() * 4));
+    CM_HIP(hipMemcpy(g.fft_t2, host.data(), host.size() * 4, hipMemcpyHostToDevice));
+  }
+  *out = g.fft_t2;
   return CM_OK;
 }
 
@@ -431,6 +453,51 @@ int compress_device(cm_sketch* sk, const void* AT, int64_t ld_at, const void* B,
     cm::ReduceArgs ra{1, 0, f64_reduce, g.partial.p, int(nchunks), d, lb, lb2, nullptr, g.V.p};
     if ((rc = timed("reduce", [&] { return cm::launch_reduce_inverse(ra, g.stream); }))) return rc;
     cm::CombineArgs ca{1, 0, g.V.p, d, lb, lb2, nullptr, sk->P};
+    return timed("combine", [&] { return cm::launch_combine(ca, g.stream); });
+  }
+
+  // fast path: FP32 FFT with b >= 2048 runs the register-resident kernel
+  const bool freg = !fwht && !f64 && !(flags & CM_FLAG_LDS_KERNEL) && cm::compress_fft_reg_supported(lb, m, p);
+  if (freg) {
+    const int lb2 = cm::compress_fft_reg_lb2();
+    const int64_t per_chunk = int64_t(d) << (lb - lb2);
+    const int64_t slots = cm::compress_fft_reg_slots();
+    const int64_t rounds = std::max<int64_t>(1, (int64_t(n) * per_chunk + slots * 64 - 1) / (slots * 64));
+    int64_t nchunks = std::max<int64_t>(1, slots * rounds / per_chunk);
+    nchunks = std::min<int64_t>(nchunks, std::max(1, n));
+    const size_t chunk_bytes = size_t(d) * b * 2 * esz;
+    nchunks = std::max<int64_t>(1, std::min<int64_t>(nchunks, int64_t((size_t(1) << 30) / chunk_bytes)));
+    const int kper = n > 0 ? int((n + nchunks - 1) / nchunks) : 0;
+    nchunks = n > 0 ? (n + kper - 1) / kper : 0;
+    const void* t2 = nullptr;
+    int rc = fft_t2_table(&t2);
+    if (rc) return rc;
+    if ((rc = ensure_buf(g.partial, std::max<size_t>(1, size_t(nchunks) * chunk_bytes)))) return rc;
+    if ((rc = ensure_buf(g.V, chunk_bytes))) return rc;
+    if ((rc = ensure_buf(g.entries, cm::compress_fft_reg_scratch_bytes(d, lb, m, p)))) return rc;
+    cm::CompressArgs caa{};
+    caa.AT = AT;
+    caa.ld_at = ld_at;
+    caa.B = B;
+    caa.ld_b = ld_b;
+    caa.hA = sk->hA;
+    caa.hB = sk->hB;
+    caa.m = m;
+    caa.n = n;
+    caa.p = p;
+    caa.d = d;
+    caa.lb = lb;
+    caa.lb2 = lb2;
+    caa.nchunks = int(nchunks);
+    caa.kper = kper;
+    caa.tw = tw;
+    caa.partial = g.partial.p;
+    if ((rc = timed("compress", [&] { return cm::launch_compress_fft_reg(caa, g.entries.p, t2, g.stream); })))
+      return rc;
+    const int f64_reduce = (flags & CM_FLAG_F64_REDUCE) || n > 16384;
+    cm::ReduceArgs ra{0, 0, f64_reduce, g.partial.p, int(nchunks), d, lb, lb2, tw, g.V.p};
+    if ((rc = timed("reduce", [&] { return cm::launch_reduce_inverse(ra, g.stream); }))) return rc;
+    cm::CombineArgs ca{0, 0, g.V.p, d, lb, lb2, tw, sk->P};
     return timed("combine", [&] { return cm::launch_combine(ca, g.stream); });
   }
 

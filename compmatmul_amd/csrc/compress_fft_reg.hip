@@ -1,0 +1,421 @@
+// Register-resident single-wave FP32 FFT compress (the fast path for the
+// cyclic-convolution variant).
+//
+// Same mathematics as the LDS kernel in compress.hip: per (repetition t,
+// slice, chunk of k) the pair-packed sketch z = S a_k + i S' b_k of the slice
+// (modulated by W_b^(h*slice)) is transformed with a 2048-point DFT, and
+// Q += z^2 is accumulated (the product spectrum is recovered from Q in the
+// reduce kernel).  Layout for one 64-lane wavefront:
+//
+//  * Element n = L + 64 r (lane L, register r < 32) after the sketch is read
+//    from LDS.  Four-step DFT with n = n1 + 64 n2, k = k2 + 32 k1:
+//      1. per lane, DFT-32 over the registers (radix-2 DIF, compile-time
+//         twiddles; register q then holds k2 = bitrev5(q));
+//      2. twiddle by W_2048^(n1 k2) (per-lane table, L1-resident);
+//      3. transpose through LDS (rows of 65 complex: conflict-free, one base
+//         address plus immediates): lane L' gets k2 = L' & 31 and the 32
+//         values n1 = r' + 32 (L' >> 5);
+//      4. DFT-64 over n1: one radix-2 DIF stage across lanes L' and L'+32
+//         with v_permlane32_swap, a second swap that regroups the halves,
+//         then a DFT-32 per lane (lanes < 32 give even k1, lanes >= 32 odd).
+//  * Q (64 VGPRs) stays in registers for the chunk of k; the final sum is
+//    reordered to natural frequency order through LDS.
+//  * The scatter of the m + p nonzeros follows a per-(t, slice) schedule of
+//    conflict-free plain LDS writes with per-item complex coefficients
+//    s * W_b^(h*slice) (times i for B), built once per product.
+#include "device_common.hpp"
+#include "kernels.hpp"
+
+namespace cm {
+
+namespace {
+
+constexpr int LB2 = 11;          // slice length 2048 (complex)
+constexpr int B2 = 1 << LB2;
+constexpr int R = 32;            // complex values per lane
+constexpr int TROW = 65;         // padded transpose row (complex)
+constexpr int SLEN = 2112;       // complex slots per wave: transpose (32 x 65) and dummy slots
+constexpr int SEG = 1024;
+constexpr int RCAP = 8;
+constexpr int MAXSEG = 128;      // m + p <= MAXSEG * SEG = 131072
+constexpr uint32_t F_RMW = 1u << 30, F_ATOMIC = 1u << 31;
+
+// exp(-2 pi i j / N), rounded to float
+constexpr float W32R[32] = {1.000000000e+00f, 9.807852804e-01f, 9.238795325e-01f, 8.314696123e-01f, 7.071067812e-01f, 5.555702330e-01f, 3.826834324e-01f, 1.950903220e-01f, 0.0f, -1.950903220e-01f, -3.826834324e-01f, -5.555702330e-01f, -7.071067812e-01f, -8.314696123e-01f, -9.238795325e-01f, -9.807852804e-01f, -1.000000000e+00f, -9.807852804e-01f, -9.238795325e-01f, -8.314696123e-01f, -7.071067812e-01f, -5.555702330e-01f, -3.826834324e-01f, -1.950903220e-01f, 0.0f, 1.950903220e-01f, 3.826834324e-01f, 5.555702330e-01f, 7.071067812e-01f, 8.314696123e-01f, 9.238795325e-01f, 9.807852804e-01f};
+constexpr float W32I[32] = {0.0f, -1.950903220e-01f, -3.826834324e-01f, -5.555702330e-01f, -7.071067812e-01f, -8.314696123e-01f, -9.238795325e-01f, -9.807852804e-01f, -1.000000000e+00f, -9.807852804e-01f, -9.238795325e-01f, -8.314696123e-01f, -7.071067812e-01f, -5.555702330e-01f, -3.826834324e-01f, -1.950903220e-01f, 0.0f, 1.950903220e-01f, 3.826834324e-01f, 5.555702330e-01f, 7.071067812e-01f, 8.314696123e-01f, 9.238795325e-01f, 9.807852804e-01f, 1.000000000e+00f, 9.807852804e-01f, 9.238795325e-01f, 8.314696123e-01f, 7.071067812e-01f, 5.555702330e-01f, 3.826834324e-01f, 1.950903220e-01f};
+constexpr float W64R[64] = {1.000000000e+00f, 9.951847267e-01f, 9.807852804e-01f, 9.569403357e-01f, 9.238795325e-01f, 8.819212643e-01f, 8.314696123e-01f, 7.730104534e-01f, 7.071067812e-01f, 6.343932842e-01f, 5.555702330e-01f, 4.713967368e-01f, 3.826834324e-01f, 2.902846773e-01f, 1.950903220e-01f, 9.801714033e-02f, 0.0f, -9.801714033e-02f, -1.950903220e-01f, -2.902846773e-01f, -3.826834324e-01f, -4.713967368e-01f, -5.555702330e-01f, -6.343932842e-01f, -7.071067812e-01f, -7.730104534e-01f, -8.314696123e-01f, -8.819212643e-01f, -9.238795325e-01f, -9.569403357e-01f, -9.807852804e-01f, -9.951847267e-01f, -1.000000000e+00f, -9.951847267e-01f, -9.807852804e-01f, -9.569403357e-01f, -9.238795325e-01f, -8.819212643e-01f, -8.314696123e-01f, -7.730104534e-01f, -7.071067812e-01f, -6.343932842e-01f, -5.555702330e-01f, -4.713967368e-01f, -3.826834324e-01f, -2.902846773e-01f, -1.950903220e-01f, -9.801714033e-02f, 0.0f, 9.801714033e-02f, 1.950903220e-01f, 2.902846773e-01f, 3.826834324e-01f, 4.713967368e-01f, 5.555702330e-01f, 6.343932842e-01f, 7.071067812e-01f, 7.730104534e-01f, 8.314696123e-01f, 8.819212643e-01f, 9.238795325e-01f, 9.569403357e-01f, 9.807852804e-01f, 9.951847267e-01f};
+constexpr float W64I[64] = {0.0f, -9.801714033e-02f, -1.950903220e-01f, -2.902846773e-01f, -3.826834324e-01f, -4.713967368e-01f, -5.555702330e-01f, -6.343932842e-01f, -7.071067812e-01f, -7.730104534e-01f, -8.314696123e-01f, -8.819212643e-01f, -9.238795325e-01f, -9.569403357e-01f, -9.807852804e-01f, -9.951847267e-01f, -1.000000000e+00f, -9.951847267e-01f, -9.807852804e-01f, -9.569403357e-01f, -9.238795325e-01f, -8.819212643e-01f, -8.314696123e-01f, -7.730104534e-01f, -7.071067812e-01f, -6.343932842e-01f, -5.555702330e-01f, -4.713967368e-01f, -3.826834324e-01f, -2.902846773e-01f, -1.950903220e-01f, -9.801714033e-02f, 0.0f, 9.801714033e-02f, 1.950903220e-01f, 2.902846773e-01f, 3.826834324e-01f, 4.713967368e-01f, 5.555702330e-01f, 6.343932842e-01f, 7.071067812e-01f, 7.730104534e-01f, 8.314696123e-01f, 8.819212643e-01f, 9.238795325e-01f, 9.569403357e-01f, 9.807852804e-01f, 9.951847267e-01f, 1.000000000e+00f, 9.951847267e-01f, 9.807852804e-01f, 9.569403357e-01f, 9.238795325e-01f, 8.819212643e-01f, 8.314696123e-01f, 7.730104534e-01f, 7.071067812e-01f, 6.343932842e-01f, 5.555702330e-01f, 4.713967368e-01f, 3.826834324e-01f, 2.902846773e-01f, 1.950903220e-01f, 9.801714033e-02f};
+
+__device__ __forceinline__ uint32_t round_flags(int r) {
+  return r == 0 ? 0u : (r < RCAP - 1 ? F_RMW : F_ATOMIC);
+}
+
+// ---------------------------------------------------------------- schedules
+// One list per (t, slice) over the concatenated keys [0, m + p): key i < m is
+// column i of A (coefficient s1(i) W_b^(h1(i) slice)), key m + j is row j of
+// B (coefficient i s2(j) W_b^(h2(j) slice)).  Entry: dst (12 bits: bucket
+// mod 2048 or dummy slot 2048 + lane) | key (17 bits) << 12 | RMW | ATOMIC.
+__global__ void __launch_bounds__(64) k_fft_lists(const uint32_t* __restrict__ hA,
+                                                  const uint32_t* __restrict__ hB, int m, int p,
+                                                  int lb, int lbs, const float2* __restrict__ tw,
+                                                  int64_t cap, uint32_t* __restrict__ L,
+                                                  float2* __restrict__ C,
+                                                  int* __restrict__ segcnt) {
+  __shared__ uint32_t cnt[B2];
+  __shared__ uint8_t rnd[SEG];
+  __shared__ int nr[RCAP];
+  const int b1 = 1 << lbs;
+  const int ts = blockIdx.x;
+  const int t = ts >> lbs;
+  const uint64_t slice = uint64_t(ts & (b1 - 1));
+  const uint64_t bmask = (uint64_t(1) << lb) - 1;
+  const int keys = m + p;
+  uint32_t* __restrict__ Ls = L + ts * cap;
+  float2* __restrict__ Cs = C + ts * cap;
+  const int lane = threadIdx.x;
+  for (int e = lane; e < B2; e += 64) cnt[e] = 0;
+  int out = 0;
+  for (int s0 = 0; s0 < keys; s0 += SEG) {
+    const int sl = min(SEG, keys - s0);
+    if (lane < RCAP) nr[lane] = 0;
+    __syncthreads();
+    for (int c = 0; c < sl; c += 64) {
+      if (c + lane < sl) {
+        const int key = s0 + c + lane;
+        const uint32_t e = key < m ? hA[int64_t(t) * m + key] : hB[int64_t(t) * p + key - m];
+        const int r = min(int(atomicAdd(&cnt[hbucket(e) & (B2 - 1)], 1u)), RCAP - 1);
+        rnd[c + lane] = uint8_t(r);
+        atomicAdd(&nr[r], 1);
+      }
+      __syncthreads();
+    }
+    int base[RCAP], run[RCAP];
+    int acc = out;
+#pragma unroll
+    for (int r = 0; r < RCAP; ++r) {
+      base[r] = acc;
+      acc += (nr[r] + 63) / 64;
+      run[r] = 0;
+    }
+    for (int c = 0; c < sl; c += 64) {
+      const bool ok = c + lane < sl;
+      const int r = ok ? int(rnd[c + lane]) : -1;
+      uint32_t ent = 0;
+      float2 coef = make_float2(0.f, 0.f);
+      if (ok) {
+        const int key = s0 + c + lane;
+        const bool isb = key >= m;
+        const uint32_t e = isb ? hB[int64_t(t) * p + key - m] : hA[int64_t(t) * m + key];
+        const uint32_t h = hbucket(e);
+        const float2 w = tw[(uint64_t(h) * slice) & bmask];
+        const float s = hneg(e) ? -1.f : 1.f;
+        coef = isb ? make_float2(-s * w.y, s * w.x) : make_float2(s * w.x, s * w.y);
+        ent = (h & (B2 - 1)) | (uint32_t(key) << 12) | round_flags(r);
+      }
+#pragma unroll
+      for (int rr = 0; rr < RCAP; ++rr) {
+        const uint64_t mask = __ballot(r == rr);
+        if (r == rr) {
+          const int64_t pos = int64_t(base[rr]) * 64 + run[rr] + __popcll(mask & ((1ull << lane) - 1ull));
+          Ls[pos] = ent;
+          Cs[pos] = coef;
+        }
+        run[rr] += __popcll(mask);
+      }
+    }
+#pragma unroll
+    for (int rr = 0; rr < RCAP; ++rr) {
+      const int full = (nr[rr] + 63) / 64 * 64;
+      for (int slot = nr[rr] + lane; slot < full; slot += 64) {
+        const int64_t pos = int64_t(base[rr]) * 64 + slot;
+        Ls[pos] = uint32_t(B2 + (slot & 63)) | (uint32_t(s0) << 12) | round_flags(rr);
+        Cs[pos] = make_float2(0.f, 0.f);
+      }
+    }
+    if (lane == 0) segcnt[int64_t(ts) * MAXSEG + s0 / SEG] = acc - out;
+    out = acc;
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------- transforms
+// in-register radix-2 DIF DFT-32: register q ends holding X[bitrev5(q)]
+__device__ __forceinline__ void dft32(float (&re)[R], float (&im)[R]) {
+#pragma unroll
+  for (int half = R / 2; half >= 1; half >>= 1) {
+#pragma unroll
+    for (int blk = 0; blk < R; blk += 2 * half) {
+#pragma unroll
+      for (int j = 0; j < half; ++j) {
+        const int a = blk + j, b = a + half;
+        const float ur = re[a] + re[b], ui = im[a] + im[b];
+        const float vr = re[a] - re[b], vi = im[a] - im[b];
+        re[a] = ur;
+        im[a] = ui;
+        const int tj = j * (R / (2 * half));  // W_{2 half}^j = W_32^tj
+        if (tj == 0) {
+          re[b] = vr;
+          im[b] = vi;
+        } else if (tj == R / 4) {  // -i
+          re[b] = vi;
+          im[b] = -vr;
+        } else {
+          const float wr = W32R[tj], wi = W32I[tj];
+          re[b] = fmaf(vr, wr, -vi * wi);
+          im[b] = fmaf(vr, wi, vi * wr);
+        }
+      }
+    }
+  }
+}
+
+__device__ __forceinline__ int brev5(int q) { return int(__brev(uint32_t(q)) >> 27); }
+
+// stage keys [seg*SEG, seg*SEG + SEG) of the concatenated (A row, B row)
+__device__ __forceinline__ void stage_rows(const float* __restrict__ ar,
+                                           const float* __restrict__ br, int m, int keys,
+                                           int seg, float* Rb, int lane) {
+  const int s0 = seg * SEG;
+  const int sl = min(SEG, keys - s0);
+  float v[SEG / 64];
+#pragma unroll
+  for (int q = 0; q < SEG / 64; ++q) {
+    const int key = s0 + min(q * 64 + lane, sl - 1);
+    v[q] = key < m ? ar[key] : br[key - m];
+  }
+#pragma unroll
+  for (int q = 0; q < SEG / 64; ++q) Rb[q * 64 + lane] = v[q];
+}
+
+constexpr int GB = 8;
+
+// scatter + 2048-point DFT of the pair-packed slice sketch; on return lane L,
+// register q holds frequency fq_of(L, q)
+__device__ __forceinline__ void sketch_fft(const float* __restrict__ ar,
+                                           const float* __restrict__ br, int m, int keys,
+                                           const uint32_t* __restrict__ list,
+                                           const float2* __restrict__ coef,
+                                           const int* __restrict__ segcnt,
+                                           const float2* __restrict__ t2, float2* S, float* Rb,
+                                           int lane, float (&re)[R], float (&im)[R]) {
+  float4* S4 = reinterpret_cast<float4*>(S);
+#pragma unroll
+  for (int j = 0; j < B2 / 2 / 64; ++j) S4[j * 64 + lane] = make_float4(0.f, 0.f, 0.f, 0.f);
+  const int nseg = (keys + SEG - 1) / SEG;
+  int gbase = 0;
+  for (int seg = 0; seg < nseg; ++seg) {
+    const int ngs = segcnt[seg];
+    __syncthreads();
+    stage_rows(ar, br, m, keys, seg, Rb, lane);
+    __syncthreads();
+    for (int g0 = 0; g0 < ngs; g0 += GB) {
+      const int nb = min(GB, ngs - g0);
+      uint32_t ent[GB];
+      float2 cf[GB];
+      float v[GB];
+#pragma unroll
+      for (int q = 0; q < GB; ++q) {
+        const int64_t pos = int64_t(gbase + g0 + min(q, nb - 1)) * 64 + lane;
+        ent[q] = list[pos];
+        cf[q] = coef[pos];
+      }
+#pragma unroll
+      for (int q = 0; q < GB; ++q) v[q] = Rb[((ent[q] >> 12) & 0x1ffffu) % SEG];
+#pragma unroll
+      for (int q = 0; q < GB; ++q) {
+        if (q < nb) {
+          const uint32_t first = __builtin_amdgcn_readfirstlane(ent[q]);
+          const uint32_t dst = ent[q] & 0xfffu;
+          const float cr = v[q] * cf[q].x, ci = v[q] * cf[q].y;
+          if (first & F_ATOMIC) {
+            float* d = reinterpret_cast<float*>(&S[dst]);
+            __hip_atomic_fetch_add(d, cr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            __hip_atomic_fetch_add(d + 1, ci, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          } else if (first & F_RMW) {
+            const float2 o = S[dst];
+            S[dst] = make_float2(o.x + cr, o.y + ci);
+          } else {
+            S[dst] = make_float2(cr, ci);
+          }
+        }
+      }
+    }
+    gbase += ngs;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const float2 z = S[r * 64 + lane];
+    re[r] = z.x;
+    im[r] = z.y;
+  }
+  __syncthreads();
+  dft32(re, im);  // over n2; register q holds k2 = bitrev5(q)
+  // the table address is laundered so the compiler reloads the twiddles
+  // (L1-resident) instead of pinning 64 VGPRs across the k loop
+  asm volatile("" : "+v"(t2));
+#pragma unroll
+  for (int q = 0; q < R; ++q) {  // twiddle W_2048^(n1 k2)
+    const float2 w = t2[q * 64 + lane];
+    const float xr = re[q], xi = im[q];
+    re[q] = fmaf(xr, w.x, -xi * w.y);
+    im[q] = fmaf(xr, w.y, xi * w.x);
+  }
+  // transpose: (n1 = L, k2) -> row k2, column n1
+#pragma unroll
+  for (int q = 0; q < R; ++q) S[brev5(q) * TROW + lane] = make_float2(re[q], im[q]);
+  __syncthreads();
+  {
+    const float2* rbase = S + (lane & 31) * TROW + 32 * (lane >> 5);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const float2 z = rbase[r];
+      re[r] = z.x;
+      im[r] = z.y;
+    }
+  }
+  __syncthreads();
+  // radix-2 DIF stage over n1 bit 5 (lane bit 5): pairs (r', lane) and
+  // (r', lane ^ 32); lanes < 32 finish r' = 2j, lanes >= 32 r' = 2j + 1
+  // (the lane half is laundered so the per-lane twiddle selection is redone
+  // here rather than held in 64 VGPRs across the k loop)
+  int lh = lane;
+  asm volatile("" : "+v"(lh));
+  const bool hi = lh >= 32;
+#pragma unroll
+  for (int j = 0; j < R / 2; ++j) {
+    const auto pr = __builtin_amdgcn_permlane32_swap(__float_as_uint(re[2 * j]),
+                                                     __float_as_uint(re[2 * j + 1]), false, false);
+    const auto pi = __builtin_amdgcn_permlane32_swap(__float_as_uint(im[2 * j]),
+                                                     __float_as_uint(im[2 * j + 1]), false, false);
+    const float ar_ = __uint_as_float(pr[0]), br_ = __uint_as_float(pr[1]);
+    const float ai_ = __uint_as_float(pi[0]), bi_ = __uint_as_float(pi[1]);
+    const float vr = ar_ - br_, vi = ai_ - bi_;
+    const float wr = hi ? W64R[2 * j + 1] : W64R[2 * j];
+    const float wi = hi ? W64I[2 * j + 1] : W64I[2 * j];
+    re[2 * j] = ar_ + br_;  // even-k1 input at r' = 2j + hi
+    im[2 * j] = ai_ + bi_;
+    re[2 * j + 1] = fmaf(vr, wr, -vi * wi);  // odd-k1 input at r' = 2j + hi
+    im[2 * j + 1] = fmaf(vr, wi, vi * wr);
+  }
+  // regroup: lanes < 32 collect the even-k1 inputs for all r', lanes >= 32
+  // the odd-k1 inputs
+#pragma unroll
+  for (int j = 0; j < R / 2; ++j) {
+    const auto pr = __builtin_amdgcn_permlane32_swap(__float_as_uint(re[2 * j]),
+                                                     __float_as_uint(re[2 * j + 1]), false, false);
+    const auto pi = __builtin_amdgcn_permlane32_swap(__float_as_uint(im[2 * j]),
+                                                     __float_as_uint(im[2 * j + 1]), false, false);
+    re[2 * j] = __uint_as_float(pr[0]);
+    re[2 * j + 1] = __uint_as_float(pr[1]);
+    im[2 * j] = __uint_as_float(pi[0]);
+    im[2 * j + 1] = __uint_as_float(pi[1]);
+  }
+  dft32(re, im);  // over n1 mod 32; register q holds kappa = bitrev5(q)
+}
+
+// frequency of lane L, register q after sketch_fft: k2 + 32 k1 with
+// k2 = L & 31 and k1 = 2 bitrev5(q) + (L >> 5)
+__device__ __forceinline__ int fq_of(int L, int q) {
+  return (L & 31) + 32 * (2 * brev5(q) + (L >> 5));
+}
+
+__global__ void __launch_bounds__(64) k_compress_fft_reg(CompressArgs a,
+                                                         const uint32_t* __restrict__ L,
+                                                         const float2* __restrict__ C,
+                                                         int64_t cap,
+                                                         const int* __restrict__ segcnt,
+                                                         const float2* __restrict__ t2,
+                                                         int units) {
+  __shared__ __align__(16) float2 S[SLEN];
+  __shared__ __align__(16) float Rb[SEG];
+  const int lbs = a.lb - LB2;
+  const int b1 = 1 << lbs;
+  const int G = gridDim.x;
+  const int g = blockIdx.x;
+  const int u = (g & 7) * (G >> 3) + (g >> 3);  // XCD-aware numbering
+  if (u >= units) return;
+  const int per_chunk = a.d * b1;
+  const int chunk = u / per_chunk;
+  const int ts = u - chunk * per_chunk;  // t * b1 + slice
+  const int lane = threadIdx.x;
+  const int k0 = chunk * a.kper;
+  const int k1 = min(a.n, k0 + a.kper);
+  const float* __restrict__ AT = static_cast<const float*>(a.AT);
+  const float* __restrict__ B = static_cast<const float*>(a.B);
+  const uint32_t* __restrict__ ls = L + ts * cap;
+  const float2* __restrict__ cs = C + ts * cap;
+  const int* __restrict__ sc = segcnt + int64_t(ts) * MAXSEG;
+  const int keys = a.m + a.p;
+
+  float qr[R], qi[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) qr[r] = qi[r] = 0.f;
+  for (int k = k0; k < k1; ++k) {
+    float re[R], im[R];
+    sketch_fft(AT + int64_t(k) * a.ld_at, B + int64_t(k) * a.ld_b, a.m, keys, ls, cs, sc, t2, S,
+               Rb, lane, re, im);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {  // Q += z^2
+      qr[r] += re[r] * re[r] - im[r] * im[r];
+      qi[r] = fmaf(2.f * re[r], im[r], qi[r]);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < R; ++r) S[fq_of(lane, r)] = make_float2(qr[r], qi[r]);
+  __syncthreads();
+  float2* out = reinterpret_cast<float2*>(static_cast<float*>(a.partial)) +
+                (int64_t(chunk) * per_chunk + ts) * B2;
+#pragma unroll
+  for (int r = 0; r < R; ++r) out[r * 64 + lane] = S[r * 64 + lane];
+}
+
+int64_t list_cap(int keys) {
+  const int64_t segs = (int64_t(keys) + SEG - 1) / SEG;
+  return int64_t(keys) + segs * 64 * RCAP;
+}
+
+}  // namespace
+
+int compress_fft_reg_supported(int lb, int m, int p) {
+  return lb >= LB2 && lb <= 30 && int64_t(m) + p <= int64_t(MAXSEG) * SEG;
+}
+
+int compress_fft_reg_lb2() { return LB2; }
+
+int compress_fft_reg_slots() {
+  int dev = 0, cus = 0, per_cu = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_compress_fft_reg, 64, 0) != hipSuccess ||
+      cus <= 0 || per_cu <= 0)
+    return 2048;
+  return cus * per_cu;
+}
+
+size_t compress_fft_reg_scratch_bytes(int d, int lb, int m, int p) {
+  const size_t lists = size_t(d) << (lb - LB2);
+  return lists * size_t(list_cap(m + p)) * 12 + lists * MAXSEG * sizeof(int) +
+         size_t(B2) * sizeof(float2) + 256;
+}
+
+hipError_t launch_compress_fft_reg(const CompressArgs& a, void* scratch, const void* t2,
+                                   hipStream_t s) {
+  if (a.nchunks == 0) return hipSuccess;
+  const int lbs = a.lb - LB2;
+  const int lists = a.d << lbs;
+  const int64_t cap = list_cap(a.m + a.p);
+  float2* C = static_cast<float2*>(scratch);
+  uint32_t* L = reinterpret_cast<uint32_t*>(C + lists * cap);
+  int* sc = reinterpret_cast<int*>(L + lists * cap);
+  k_fft_lists<<<dim3(unsigned(lists)), dim3(64), 0, s>>>(
+      a.hA, a.hB, a.m, a.p, a.lb, lbs, static_cast<const float2*>(a.tw), cap, L, C, sc);
+  const int units = a.nchunks * lists;
+  const int grid = (units + 7) / 8 * 8;
+  k_compress_fft_reg<<<dim3(unsigned(grid)), dim3(64), 0, s>>>(
+      a, L, C, cap, sc, static_cast<const float2*>(t2), units);
+  return hipGetLastError();
+}
+
+}  // namespace cm

@@ -51,6 +51,17 @@ int compress_fwht_reg_slots();  // resident workgroups on the device
 size_t compress_fwht_reg_entries_bytes(int d, int lb, int m, int p);
 hipError_t launch_compress_fwht_reg(const CompressArgs& a, void* entries, hipStream_t s);
 
+// Register-resident single-wave FP32 FFT compress (fast path): slices of
+// 2^compress_fft_reg_lb2() frequencies; `scratch` holds the scatter
+// schedules (compress_fft_reg_scratch_bytes()); `t2` is the table
+// W_2048^(L * bitrev5(q)) at [q * 64 + L] (float2, 2048 entries).
+int compress_fft_reg_supported(int lb, int m, int p);
+int compress_fft_reg_lb2();
+int compress_fft_reg_slots();
+size_t compress_fft_reg_scratch_bytes(int d, int lb, int m, int p);
+hipError_t launch_compress_fft_reg(const CompressArgs& a, void* scratch, const void* t2,
+                                   hipStream_t s);
+
 // Sum of the per-chunk partial spectra, then the per-slice inverse transform.
 struct ReduceArgs {
   int fwht, f64, f64_reduce;

@@ -33,6 +33,8 @@ CONFIGS = {
                         b=4096, d=9, tau=8.0, dtype=np.float64),
     "fwht2048_fp32": dict(m=2048, n=2048, p=2048, plants=128, magnitude=6.0,
                           variant="fwht", b=1 << 14, d=15, tau=18.0, dtype=np.float32),
+    "fft2048_fp32": dict(m=2048, n=2048, p=2048, plants=128, magnitude=6.0, variant="fft",
+                         b=1 << 14, d=15, tau=18.0, dtype=np.float32),
     "fft4096x16384_fp32": dict(m=4096, n=16384, p=4096, plants=512, magnitude=5.0,
                                variant="fft", b=1 << 16, d=11, tau=12.5,
                                dtype=np.float32),

@@ -51,8 +51,8 @@ extern "C" {
 #define CM_FLAG_F64_REDUCE 1u /* FP32 path: sum per-tile spectra in FP64 */
 #define CM_FLAG_PLAIN 2u      /* unfused stage-by-stage path (sketches and
                                  spectra materialised in HBM) */
-#define CM_FLAG_LDS_KERNEL 4u /* FP32 FWHT: use the general LDS-resident fused
-                                 kernel instead of the register-resident one */
+#define CM_FLAG_LDS_KERNEL 4u /* FP32: use the general LDS-resident fused
+                                 kernel instead of the register-resident ones */
 
 typedef struct cm_sketch cm_sketch; /* device-resident compressed product */
 

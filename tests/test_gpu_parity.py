@@ -199,3 +199,27 @@ def test_fwht_register_kernel_matches_lds_kernel(m, n, p, b):
     want = orc.sketch_from_product("fwht", AT.astype(np.float64).T @ B.astype(np.float64),
                                    hA, hB, b)
     assert rel_l2(v1, want) <= 1e-5
+
+
+@pytest.mark.parametrize("m,n,p,b", [(2048, 40, 2048, 1 << 14), (100, 70, 37, 1 << 11),
+                                     (1, 5, 3, 1 << 13), (300, 33, 0, 1 << 12),
+                                     (60000, 3, 50, 1 << 11), (5000, 20, 4100, 1 << 16)])
+def test_fft_register_kernel_matches_lds_kernel(m, n, p, b):
+    print(f"fft reg vs lds m={m} n={n} p={p} b={b}", flush=True)
+    AT, B, _ = workloads.planted(m, n, max(p, 1), 4, 5.0, m + n + 1, np.float32)
+    B = np.ascontiguousarray(B[:, :p])
+    d = 5
+    reg = cma.compress("fft", AT, B, d, b, seed=11, dtype=np.float32)
+    lds = cma.compress("fft", AT, B, d, b, seed=11, dtype=np.float32,
+                       flags=cma.FLAG_LDS_KERNEL)
+    again = cma.compress("fft", AT, B, d, b, seed=11, dtype=np.float32)
+    v1, v2 = reg.values(), lds.values()
+    assert np.array_equal(v1, again.values())
+    hA, hB = reg.hashes()
+    want = orc.sketch_from_product("fft", AT.astype(np.float64).T @ B.astype(np.float64),
+                                   hA, hB, b)
+    if p == 0 or m == 0:
+        assert np.abs(v1).max() < 1e-6 and np.abs(v2).max() < 1e-6
+        return
+    assert rel_l2(v1, want) <= 1e-5
+    assert rel_l2(v2, want) <= 1e-5
