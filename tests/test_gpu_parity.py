@@ -219,7 +219,10 @@ def test_fft_register_kernel_matches_lds_kernel(m, n, p, b):
     want = orc.sketch_from_product("fft", AT.astype(np.float64).T @ B.astype(np.float64),
                                    hA, hB, b)
     if p == 0 or m == 0:
-        assert np.abs(v1).max() < 1e-6 and np.abs(v2).max() < 1e-6
+        # the pair packing extracts A*B by cancellation: with B empty the
+        # residue is rounding of |A|^2 (~1e-7 relative), not exactly zero
+        scale = float((AT.astype(np.float64) ** 2).sum(axis=1).max())
+        assert np.abs(v1).max() < 1e-6 * scale and np.abs(v2).max() < 1e-6 * scale
         return
     assert rel_l2(v1, want) <= 1e-5
     assert rel_l2(v2, want) <= 1e-5
