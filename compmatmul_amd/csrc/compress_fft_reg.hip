@@ -38,7 +38,6 @@ constexpr int SLEN = 2112;       // complex slots per wave: transpose (32 x 65) 
 constexpr int SEG = 1024;
 constexpr int RCAP = 8;
 constexpr int MAXSEG = 128;      // m + p <= MAXSEG * SEG = 131072
-constexpr uint32_t F_RMW = 1u << 30, F_ATOMIC = 1u << 31;
 
 // exp(-2 pi i j / N), rounded to float
 constexpr float W32R[32] = {1.000000000e+00f, 9.807852804e-01f, 9.238795325e-01f, 8.314696123e-01f, 7.071067812e-01f, 5.555702330e-01f, 3.826834324e-01f, 1.950903220e-01f, 0.0f, -1.950903220e-01f, -3.826834324e-01f, -5.555702330e-01f, -7.071067812e-01f, -8.314696123e-01f, -9.238795325e-01f, -9.807852804e-01f, -1.000000000e+00f, -9.807852804e-01f, -9.238795325e-01f, -8.314696123e-01f, -7.071067812e-01f, -5.555702330e-01f, -3.826834324e-01f, -1.950903220e-01f, 0.0f, 1.950903220e-01f, 3.826834324e-01f, 5.555702330e-01f, 7.071067812e-01f, 8.314696123e-01f, 9.238795325e-01f, 9.807852804e-01f};
@@ -46,15 +45,16 @@ constexpr float W32I[32] = {0.0f, -1.950903220e-01f, -3.826834324e-01f, -5.55570
 constexpr float W64R[64] = {1.000000000e+00f, 9.951847267e-01f, 9.807852804e-01f, 9.569403357e-01f, 9.238795325e-01f, 8.819212643e-01f, 8.314696123e-01f, 7.730104534e-01f, 7.071067812e-01f, 6.343932842e-01f, 5.555702330e-01f, 4.713967368e-01f, 3.826834324e-01f, 2.902846773e-01f, 1.950903220e-01f, 9.801714033e-02f, 0.0f, -9.801714033e-02f, -1.950903220e-01f, -2.902846773e-01f, -3.826834324e-01f, -4.713967368e-01f, -5.555702330e-01f, -6.343932842e-01f, -7.071067812e-01f, -7.730104534e-01f, -8.314696123e-01f, -8.819212643e-01f, -9.238795325e-01f, -9.569403357e-01f, -9.807852804e-01f, -9.951847267e-01f, -1.000000000e+00f, -9.951847267e-01f, -9.807852804e-01f, -9.569403357e-01f, -9.238795325e-01f, -8.819212643e-01f, -8.314696123e-01f, -7.730104534e-01f, -7.071067812e-01f, -6.343932842e-01f, -5.555702330e-01f, -4.713967368e-01f, -3.826834324e-01f, -2.902846773e-01f, -1.950903220e-01f, -9.801714033e-02f, 0.0f, 9.801714033e-02f, 1.950903220e-01f, 2.902846773e-01f, 3.826834324e-01f, 4.713967368e-01f, 5.555702330e-01f, 6.343932842e-01f, 7.071067812e-01f, 7.730104534e-01f, 8.314696123e-01f, 8.819212643e-01f, 9.238795325e-01f, 9.569403357e-01f, 9.807852804e-01f, 9.951847267e-01f};
 constexpr float W64I[64] = {0.0f, -9.801714033e-02f, -1.950903220e-01f, -2.902846773e-01f, -3.826834324e-01f, -4.713967368e-01f, -5.555702330e-01f, -6.343932842e-01f, -7.071067812e-01f, -7.730104534e-01f, -8.314696123e-01f, -8.819212643e-01f, -9.238795325e-01f, -9.569403357e-01f, -9.807852804e-01f, -9.951847267e-01f, -1.000000000e+00f, -9.951847267e-01f, -9.807852804e-01f, -9.569403357e-01f, -9.238795325e-01f, -8.819212643e-01f, -8.314696123e-01f, -7.730104534e-01f, -7.071067812e-01f, -6.343932842e-01f, -5.555702330e-01f, -4.713967368e-01f, -3.826834324e-01f, -2.902846773e-01f, -1.950903220e-01f, -9.801714033e-02f, 0.0f, 9.801714033e-02f, 1.950903220e-01f, 2.902846773e-01f, 3.826834324e-01f, 4.713967368e-01f, 5.555702330e-01f, 6.343932842e-01f, 7.071067812e-01f, 7.730104534e-01f, 8.314696123e-01f, 8.819212643e-01f, 9.238795325e-01f, 9.569403357e-01f, 9.807852804e-01f, 9.951847267e-01f, 1.000000000e+00f, 9.951847267e-01f, 9.807852804e-01f, 9.569403357e-01f, 9.238795325e-01f, 8.819212643e-01f, 8.314696123e-01f, 7.730104534e-01f, 7.071067812e-01f, 6.343932842e-01f, 5.555702330e-01f, 4.713967368e-01f, 3.826834324e-01f, 2.902846773e-01f, 1.950903220e-01f, 9.801714033e-02f};
 
-__device__ __forceinline__ uint32_t round_flags(int r) {
-  return r == 0 ? 0u : (r < RCAP - 1 ? F_RMW : F_ATOMIC);
-}
 
 // ---------------------------------------------------------------- schedules
 // One list per (t, slice) over the concatenated keys [0, m + p): key i < m is
 // column i of A (coefficient s1(i) W_b^(h1(i) slice)), key m + j is row j of
-// B (coefficient i s2(j) W_b^(h2(j) slice)).  Entry: dst (12 bits: bucket
-// mod 2048 or dummy slot 2048 + lane) | key (17 bits) << 12 | RMW | ATOMIC.
+// B (coefficient i s2(j) W_b^(h2(j) slice)).  Entry: LDS byte offset of the
+// destination (bits 0..15: bucket mod 2048, or dummy slot 2048 + lane) |
+// byte offset of the key in the staged segment << 16.  Per segment the groups
+// are laid out round by round (round 0 plain writes, rounds 1..RCAP-2
+// read-add-writes, the last round LDS atomics) with every round's group
+// count in segcnt.
 __global__ void __launch_bounds__(64) k_fft_lists(const uint32_t* __restrict__ hA,
                                                   const uint32_t* __restrict__ hB, int m, int p,
                                                   int lb, int lbs, const float2* __restrict__ tw,
@@ -110,7 +110,7 @@ __global__ void __launch_bounds__(64) k_fft_lists(const uint32_t* __restrict__ h
         const float2 w = tw[(uint64_t(h) * slice) & bmask];
         const float s = hneg(e) ? -1.f : 1.f;
         coef = isb ? make_float2(-s * w.y, s * w.x) : make_float2(s * w.x, s * w.y);
-        ent = (h & (B2 - 1)) | (uint32_t(key) << 12) | round_flags(r);
+        ent = ((h & (B2 - 1)) << 3) | (uint32_t(c + lane) << 18);
       }
 #pragma unroll
       for (int rr = 0; rr < RCAP; ++rr) {
@@ -128,11 +128,14 @@ __global__ void __launch_bounds__(64) k_fft_lists(const uint32_t* __restrict__ h
       const int full = (nr[rr] + 63) / 64 * 64;
       for (int slot = nr[rr] + lane; slot < full; slot += 64) {
         const int64_t pos = int64_t(base[rr]) * 64 + slot;
-        Ls[pos] = uint32_t(B2 + (slot & 63)) | (uint32_t(s0) << 12) | round_flags(rr);
+        Ls[pos] = uint32_t(B2 + (slot & 63)) << 3;
         Cs[pos] = make_float2(0.f, 0.f);
       }
     }
-    if (lane == 0) segcnt[int64_t(ts) * MAXSEG + s0 / SEG] = acc - out;
+    if (lane < RCAP) {
+      const int64_t row = int64_t(ts) * MAXSEG + s0 / SEG;
+      segcnt[row * RCAP + lane] = (nr[lane] + 63) / 64;
+    }
     out = acc;
     __syncthreads();
   }
@@ -189,6 +192,50 @@ __device__ __forceinline__ void stage_rows(const float* __restrict__ ar,
 
 constexpr int GB = 8;
 
+// One round of a segment's schedule: KIND 0 plain writes, 1 read-add-
+// writes, 2 LDS atomics; groups run in batches of GB whose entries,
+// coefficients and staged values are loaded before the batch's LDS writes.
+template <int KIND>
+__device__ __forceinline__ void scatter_section(const uint32_t* __restrict__ lp,
+                                                const float2* __restrict__ cp, int ng, float2* S,
+                                                const float* Rb, int lane) {
+  const char* Rbb = reinterpret_cast<const char*>(Rb);
+  char* Sb = reinterpret_cast<char*>(S);
+  for (int g0 = 0; g0 < ng; g0 += GB) {
+    const int nb = min(GB, ng - g0);
+    uint32_t ent[GB];
+    float2 cf[GB];
+    float v[GB];
+#pragma unroll
+    for (int q = 0; q < GB; ++q) {
+      const int64_t pos = int64_t(g0 + min(q, nb - 1)) * 64 + lane;
+      ent[q] = lp[pos];
+      cf[q] = cp[pos];
+    }
+#pragma unroll
+    for (int q = 0; q < GB; ++q) v[q] = *reinterpret_cast<const float*>(Rbb + ((ent[q] >> 16) & 0xffcu));
+    float2 c[GB];
+#pragma unroll
+    for (int q = 0; q < GB; ++q) c[q] = make_float2(v[q] * cf[q].x, v[q] * cf[q].y);
+#pragma unroll
+    for (int q = 0; q < GB; ++q) {
+      if (q < nb) {
+        float2* d = reinterpret_cast<float2*>(Sb + (ent[q] & 0xffffu));
+        if (KIND == 2) {
+          float* df = reinterpret_cast<float*>(d);
+          __hip_atomic_fetch_add(df, c[q].x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          __hip_atomic_fetch_add(df + 1, c[q].y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        } else if (KIND == 1) {
+          const float2 o = *d;
+          *d = make_float2(o.x + c[q].x, o.y + c[q].y);
+        } else {
+          *d = c[q];
+        }
+      }
+    }
+  }
+}
+
 // scatter + 2048-point DFT of the pair-packed slice sketch; on return lane L,
 // register q holds frequency fq_of(L, q)
 __device__ __forceinline__ void sketch_fft(const float* __restrict__ ar,
@@ -202,45 +249,20 @@ __device__ __forceinline__ void sketch_fft(const float* __restrict__ ar,
 #pragma unroll
   for (int j = 0; j < B2 / 2 / 64; ++j) S4[j * 64 + lane] = make_float4(0.f, 0.f, 0.f, 0.f);
   const int nseg = (keys + SEG - 1) / SEG;
-  int gbase = 0;
+  int64_t off = 0;
   for (int seg = 0; seg < nseg; ++seg) {
-    const int ngs = segcnt[seg];
+    const int* __restrict__ nr = segcnt + seg * RCAP;
     __syncthreads();
     stage_rows(ar, br, m, keys, seg, Rb, lane);
     __syncthreads();
-    for (int g0 = 0; g0 < ngs; g0 += GB) {
-      const int nb = min(GB, ngs - g0);
-      uint32_t ent[GB];
-      float2 cf[GB];
-      float v[GB];
-#pragma unroll
-      for (int q = 0; q < GB; ++q) {
-        const int64_t pos = int64_t(gbase + g0 + min(q, nb - 1)) * 64 + lane;
-        ent[q] = list[pos];
-        cf[q] = coef[pos];
-      }
-#pragma unroll
-      for (int q = 0; q < GB; ++q) v[q] = Rb[((ent[q] >> 12) & 0x1ffffu) % SEG];
-#pragma unroll
-      for (int q = 0; q < GB; ++q) {
-        if (q < nb) {
-          const uint32_t first = __builtin_amdgcn_readfirstlane(ent[q]);
-          const uint32_t dst = ent[q] & 0xfffu;
-          const float cr = v[q] * cf[q].x, ci = v[q] * cf[q].y;
-          if (first & F_ATOMIC) {
-            float* d = reinterpret_cast<float*>(&S[dst]);
-            __hip_atomic_fetch_add(d, cr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-            __hip_atomic_fetch_add(d + 1, ci, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-          } else if (first & F_RMW) {
-            const float2 o = S[dst];
-            S[dst] = make_float2(o.x + cr, o.y + ci);
-          } else {
-            S[dst] = make_float2(cr, ci);
-          }
-        }
-      }
-    }
-    gbase += ngs;
+    int nw = 0;
+    for (int r = 1; r < RCAP - 1; ++r) nw += nr[r];
+    scatter_section<0>(list + off, coef + off, nr[0], S, Rb, lane);
+    off += int64_t(nr[0]) * 64;
+    scatter_section<1>(list + off, coef + off, nw, S, Rb, lane);  // rounds 1..RCAP-2
+    off += int64_t(nw) * 64;
+    scatter_section<2>(list + off, coef + off, nr[RCAP - 1], S, Rb, lane);
+    off += int64_t(nr[RCAP - 1]) * 64;
   }
   __syncthreads();
 #pragma unroll
@@ -345,7 +367,7 @@ __global__ void __launch_bounds__(64) k_compress_fft_reg(CompressArgs a,
   const float* __restrict__ B = static_cast<const float*>(a.B);
   const uint32_t* __restrict__ ls = L + ts * cap;
   const float2* __restrict__ cs = C + ts * cap;
-  const int* __restrict__ sc = segcnt + int64_t(ts) * MAXSEG;
+  const int* __restrict__ sc = segcnt + int64_t(ts) * MAXSEG * RCAP;
   const int keys = a.m + a.p;
 
   float qr[R], qi[R];
@@ -396,7 +418,7 @@ int compress_fft_reg_slots() {
 
 size_t compress_fft_reg_scratch_bytes(int d, int lb, int m, int p) {
   const size_t lists = size_t(d) << (lb - LB2);
-  return lists * size_t(list_cap(m + p)) * 12 + lists * MAXSEG * sizeof(int) +
+  return lists * size_t(list_cap(m + p)) * 12 + lists * MAXSEG * RCAP * sizeof(int) +
          size_t(B2) * sizeof(float2) + 256;
 }
 

@@ -51,17 +51,16 @@ __device__ __forceinline__ float flip(float v, uint32_t sgn) {
 // atomics.  Items of one group (64 lanes) never share a bucket, and every
 // group runs after the groups holding earlier items of its buckets, so the
 // result is exact FP32 in a fixed order (deterministic).
-// Entry: dst (13 bits: bucket mod 2048, or a per-lane dummy slot 2048+lane
-// for padding) | src (16 bits: key index) << 13 | sign << 29 | RMW << 30 |
-// ATOMIC << 31.  Flags are uniform per group.
+// Entry: LDS byte offset of the destination (bits 0..15: bucket mod 2048,
+// or a per-lane dummy slot 2048+lane for padding) | byte offset of the key
+// in the staged segment << 16 (bits 16..27) | sign << 31.  Per segment the
+// groups are laid out round by round (round 0 plain writes, rounds
+// 1..RCAP-2 read-add-writes, the last round LDS atomics) and the group
+// count of every round is stored.
 constexpr int SEG = 1024;
 constexpr int RCAP = 8;
 constexpr int MAXSEG = 64;  // keys <= MAXSEG * SEG = 65536
-constexpr uint32_t F_SIGN = 1u << 29, F_RMW = 1u << 30, F_ATOMIC = 1u << 31;
-
-__device__ __forceinline__ uint32_t round_flags(int r) {
-  return r == 0 ? 0u : (r < RCAP - 1 ? F_RMW : F_ATOMIC);
-}
+constexpr uint32_t F_SIGN = 1u << 31;
 
 __global__ void __launch_bounds__(64) k_reg_lists(const uint32_t* __restrict__ hA,
                                                   const uint32_t* __restrict__ hB, int m, int p,
@@ -114,8 +113,7 @@ __global__ void __launch_bounds__(64) k_reg_lists(const uint32_t* __restrict__ h
         const uint32_t e = hh[s0 + c + lane];
         const uint32_t h = hbucket(e);
         const uint32_t neg = hneg(e) ^ (__popc((h >> LB2) & slice) & 1u);
-        ent = (h & (B2 - 1)) | (uint32_t(s0 + c + lane) << 13) | (neg ? F_SIGN : 0u) |
-              round_flags(r);
+        ent = ((h & (B2 - 1)) << 2) | (uint32_t(c + lane) << 18) | (neg ? F_SIGN : 0u);
       }
 #pragma unroll
       for (int rr = 0; rr < RCAP; ++rr) {
@@ -133,9 +131,12 @@ __global__ void __launch_bounds__(64) k_reg_lists(const uint32_t* __restrict__ h
       const int full = (nr[rr] + 63) / 64 * 64;
       for (int slot = nr[rr] + lane; slot < full; slot += 64)
         L[int64_t(base[rr]) * 64 + slot] =
-            uint32_t(B2 + (slot & 63)) | (uint32_t(s0) << 13) | round_flags(rr);
+            uint32_t(B2 + (slot & 63)) << 2;
     }
-    if (lane == 0) ngroups[g * MAXSEG + s0 / SEG] = acc - out;
+    if (lane < RCAP) {
+      const int64_t row = int64_t(g) * MAXSEG + s0 / SEG;
+      ngroups[row * RCAP + lane] = (nr[lane] + 63) / 64;
+    }
     out = acc;
     __syncthreads();
   }
@@ -154,6 +155,8 @@ __device__ __forceinline__ void wht_reg(float (&x)[R]) {
   }
 }
 
+constexpr int GB = 16;  // schedule groups per batch
+
 // stage keys [seg*SEG, seg*SEG + SEG) of a row into Rb
 __device__ __forceinline__ void stage_row(const float* __restrict__ row, int keys, int seg,
                                           float* Rb, int lane) {
@@ -166,13 +169,41 @@ __device__ __forceinline__ void stage_row(const float* __restrict__ row, int key
   for (int q = 0; q < SEG / 64; ++q) Rb[q * 64 + lane] = v[q];
 }
 
+// One round of a segment's schedule: KIND 0 plain writes, 1 read-add-
+// writes, 2 LDS atomics; groups run in batches of GB whose entries and
+// staged values are loaded before the batch's LDS writes.
+template <int KIND>
+__device__ __forceinline__ void scatter_section(const uint32_t* __restrict__ lp, int ng, float* S,
+                                                const float* Rb, int lane) {
+  const char* Rbb = reinterpret_cast<const char*>(Rb);
+  char* Sb = reinterpret_cast<char*>(S);
+  for (int g0 = 0; g0 < ng; g0 += GB) {
+    const int nb = min(GB, ng - g0);
+    uint32_t ent[GB];
+    float v[GB];
+#pragma unroll
+    for (int q = 0; q < GB; ++q) ent[q] = lp[int64_t(g0 + min(q, nb - 1)) * 64 + lane];
+#pragma unroll
+    for (int q = 0; q < GB; ++q)
+      v[q] = flip(*reinterpret_cast<const float*>(Rbb + ((ent[q] >> 16) & 0xffcu)), ent[q] & F_SIGN);
+#pragma unroll
+    for (int q = 0; q < GB; ++q) {
+      if (q < nb) {
+        float* d = reinterpret_cast<float*>(Sb + (ent[q] & 0xffffu));
+        if (KIND == 2)
+          __hip_atomic_fetch_add(d, v[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        else if (KIND == 1)
+          *d += v[q];
+        else
+          *d = v[q];
+      }
+    }
+  }
+}
+
 // scatter one row into S following its schedule, read the sketch into x,
 // full WHT of x (the transpose leaves S dirty; the next scatter zeroes it).
-// Groups run in batches of GB: the batch's schedule entries and staged
-// values are all loaded before its LDS writes, so the gathers overlap.
 // On return lane L, register r holds frequency f_of(L, r).
-constexpr int GB = 16;
-
 __device__ __forceinline__ void sketch_transform(const float* __restrict__ row, int keys,
                                                  const uint32_t* __restrict__ list,
                                                  const int* __restrict__ segcnt, float* S,
@@ -181,36 +212,20 @@ __device__ __forceinline__ void sketch_transform(const float* __restrict__ row, 
 #pragma unroll
   for (int j = 0; j < B2 / 4 / 64; ++j) S4[j * 64 + lane] = make_float4(0.f, 0.f, 0.f, 0.f);
   const int nseg = (keys + SEG - 1) / SEG;
-  int gbase = 0;
+  const uint32_t* __restrict__ lp = list;
   for (int seg = 0; seg < nseg; ++seg) {
-    const int ngs = segcnt[seg];
+    const int* __restrict__ nr = segcnt + seg * RCAP;
     __syncthreads();
     stage_row(row, keys, seg, Rb, lane);
     __syncthreads();
-    for (int g0 = 0; g0 < ngs; g0 += GB) {
-      const int nb = min(GB, ngs - g0);
-      uint32_t ent[GB];
-      float v[GB];
-#pragma unroll
-      for (int q = 0; q < GB; ++q) ent[q] = list[int64_t(gbase + g0 + min(q, nb - 1)) * 64 + lane];
-#pragma unroll
-      for (int q = 0; q < GB; ++q) v[q] = flip(Rb[((ent[q] >> 13) & 0xffffu) % SEG], (ent[q] & F_SIGN) << 2);
-#pragma unroll
-      for (int q = 0; q < GB; ++q) {
-        if (q < nb) {
-          const uint32_t first = __builtin_amdgcn_readfirstlane(ent[q]);
-          const uint32_t dst = ent[q] & 0x1fffu;
-          if (first & F_ATOMIC) {
-            __hip_atomic_fetch_add(&S[dst], v[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-          } else {
-            float w = v[q];
-            if (first & F_RMW) w += S[dst];
-            S[dst] = w;
-          }
-        }
-      }
-    }
-    gbase += ngs;
+    int nw = 0;
+    for (int r = 1; r < RCAP - 1; ++r) nw += nr[r];
+    scatter_section<0>(lp, nr[0], S, Rb, lane);
+    lp += int64_t(nr[0]) * 64;
+    scatter_section<1>(lp, nw, S, Rb, lane);  // rounds 1..RCAP-2, in order
+    lp += int64_t(nw) * 64;
+    scatter_section<2>(lp, nr[RCAP - 1], S, Rb, lane);
+    lp += int64_t(nr[RCAP - 1]) * 64;
   }
   __syncthreads();
 #pragma unroll
@@ -276,8 +291,8 @@ __global__ void __launch_bounds__(64) k_compress_fwht_reg(CompressArgs a,
   const float* __restrict__ B = static_cast<const float*>(a.B);
   const uint32_t* __restrict__ la = LA + ts * cap_a;
   const uint32_t* __restrict__ lb = LB + ts * cap_b;
-  const int* __restrict__ sca = ngroups + int64_t(ts) * MAXSEG;
-  const int* __restrict__ scb = ngroups + int64_t(per_chunk + ts) * MAXSEG;
+  const int* __restrict__ sca = ngroups + int64_t(ts) * MAXSEG * RCAP;
+  const int* __restrict__ scb = ngroups + int64_t(per_chunk + ts) * MAXSEG * RCAP;
 
   float acc[R];
 #pragma unroll
@@ -324,7 +339,7 @@ int compress_fwht_reg_slots() {
 
 size_t compress_fwht_reg_entries_bytes(int d, int lb, int m, int p) {
   const size_t lists = size_t(d) << (lb - LB2);
-  return lists * size_t(list_cap(m) + list_cap(p)) * 4 + 2 * lists * MAXSEG * sizeof(int) + 256;
+  return lists * size_t(list_cap(m) + list_cap(p)) * 4 + 2 * lists * MAXSEG * RCAP * sizeof(int) + 256;
 }
 
 hipError_t launch_compress_fwht_reg(const CompressArgs& a, void* entries, hipStream_t s) {
