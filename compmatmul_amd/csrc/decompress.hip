@@ -86,16 +86,19 @@ __global__ void __launch_bounds__(NT) k_decompress(DecompressArgs a) {
 }
 
 // LDS-staged variant for sketch rows of at most 64 KB: the workgroup (512
-// threads, one column each, 6 (float) or 3 (double) rows per thread) streams the d rows of the
+// threads, one column each, up to 12 rows per thread) streams the d rows of the
 // sketch through two LDS buffers (row t+1 is fetched while row t is
 // gathered), so the random gathers hit LDS instead of 128-byte L2 lines.
 constexpr int LNT = 512;
-template <typename T>
-constexpr int rows_per_thread() { return sizeof(T) == 8 ? 3 : 6; }
+// rows per thread: as many as fit ~112 VGPRs of gathered values
+template <typename T, int D>
+constexpr int rows_per_thread() {
+  return (sizeof(T) == 8 ? 56 : 112) / D < 1 ? 1 : ((sizeof(T) == 8 ? 56 : 112) / D > 12 ? 12 : (sizeof(T) == 8 ? 56 : 112) / D);
+}
 
 template <typename T, bool FWHT, int D>
 __global__ void __launch_bounds__(LNT) k_decompress_lds(DecompressArgs a) {
-  constexpr int ER = rows_per_thread<T>();
+  constexpr int ER = rows_per_thread<T, D>();
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int b = a.b;
   T* const buf0 = reinterpret_cast<T*>(smem_raw);
@@ -182,8 +185,6 @@ __global__ void __launch_bounds__(NT) k_decompress_any(DecompressArgs a) {
 template <typename T, bool FWHT>
 hipError_t launch_t(const DecompressArgs& a, hipStream_t s) {
   if (a.d <= 16 && size_t(a.b) * sizeof(T) <= 65536 && size_t(a.b) * sizeof(T) >= 16) {
-    constexpr int ER = rows_per_thread<T>();
-    const dim3 lgrid(unsigned(int64_t((a.p + LNT - 1) / LNT) * ((a.m + ER - 1) / ER)));
     const size_t lds = 2 * size_t(a.b) * sizeof(T);
     switch (a.d) {
 #define CM_D(N)                                                                       \
@@ -192,7 +193,10 @@ hipError_t launch_t(const DecompressArgs& a, hipStream_t s) {
                                            hipFuncAttributeMaxDynamicSharedMemorySize,  \
                                            int(lds)) != hipSuccess)                     \
       break;                                                                        \
-    k_decompress_lds<T, FWHT, N><<<lgrid, dim3(LNT), lds, s>>>(a);                  \
+    k_decompress_lds<T, FWHT, N><<<dim3(unsigned(int64_t((a.p + LNT - 1) / LNT) *           \
+                                          ((a.m + rows_per_thread<T, N>() - 1) /         \
+                                           rows_per_thread<T, N>()))),                   \
+                                   dim3(LNT), lds, s>>>(a);                              \
     return hipGetLastError();
       CM_D(1) CM_D(2) CM_D(3) CM_D(4) CM_D(5) CM_D(6) CM_D(7) CM_D(8)
       CM_D(9) CM_D(10) CM_D(11) CM_D(12) CM_D(13) CM_D(14) CM_D(15) CM_D(16)
