@@ -73,6 +73,27 @@ def cpu_baseline(c, AT, B):
                       f"{c['m']}x{c['p']} median-of-{c['d']} decompression"}
 
 
+def copy_probe(nbytes):
+    src = cma.DeviceArray((nbytes // 4,), np.float32)
+    dst = cma.DeviceArray((nbytes // 4,), np.float32)
+    for _ in range(5):
+        cma.check(cma.lib().cm_memcpy(dst.ptr, src.ptr, nbytes, cma.DEVICE, cma.DEVICE))
+    cma.synchronize()
+    print(json.dumps({"copy_probe_bytes": nbytes, "copies": 5}), flush=True)
+
+
+def traffic_from_profile(name):
+    """HBM bytes per compress launch for `name`, from the committed counter
+    summary (FETCH_SIZE + WRITE_SIZE per dispatch, scaled by the same counters
+    read on a device-to-device copy of known size), or None."""
+    path = os.path.join(HERE, "profiles", f"{name}_compress_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        t = json.load(f)
+    return t.get("traffic_bytes")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -82,7 +103,13 @@ def main():
     ap.add_argument("--plain", action="store_true", help="time the unfused path instead")
     ap.add_argument("--workload", default=DEFAULT_WORKLOAD, choices=sorted(workloads.CONFIGS),
                     help="named workload (default: the metric's configuration)")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run 5 device-to-device copies of MIB MiB (the known-size "
+                         "kernel that scales FETCH_SIZE / WRITE_SIZE counter readings)")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe(args.copy_probe << 20)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
               "(the product shards by inner dimension or by product; see DESIGN.md)",
@@ -164,7 +191,8 @@ def main():
                    "parallelism": "single GPU"},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak,
                      "unit": "TFLOP/s", "frac": achieved / peak,
-                     "traffic": None, "kernel": kname,
+                     "traffic": None if args.plain else traffic_from_profile(name),
+                     "kernel": kname,
                      "kernel_ms": k_avg_s * 1e3, "flops_per_launch": flops},
         "detail": {"recall": recall, "precision": precision, "recovered": len(got),
                    "truth": len(truth), "C_threshold_set_equal": est_set == got,
