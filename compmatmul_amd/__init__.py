@@ -5,7 +5,8 @@ Python host layer over the C-ABI library ``libcompmatmul_amd.so``
 reference's interface is ``compmatmul_amd.compmatmul`` (importable as plain
 ``compmatmul`` with this directory on ``sys.path``); this module adds the
 extended interface: float32 inputs, device-resident inputs and sketches,
-sketch values, hash arrays, large-entry recovery and kernel timing.
+sketch values, hash arrays, large-entry recovery, kernel timing, and
+batches of many products (Batch).
 
 There is no CPU fallback: if the native library is missing or no gfx950
 device is present, the calls raise.
@@ -74,6 +75,14 @@ def lib():
     L.cm_gemm.argtypes = [ctypes.c_int] * 3 + [_vp] * 3
     L.cm_timing_enable.argtypes = [ctypes.c_int]
     L.cm_timing_get.argtypes = [ctypes.c_char_p, _f64p, _i64p]
+    L.cm_batch_plan.argtypes = [ctypes.c_int, _vp, ctypes.c_uint, ctypes.c_int,
+                                ctypes.POINTER(_vp)]
+    L.cm_batch_run.argtypes = [_vp]
+    L.cm_batch_grain.argtypes = [_vp]
+    L.cm_batch_recovered.argtypes = [_vp, ctypes.c_int, ctypes.c_int64, _i32p, _i32p, _vp, _i64p]
+    L.cm_batch_values.argtypes = [_vp, ctypes.c_int, _vp]
+    L.cm_batch_free.argtypes = [_vp]
+    L.cm_batch_free.restype = None
     _LIB = L
     return L
 
@@ -109,6 +118,14 @@ class DeviceArray:
         if a.nbytes:
             check(lib().cm_memcpy(out.ptr, a.ctypes.data, a.nbytes, DEVICE, HOST))
         return out
+
+    def upload(self, a):
+        """Copy host array `a` (same shape and dtype) into this buffer."""
+        a = np.ascontiguousarray(a, self.dtype)
+        if a.shape != self.shape:
+            raise ValueError(f"shape {a.shape} != {self.shape}")
+        if a.nbytes:
+            check(lib().cm_memcpy(self.ptr, a.ctypes.data, a.nbytes, DEVICE, HOST))
 
     def numpy(self):
         out = np.empty(self.shape, self.dtype)
@@ -262,6 +279,104 @@ def timing_get(name):
     n = ctypes.c_int64()
     check(lib().cm_timing_get(name.encode(), ctypes.byref(ms), ctypes.byref(n)))
     return ms.value, n.value
+
+
+class _Product(ctypes.Structure):
+    _fields_ = [("variant", ctypes.c_int), ("m", ctypes.c_int), ("n", ctypes.c_int),
+                ("p", ctypes.c_int), ("AT", _vp), ("ld_at", ctypes.c_int64), ("B", _vp),
+                ("ld_b", ctypes.c_int64), ("C", _vp), ("ldc", ctypes.c_int64),
+                ("d", ctypes.c_int), ("b", ctypes.c_int), ("seed", ctypes.c_uint64),
+                ("family", ctypes.c_int), ("tau", ctypes.c_double), ("max_out", ctypes.c_int64)]
+
+
+BATCH_GRAPH = 1
+
+
+class Product:
+    """One product of a Batch: float32 DeviceArray operands AT (n x m) and
+    B (n x p), optional DeviceArray output C (m x p), recovery threshold tau
+    (None: no recovery) with room for max_out entries."""
+
+    def __init__(self, variant, AT, B, d, b, seed, hash=None, C=None, tau=None, max_out=4096):
+        for name, x in (("AT", AT), ("B", B), ("C", C)):
+            if x is not None and (not isinstance(x, DeviceArray) or x.dtype != np.float32):
+                raise ValueError(f"{name} must be a float32 DeviceArray")
+        if AT.shape[0] != B.shape[0]:
+            raise ValueError("Shape mismatch! Note that AT is assumed to be transposed")
+        if C is not None and C.shape != (AT.shape[1], B.shape[1]):
+            raise ValueError("C must be m x p")
+        if hash not in FAMILIES:
+            raise ValueError(f"Invalid hash `{hash}'")
+        self.variant, self.AT, self.B, self.C = VARIANTS[variant], AT, B, C
+        self.d, self.b, self.seed, self.hash = int(d), int(b), int(seed), hash
+        self.tau, self.max_out = tau, int(max_out)
+        self.n, self.m = AT.shape
+        self.p = B.shape[1]
+
+    def _c(self):
+        q = _Product()
+        q.variant, q.m, q.n, q.p = self.variant, self.m, self.n, self.p
+        q.AT, q.ld_at = self.AT.ptr, self.m
+        q.B, q.ld_b = self.B.ptr, self.p
+        q.C, q.ldc = (self.C.ptr if self.C is not None else None), self.p
+        q.d, q.b, q.seed, q.family = self.d, self.b, self.seed, FAMILIES[self.hash]
+        q.tau = float("nan") if self.tau is None else float(self.tau)
+        q.max_out = self.max_out if self.tau is not None else 0
+        return q
+
+
+class Batch:
+    """Many float32 products planned once and run in one pass: one persistent
+    compress launch per transform variant over XCD-local work queues, one
+    launch per finishing step and per decompression class.  grain=0 uses the
+    autotuned grain for these shapes; graph=True replays a HIP graph."""
+
+    def __init__(self, products, grain=0, graph=False):
+        self.products = list(products)
+        arr = (_Product * max(1, len(self.products)))()
+        for i, q in enumerate(self.products):
+            arr[i] = q._c()
+        self.handle = _vp()
+        check(lib().cm_batch_plan(len(self.products), arr, BATCH_GRAPH if graph else 0,
+                                  int(grain), ctypes.byref(self.handle)))
+
+    @property
+    def grain(self):
+        return int(lib().cm_batch_grain(self.handle))
+
+    def run(self):
+        check(lib().cm_batch_run(self.handle))
+
+    def recovered(self, i):
+        """(i, j, value) arrays of product i's entries > tau, sorted by (i, j)."""
+        cap = self.products[i].max_out
+        oi = np.empty(max(1, cap), np.int32)
+        oj = np.empty(max(1, cap), np.int32)
+        ov = np.empty(max(1, cap), np.float32)
+        cnt = ctypes.c_int64()
+        check(lib().cm_batch_recovered(self.handle, i, cap, oi.ctypes.data_as(_i32p),
+                                       oj.ctypes.data_as(_i32p), ov.ctypes.data, ctypes.byref(cnt)))
+        if cnt.value > cap:
+            raise RuntimeError(f"product {i}: {cnt.value} entries above tau, room for {cap}")
+        k = cnt.value
+        return oi[:k].copy(), oj[:k].copy(), ov[:k].copy()
+
+    def values(self, i):
+        q = self.products[i]
+        out = np.empty((q.d, q.b), np.float32)
+        check(lib().cm_batch_values(self.handle, i, out.ctypes.data))
+        return out
+
+    def free(self):
+        if self.handle:
+            lib().cm_batch_free(self.handle)
+            self.handle = _vp()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 def dropin():

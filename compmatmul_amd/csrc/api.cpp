@@ -17,6 +17,7 @@
 
 #include "host_hash.hpp"
 #include "kernels.hpp"
+#include "runtime.hpp"
 
 struct cm_sketch {
   int fwht, f64, d, b, lb, m, p, family;
@@ -30,6 +31,10 @@ namespace {
 
 thread_local std::string g_err;
 std::recursive_mutex g_mu;
+
+// smallest log2(b) sent to the register kernels (padding to 2048 costs at
+// most 2x there; smaller sketches use the LDS kernel)
+constexpr int kRegMinLb = 10;
 
 int fail(int code, const std::string& msg) {
   g_err = msg;
@@ -409,95 +414,63 @@ int compress_device(cm_sketch* sk, const void* AT, int64_t ld_at, const void* B,
     if (rc) return rc;
     ra.V = g.V.p;
     if ((rc = timed("reduce", [&] { return cm::launch_reduce_inverse(ra, g.stream); }))) return rc;
-    cm::CombineArgs ca{fwht, f64, g.V.p, d, lb, lb, tw, sk->P};
+    cm::CombineArgs ca{fwht, f64, g.V.p, d, lb, lb, tw, sk->P, lb};
     return timed("combine", [&] { return cm::launch_combine(ca, g.stream); });
   }
 
-  // fast path: FP32 FWHT with b >= 4096 runs the register-resident kernel
-  const bool reg = fwht && !f64 && !(flags & CM_FLAG_LDS_KERNEL) && cm::compress_fwht_reg_supported(lb, m, p);
+  // fast path: FP32 with b >= 1024 runs the register-resident kernels
+  // (2048-frequency slices; smaller b zero-padded to one slice)
+  const bool reg = !f64 && !(flags & CM_FLAG_LDS_KERNEL) && lb >= kRegMinLb &&
+                   (fwht ? cm::fwht_reg_supported(lb, m, p) : cm::fft_reg_supported(lb, m, p));
   if (reg) {
-    const int lb2 = cm::compress_fwht_reg_lb2();
-    const int64_t per_chunk = int64_t(d) << (lb - lb2);
-    const int64_t slots = cm::compress_fwht_reg_slots();  // resident single-wave workgroups
+    const int lbv = std::max(lb, cm::kRegLb2);  // transform length actually run
+    const int lbs = cm::reg_lbs(lb);
+    const int64_t per_chunk = int64_t(d) << lbs;
+    const int64_t slots = fwht ? cm::fwht_reg_slots() : cm::fft_reg_slots();
     const int64_t rounds = std::max<int64_t>(1, (int64_t(n) * per_chunk + slots * 64 - 1) / (slots * 64));
     int64_t nchunks = std::max<int64_t>(1, slots * rounds / per_chunk);
     nchunks = std::min<int64_t>(nchunks, std::max(1, n));
-    const size_t chunk_bytes = size_t(d) * b * esz;
+    const size_t chunk_bytes = (size_t(d) << lbv) * cplx * esz;
     nchunks = std::max<int64_t>(1, std::min<int64_t>(nchunks, int64_t((size_t(1) << 30) / chunk_bytes)));
     const int kper = n > 0 ? int((n + nchunks - 1) / nchunks) : 0;
     nchunks = n > 0 ? (n + kper - 1) / kper : 0;
-    int rc = ensure_buf(g.partial, std::max<size_t>(1, size_t(nchunks) * chunk_bytes));
-    if (rc) return rc;
-    if ((rc = ensure_buf(g.V, chunk_bytes))) return rc;
-    if ((rc = ensure_buf(g.entries, cm::compress_fwht_reg_entries_bytes(d, lb, m, p)))) return rc;
-    cm::CompressArgs caa{};
-    caa.fwht = 1;
-    caa.AT = AT;
-    caa.ld_at = ld_at;
-    caa.B = B;
-    caa.ld_b = ld_b;
-    caa.hA = sk->hA;
-    caa.hB = sk->hB;
-    caa.m = m;
-    caa.n = n;
-    caa.p = p;
-    caa.d = d;
-    caa.lb = lb;
-    caa.lb2 = lb2;
-    caa.nchunks = int(nchunks);
-    caa.kper = kper;
-    caa.partial = g.partial.p;
-    if ((rc = timed("compress", [&] { return cm::launch_compress_fwht_reg(caa, g.entries.p, g.stream); })))
-      return rc;
-    const int f64_reduce = (flags & CM_FLAG_F64_REDUCE) || n > 16384;
-    cm::ReduceArgs ra{1, 0, f64_reduce, g.partial.p, int(nchunks), d, lb, lb2, nullptr, g.V.p};
-    if ((rc = timed("reduce", [&] { return cm::launch_reduce_inverse(ra, g.stream); }))) return rc;
-    cm::CombineArgs ca{1, 0, g.V.p, d, lb, lb2, nullptr, sk->P};
-    return timed("combine", [&] { return cm::launch_combine(ca, g.stream); });
-  }
-
-  // fast path: FP32 FFT with b >= 2048 runs the register-resident kernel
-  const bool freg = !fwht && !f64 && !(flags & CM_FLAG_LDS_KERNEL) && cm::compress_fft_reg_supported(lb, m, p);
-  if (freg) {
-    const int lb2 = cm::compress_fft_reg_lb2();
-    const int64_t per_chunk = int64_t(d) << (lb - lb2);
-    const int64_t slots = cm::compress_fft_reg_slots();
-    const int64_t rounds = std::max<int64_t>(1, (int64_t(n) * per_chunk + slots * 64 - 1) / (slots * 64));
-    int64_t nchunks = std::max<int64_t>(1, slots * rounds / per_chunk);
-    nchunks = std::min<int64_t>(nchunks, std::max(1, n));
-    const size_t chunk_bytes = size_t(d) * b * 2 * esz;
-    nchunks = std::max<int64_t>(1, std::min<int64_t>(nchunks, int64_t((size_t(1) << 30) / chunk_bytes)));
-    const int kper = n > 0 ? int((n + nchunks - 1) / nchunks) : 0;
-    nchunks = n > 0 ? (n + kper - 1) / kper : 0;
+    const void* twv = nullptr;  // twiddles of the transform length run
     const void* t2 = nullptr;
-    int rc = fft_t2_table(&t2);
-    if (rc) return rc;
+    int rc;
+    if (!fwht) {
+      if ((rc = twiddles(lbv, 0, &twv)) || (rc = fft_t2_table(&t2))) return rc;
+    }
     if ((rc = ensure_buf(g.partial, std::max<size_t>(1, size_t(nchunks) * chunk_bytes)))) return rc;
     if ((rc = ensure_buf(g.V, chunk_bytes))) return rc;
-    if ((rc = ensure_buf(g.entries, cm::compress_fft_reg_scratch_bytes(d, lb, m, p)))) return rc;
-    cm::CompressArgs caa{};
-    caa.AT = AT;
-    caa.ld_at = ld_at;
-    caa.B = B;
-    caa.ld_b = ld_b;
-    caa.hA = sk->hA;
-    caa.hB = sk->hB;
-    caa.m = m;
-    caa.n = n;
-    caa.p = p;
-    caa.d = d;
-    caa.lb = lb;
-    caa.lb2 = lb2;
-    caa.nchunks = int(nchunks);
-    caa.kper = kper;
-    caa.tw = tw;
-    caa.partial = g.partial.p;
-    if ((rc = timed("compress", [&] { return cm::launch_compress_fft_reg(caa, g.entries.p, t2, g.stream); })))
+    const size_t lists_bytes = fwht ? cm::fwht_reg_lists_bytes(d, lb, m, p)
+                                    : cm::fft_reg_lists_bytes(d, lb, m, p);
+    if ((rc = ensure_buf(g.entries, lists_bytes))) return rc;
+    cm::RegJob j{};
+    j.AT = static_cast<const float*>(AT);
+    j.ld_at = ld_at;
+    j.B = static_cast<const float*>(B);
+    j.ld_b = ld_b;
+    j.hA = sk->hA;
+    j.hB = sk->hB;
+    j.m = m;
+    j.n = n;
+    j.p = p;
+    j.d = d;
+    j.lb = lb;
+    j.lbs = lbs;
+    j.kper = kper;
+    j.nchunks = int(nchunks);
+    j.lists = g.entries.p;
+    j.tw = twv;
+    j.t2 = t2;
+    j.partial = static_cast<float*>(g.partial.p);
+    if ((rc = timed("compress", [&] { return fwht ? cm::launch_fwht_reg(j, g.stream)
+                                                  : cm::launch_fft_reg(j, g.stream); })))
       return rc;
     const int f64_reduce = (flags & CM_FLAG_F64_REDUCE) || n > 16384;
-    cm::ReduceArgs ra{0, 0, f64_reduce, g.partial.p, int(nchunks), d, lb, lb2, tw, g.V.p};
+    cm::ReduceArgs ra{fwht, 0, f64_reduce, g.partial.p, int(nchunks), d, lbv, cm::kRegLb2, twv, g.V.p};
     if ((rc = timed("reduce", [&] { return cm::launch_reduce_inverse(ra, g.stream); }))) return rc;
-    cm::CombineArgs ca{0, 0, g.V.p, d, lb, lb2, tw, sk->P};
+    cm::CombineArgs ca{fwht, 0, g.V.p, d, lbv, cm::kRegLb2, twv, sk->P, lb};
     return timed("combine", [&] { return cm::launch_combine(ca, g.stream); });
   }
 
@@ -539,7 +512,7 @@ int compress_device(cm_sketch* sk, const void* AT, int64_t ld_at, const void* B,
   const int f64_reduce = (flags & CM_FLAG_F64_REDUCE) || n > 16384;
   cm::ReduceArgs ra{fwht, f64, f64_reduce, g.partial.p, int(nchunks), d, lb, lb2, tw, g.V.p};
   if ((rc = timed("reduce", [&] { return cm::launch_reduce_inverse(ra, g.stream); }))) return rc;
-  cm::CombineArgs ca{fwht, f64, g.V.p, d, lb, lb2, tw, sk->P};
+  cm::CombineArgs ca{fwht, f64, g.V.p, d, lb, lb2, tw, sk->P, lb};
   return timed("combine", [&] { return cm::launch_combine(ca, g.stream); });
 }
 
@@ -651,6 +624,20 @@ int matmul_dropin(int variant, int n_a, int n_inner, int n_b, double* C, const d
 }
 
 }  // namespace
+
+namespace cm {
+namespace rt {
+std::recursive_mutex& mutex() { return g_mu; }
+int init() { return init_locked(-1); }
+hipStream_t stream() { return g.stream; }
+int error(int code, const std::string& msg) { return fail(code, msg); }
+int twiddles(int lb, int f64, const void** out) { return ::twiddles(lb, f64, out); }
+int fft_t2_table(const void** out) { return ::fft_t2_table(out); }
+int timed(const char* name, const std::function<hipError_t()>& launch) {
+  return ::timed(name, launch);
+}
+}  // namespace rt
+}  // namespace cm
 
 extern "C" {
 

@@ -186,22 +186,27 @@ __global__ void __launch_bounds__(NT) k_compress_lds(CompressArgs a) {
 // sum of the per-chunk partial spectra, written over chunk 0 (each thread
 // owns one element and reads every chunk in order: deterministic)
 template <typename T, typename ACC>
-__global__ void __launch_bounds__(NT) k_chunk_sum(T* __restrict__ P, int64_t elems, int nchunks) {
-  const int64_t idx = int64_t(blockIdx.x) * NT + threadIdx.x;
+__device__ __forceinline__ void chunk_sum_body(T* __restrict__ P, int64_t elems, int nchunks, int blk) {
+  const int64_t idx = int64_t(blk) * NT + threadIdx.x;
   if (idx >= elems) return;
   ACC s = ACC(0);
   for (int c = 0; c < nchunks; ++c) s += ACC(P[c * elems + idx]);
   P[idx] = T(s);
 }
 
+template <typename T, typename ACC>
+__global__ void __launch_bounds__(NT) k_chunk_sum(T* __restrict__ P, int64_t elems, int nchunks) {
+  chunk_sum_body<T, ACC>(P, elems, nchunks, blockIdx.x);
+}
+
 template <typename T, typename ACC, bool FWHT>
-__global__ void __launch_bounds__(NT) k_reduce_inverse(ReduceArgs a) {
+__device__ __forceinline__ void reduce_body(const ReduceArgs& a, int blk) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int b2 = 1 << a.lb2;
   const int lbs = a.lb - a.lb2;
   const int b1 = 1 << lbs;
-  const int slice = blockIdx.x & (b1 - 1);
-  const int t = blockIdx.x >> lbs;
+  const int slice = blk & (b1 - 1);
+  const int t = blk >> lbs;
   const int tid = threadIdx.x;
   const int64_t chunk_stride = int64_t(a.d) * b1 * b2;
   const int64_t off = (int64_t(t) * b1 + slice) * b2;
@@ -237,37 +242,94 @@ __global__ void __launch_bounds__(NT) k_reduce_inverse(ReduceArgs a) {
   }
 }
 
-// p[t][h] from the per-slice inverse transforms V[t][slice][h mod b2]
+template <typename T, typename ACC, bool FWHT>
+__global__ void __launch_bounds__(NT) k_reduce_inverse(ReduceArgs a) {
+  reduce_body<T, ACC, FWHT>(a, blockIdx.x);
+}
+
+// p[t][h] from the per-slice inverse transforms V[t][slice][h mod b2].  A
+// zero-padded product (lb_out < lb) sums the 2^(lb - lb_out) positions
+// h = ho + f 2^lb_out: the wrapped part of the linear convolution for FFT,
+// exact zeros for FWHT.
 template <typename T, bool FWHT>
-__global__ void __launch_bounds__(NT) k_combine(CombineArgs a) {
+__device__ __forceinline__ void combine_body(const CombineArgs& a, int64_t idx) {
+  const int64_t bo = int64_t(1) << a.lb_out;
+  if (idx >= int64_t(a.d) * bo) return;
   const int64_t b = int64_t(1) << a.lb;
-  const int64_t idx = int64_t(blockIdx.x) * NT + threadIdx.x;
-  if (idx >= int64_t(a.d) * b) return;
-  const int t = int(idx >> a.lb);
-  const uint32_t h = uint32_t(idx & (b - 1));
+  const int t = int(idx >> a.lb_out);
+  const uint32_t ho = uint32_t(idx & (bo - 1));
   const int b2 = 1 << a.lb2;
   const int b1 = 1 << (a.lb - a.lb2);
-  const uint32_t lo = h & uint32_t(b2 - 1), hi = h >> a.lb2;
   T* P = static_cast<T*>(a.P);
-  if constexpr (FWHT) {
-    const T* V = static_cast<const T*>(a.V) + int64_t(t) * b;
-    T s = T(0);
-    for (int sl = 0; sl < b1; ++sl) {
-      const T v = V[int64_t(sl) * b2 + lo];
-      s += (__popc(uint32_t(sl) & hi) & 1) ? -v : v;
+  T total = T(0);
+  for (int64_t f = 0; f < (b >> a.lb_out); ++f) {
+    const uint32_t h = ho + uint32_t(f << a.lb_out);
+    const uint32_t lo = h & uint32_t(b2 - 1), hi = h >> a.lb2;
+    if constexpr (FWHT) {
+      const T* V = static_cast<const T*>(a.V) + int64_t(t) * b;
+      T s = T(0);
+      for (int sl = 0; sl < b1; ++sl) {
+        const T v = V[int64_t(sl) * b2 + lo];
+        s += (__popc(uint32_t(sl) & hi) & 1) ? -v : v;
+      }
+      total += s * (T(1) / T(b));
+    } else {
+      const cpx<T>* V = static_cast<const cpx<T>*>(a.V) + int64_t(t) * b;
+      const cpx<T>* tw = static_cast<const cpx<T>*>(a.tw);
+      T im = T(0);
+      for (int sl = 0; sl < b1; ++sl) {
+        const cpx<T> v = V[int64_t(sl) * b2 + lo];
+        const cpx<T> w = tw[(uint64_t(sl) * h) & uint64_t(b - 1)];
+        im += v.y * w.x - v.x * w.y;  // Im(v * conj(w))
+      }
+      total += im * (T(1) / T(2 * b));
     }
-    P[idx] = s * (T(1) / T(b));
-  } else {
-    const cpx<T>* V = static_cast<const cpx<T>*>(a.V) + int64_t(t) * b;
-    const cpx<T>* tw = static_cast<const cpx<T>*>(a.tw);
-    T im = T(0);
-    for (int sl = 0; sl < b1; ++sl) {
-      const cpx<T> v = V[int64_t(sl) * b2 + lo];
-      const cpx<T> w = tw[(uint64_t(sl) * h) & uint64_t(b - 1)];
-      im += v.y * w.x - v.x * w.y;  // Im(v * conj(w))
-    }
-    P[idx] = im * (T(1) / T(2 * b));
   }
+  P[idx] = total;
+}
+
+template <typename T, bool FWHT>
+__global__ void __launch_bounds__(NT) k_combine(CombineArgs a) {
+  combine_body<T, FWHT>(a, int64_t(blockIdx.x) * NT + threadIdx.x);
+}
+
+// ---------------------------------------------------------------- batches
+// FP32 jobs of either variant; the variant and the reduction width are read
+// per workgroup.
+__global__ void __launch_bounds__(NT) k_chunk_sum_batch(const FinishJob* __restrict__ jobs,
+                                                        const int2* __restrict__ blocks) {
+  const int2 e = blocks[blockIdx.x];
+  const ReduceArgs& r = jobs[e.x].r;
+  const int64_t elems = (int64_t(r.d) << r.lb) * (r.fwht ? 1 : 2);
+  float* P = static_cast<float*>(const_cast<void*>(r.partial));
+  if (r.f64_reduce)
+    chunk_sum_body<float, double>(P, elems, r.nchunks, e.y);
+  else
+    chunk_sum_body<float, float>(P, elems, r.nchunks, e.y);
+}
+
+// the partials are already summed over chunks (chunk 0 holds the sum; no
+// chunk at all for an empty inner dimension)
+__global__ void __launch_bounds__(NT) k_reduce_batch(const FinishJob* __restrict__ jobs,
+                                                     const int2* __restrict__ blocks) {
+  const int2 e = blocks[blockIdx.x];
+  ReduceArgs r = jobs[e.x].r;
+  r.nchunks = r.nchunks > 0 ? 1 : 0;
+  if (r.fwht)
+    reduce_body<float, float, true>(r, e.y);
+  else
+    reduce_body<float, float, false>(r, e.y);
+}
+
+__global__ void __launch_bounds__(NT) k_combine_batch(const FinishJob* __restrict__ jobs,
+                                                      const int2* __restrict__ blocks) {
+  const int2 e = blocks[blockIdx.x];
+  const CombineArgs& c = jobs[e.x].c;
+  const int64_t idx = int64_t(e.y) * NT + threadIdx.x;
+  if (c.fwht)
+    combine_body<float, true>(c, idx);
+  else
+    combine_body<float, false>(c, idx);
 }
 
 template <typename T, bool FWHT>
@@ -306,7 +368,7 @@ hipError_t reduce_t(const ReduceArgs& a0, hipStream_t s) {
 
 template <typename T, bool FWHT>
 hipError_t combine_t(const CombineArgs& a, hipStream_t s) {
-  const int64_t total = int64_t(a.d) << a.lb;
+  const int64_t total = int64_t(a.d) << a.lb_out;
   k_combine<T, FWHT><<<dim3(unsigned((total + NT - 1) / NT)), dim3(NT), 0, s>>>(a);
   return hipGetLastError();
 }
@@ -327,6 +389,22 @@ hipError_t launch_compress_lds(const CompressArgs& a, hipStream_t s) {
 hipError_t launch_reduce_inverse(const ReduceArgs& a, hipStream_t s) {
   if (a.f64) return a.fwht ? reduce_t<double, true>(a, s) : reduce_t<double, false>(a, s);
   return a.fwht ? reduce_t<float, true>(a, s) : reduce_t<float, false>(a, s);
+}
+
+int finish_batch_blocks(const FinishJob& j, int* sum_blocks, int* red_blocks) {
+  const int64_t elems = (int64_t(j.r.d) << j.r.lb) * (j.r.fwht ? 1 : 2);
+  *sum_blocks = j.r.nchunks > 1 ? int((elems + NT - 1) / NT) : 0;
+  *red_blocks = j.r.d << (j.r.lb - j.r.lb2);
+  return int(((int64_t(j.c.d) << j.c.lb_out) + NT - 1) / NT);
+}
+
+hipError_t launch_finish_batch(const FinishJob* jobs, const int2* sum_blocks, int n_sum,
+                               const int2* red_blocks, int n_red, size_t red_lds,
+                               const int2* comb_blocks, int n_comb, hipStream_t s) {
+  if (n_sum > 0) k_chunk_sum_batch<<<dim3(unsigned(n_sum)), dim3(NT), 0, s>>>(jobs, sum_blocks);
+  if (n_red > 0) k_reduce_batch<<<dim3(unsigned(n_red)), dim3(NT), red_lds, s>>>(jobs, red_blocks);
+  if (n_comb > 0) k_combine_batch<<<dim3(unsigned(n_comb)), dim3(NT), 0, s>>>(jobs, comb_blocks);
+  return hipGetLastError();
 }
 
 hipError_t launch_combine(const CombineArgs& a, hipStream_t s) {

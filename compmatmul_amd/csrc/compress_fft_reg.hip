@@ -55,17 +55,39 @@ constexpr float W64I[64] = {0.0f, -9.801714033e-02f, -1.950903220e-01f, -2.90284
 // are laid out round by round (round 0 plain writes, rounds 1..RCAP-2
 // read-add-writes, the last round LDS atomics) with every round's group
 // count in segcnt.
-__global__ void __launch_bounds__(64) k_fft_lists(const uint32_t* __restrict__ hA,
-                                                  const uint32_t* __restrict__ hB, int m, int p,
-                                                  int lb, int lbs, const float2* __restrict__ tw,
-                                                  int64_t cap, uint32_t* __restrict__ L,
-                                                  float2* __restrict__ C,
-                                                  int* __restrict__ segcnt) {
+__host__ __device__ __forceinline__ int64_t list_cap(int keys) {
+  const int64_t segs = (int64_t(keys) + SEG - 1) / SEG;
+  return int64_t(keys) + segs * 64 * RCAP;
+}
+
+struct ListLayout {
+  float2* C; uint32_t* L; int* segcnt; int64_t cap; int lists;
+};
+
+__host__ __device__ __forceinline__ ListLayout list_layout(const RegJob& j) {
+  ListLayout o;
+  o.lists = j.d << j.lbs;
+  o.cap = list_cap(j.m + j.p);
+  o.C = static_cast<float2*>(j.lists);
+  o.L = reinterpret_cast<uint32_t*>(o.C + o.lists * o.cap);
+  o.segcnt = reinterpret_cast<int*>(o.L + o.lists * o.cap);
+  return o;
+}
+
+__device__ __forceinline__ void fft_lists_body(const RegJob& j, int ts) {
+  const ListLayout lay = list_layout(j);
+  const uint32_t* __restrict__ hA = j.hA;
+  const uint32_t* __restrict__ hB = j.hB;
+  const int m = j.m, p = j.p, lb = j.lb, lbs = j.lbs;
+  const float2* __restrict__ tw = static_cast<const float2*>(j.tw);
+  const int64_t cap = lay.cap;
+  uint32_t* __restrict__ L = lay.L;
+  float2* __restrict__ C = lay.C;
+  int* __restrict__ segcnt = lay.segcnt;
   __shared__ uint32_t cnt[B2];
   __shared__ uint8_t rnd[SEG];
   __shared__ int nr[RCAP];
   const int b1 = 1 << lbs;
-  const int ts = blockIdx.x;
   const int t = ts >> lbs;
   const uint64_t slice = uint64_t(ts & (b1 - 1));
   const uint64_t bmask = (uint64_t(1) << lb) - 1;
@@ -139,6 +161,14 @@ __global__ void __launch_bounds__(64) k_fft_lists(const uint32_t* __restrict__ h
     out = acc;
     __syncthreads();
   }
+}
+
+__global__ void __launch_bounds__(64) k_fft_lists(RegJob j) { fft_lists_body(j, blockIdx.x); }
+
+__global__ void __launch_bounds__(64) k_fft_lists_batch(const RegJob* __restrict__ jobs,
+                                                        const int2* __restrict__ blocks) {
+  const int2 e = blocks[blockIdx.x];
+  fft_lists_body(jobs[e.x], e.y);
 }
 
 // ---------------------------------------------------------------- transforms
@@ -342,40 +372,29 @@ __device__ __forceinline__ int fq_of(int L, int q) {
   return (L & 31) + 32 * (2 * brev5(q) + (L >> 5));
 }
 
-__global__ void __launch_bounds__(64) k_compress_fft_reg(CompressArgs a,
-                                                         const uint32_t* __restrict__ L,
-                                                         const float2* __restrict__ C,
-                                                         int64_t cap,
-                                                         const int* __restrict__ segcnt,
-                                                         const float2* __restrict__ t2,
-                                                         int units) {
-  __shared__ __align__(16) float2 S[SLEN];
-  __shared__ __align__(16) float Rb[SEG];
-  const int lbs = a.lb - LB2;
-  const int b1 = 1 << lbs;
-  const int G = gridDim.x;
-  const int g = blockIdx.x;
-  const int u = (g & 7) * (G >> 3) + (g >> 3);  // XCD-aware numbering
-  if (u >= units) return;
-  const int per_chunk = a.d * b1;
+// One unit: (chunk of k, t, slice) = u of job j.
+__device__ __forceinline__ void fft_unit(const RegJob& j, int u, float2* S, float* Rb) {
+  const ListLayout lay = list_layout(j);
+  const int per_chunk = lay.lists;
   const int chunk = u / per_chunk;
   const int ts = u - chunk * per_chunk;  // t * b1 + slice
   const int lane = threadIdx.x;
-  const int k0 = chunk * a.kper;
-  const int k1 = min(a.n, k0 + a.kper);
-  const float* __restrict__ AT = static_cast<const float*>(a.AT);
-  const float* __restrict__ B = static_cast<const float*>(a.B);
-  const uint32_t* __restrict__ ls = L + ts * cap;
-  const float2* __restrict__ cs = C + ts * cap;
-  const int* __restrict__ sc = segcnt + int64_t(ts) * MAXSEG * RCAP;
-  const int keys = a.m + a.p;
+  const int k0 = chunk * j.kper;
+  const int k1 = min(j.n, k0 + j.kper);
+  const float* __restrict__ AT = j.AT;
+  const float* __restrict__ B = j.B;
+  const uint32_t* __restrict__ ls = lay.L + ts * lay.cap;
+  const float2* __restrict__ cs = lay.C + ts * lay.cap;
+  const int* __restrict__ sc = lay.segcnt + int64_t(ts) * MAXSEG * RCAP;
+  const float2* __restrict__ t2 = static_cast<const float2*>(j.t2);
+  const int keys = j.m + j.p;
 
   float qr[R], qi[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) qr[r] = qi[r] = 0.f;
   for (int k = k0; k < k1; ++k) {
     float re[R], im[R];
-    sketch_fft(AT + int64_t(k) * a.ld_at, B + int64_t(k) * a.ld_b, a.m, keys, ls, cs, sc, t2, S,
+    sketch_fft(AT + int64_t(k) * j.ld_at, B + int64_t(k) * j.ld_b, j.m, keys, ls, cs, sc, t2, S,
                Rb, lane, re, im);
 #pragma unroll
     for (int r = 0; r < R; ++r) {  // Q += z^2
@@ -387,26 +406,61 @@ __global__ void __launch_bounds__(64) k_compress_fft_reg(CompressArgs a,
 #pragma unroll
   for (int r = 0; r < R; ++r) S[fq_of(lane, r)] = make_float2(qr[r], qi[r]);
   __syncthreads();
-  float2* out = reinterpret_cast<float2*>(static_cast<float*>(a.partial)) +
-                (int64_t(chunk) * per_chunk + ts) * B2;
+  float2* out = reinterpret_cast<float2*>(j.partial) + (int64_t(chunk) * per_chunk + ts) * B2;
 #pragma unroll
   for (int r = 0; r < R; ++r) out[r * 64 + lane] = S[r * 64 + lane];
 }
 
-int64_t list_cap(int keys) {
-  const int64_t segs = (int64_t(keys) + SEG - 1) / SEG;
-  return int64_t(keys) + segs * 64 * RCAP;
+// Units are numbered XCD-aware (workgroup g runs on XCD g % 8 and takes a
+// contiguous range of units, so the units of one chunk share an L2).  The
+// loop form (one pass when the grid covers every unit) keeps the register
+// allocation at two waves per SIMD.
+__global__ void __launch_bounds__(64) k_compress_fft_reg(RegJob j, int units) {
+  __shared__ __align__(16) float2 S[SLEN];
+  __shared__ __align__(16) float Rb[SEG];
+  const int G = gridDim.x;
+  const int per = (units + G - 1) / G;
+  const int x = blockIdx.x & 7, w = blockIdx.x >> 3;
+  const int gx = G >> 3;
+  for (int r = 0; r < per; ++r) {
+    const int u = x * (gx * per) + r * gx + w;
+    if (u >= units) break;
+    fft_unit(j, u, S, Rb);
+    __syncthreads();
+  }
+}
+
+// Persistent form for many products (queues as in the FWHT batch kernel).
+__global__ void __launch_bounds__(64) k_compress_fft_batch(const RegJob* __restrict__ jobs,
+                                                           const int2* __restrict__ items,
+                                                           const int* __restrict__ qstart,
+                                                           unsigned* __restrict__ counters) {
+  __shared__ __align__(16) float2 S[SLEN];
+  __shared__ __align__(16) float Rb[SEG];
+  const int home = blockIdx.x & 7;
+  for (int qi = 0; qi < 8; ++qi) {
+    const int q = (home + qi) & 7;
+    const int base = qstart[q];
+    const int len = qstart[q + 1] - base;
+    for (;;) {
+      unsigned it = 0;
+      if (threadIdx.x == 0) it = atomicAdd(&counters[q], 1u);
+      it = __builtin_amdgcn_readfirstlane(it);
+      if (int(it) >= len) break;
+      const int2 e = items[base + int(it)];
+      fft_unit(jobs[e.x], e.y, S, Rb);
+      __syncthreads();
+    }
+  }
 }
 
 }  // namespace
 
-int compress_fft_reg_supported(int lb, int m, int p) {
-  return lb >= LB2 && lb <= 30 && int64_t(m) + p <= int64_t(MAXSEG) * SEG;
+int fft_reg_supported(int lb, int m, int p) {
+  return lb >= 1 && lb <= 30 && int64_t(m) + p <= int64_t(MAXSEG) * SEG;
 }
 
-int compress_fft_reg_lb2() { return LB2; }
-
-int compress_fft_reg_slots() {
+int fft_reg_slots() {
   int dev = 0, cus = 0, per_cu = 0;
   if (hipGetDevice(&dev) != hipSuccess ||
       hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
@@ -416,27 +470,32 @@ int compress_fft_reg_slots() {
   return cus * per_cu;
 }
 
-size_t compress_fft_reg_scratch_bytes(int d, int lb, int m, int p) {
-  const size_t lists = size_t(d) << (lb - LB2);
-  return lists * size_t(list_cap(m + p)) * 12 + lists * MAXSEG * RCAP * sizeof(int) +
-         size_t(B2) * sizeof(float2) + 256;
+size_t fft_reg_lists_bytes(int d, int lb, int m, int p) {
+  const size_t lists = size_t(d) << reg_lbs(lb);
+  return lists * size_t(list_cap(m + p)) * 12 + lists * MAXSEG * RCAP * sizeof(int) + 256;
 }
 
-hipError_t launch_compress_fft_reg(const CompressArgs& a, void* scratch, const void* t2,
-                                   hipStream_t s) {
-  if (a.nchunks == 0) return hipSuccess;
-  const int lbs = a.lb - LB2;
-  const int lists = a.d << lbs;
-  const int64_t cap = list_cap(a.m + a.p);
-  float2* C = static_cast<float2*>(scratch);
-  uint32_t* L = reinterpret_cast<uint32_t*>(C + lists * cap);
-  int* sc = reinterpret_cast<int*>(L + lists * cap);
-  k_fft_lists<<<dim3(unsigned(lists)), dim3(64), 0, s>>>(
-      a.hA, a.hB, a.m, a.p, a.lb, lbs, static_cast<const float2*>(a.tw), cap, L, C, sc);
-  const int units = a.nchunks * lists;
+hipError_t launch_fft_reg(const RegJob& j, hipStream_t s) {
+  if (j.nchunks == 0) return hipSuccess;
+  const int lists = j.d << j.lbs;
+  k_fft_lists<<<dim3(unsigned(lists)), dim3(64), 0, s>>>(j);
+  const int units = j.nchunks * lists;
   const int grid = (units + 7) / 8 * 8;
-  k_compress_fft_reg<<<dim3(unsigned(grid)), dim3(64), 0, s>>>(
-      a, L, C, cap, sc, static_cast<const float2*>(t2), units);
+  k_compress_fft_reg<<<dim3(unsigned(grid)), dim3(64), 0, s>>>(j, units);
+  return hipGetLastError();
+}
+
+hipError_t launch_fft_reg_lists_batch(const RegJob* jobs, const int2* blocks, int nblocks,
+                                      hipStream_t s) {
+  if (nblocks <= 0) return hipSuccess;
+  k_fft_lists_batch<<<dim3(unsigned(nblocks)), dim3(64), 0, s>>>(jobs, blocks);
+  return hipGetLastError();
+}
+
+hipError_t launch_fft_reg_batch(const RegJob* jobs, const int2* items, const int* qstart,
+                                unsigned* counters, int grid, hipStream_t s) {
+  if (grid <= 0) return hipSuccess;
+  k_compress_fft_batch<<<dim3(unsigned(grid)), dim3(64), 0, s>>>(jobs, items, qstart, counters);
   return hipGetLastError();
 }
 

@@ -62,18 +62,41 @@ constexpr int RCAP = 8;
 constexpr int MAXSEG = 64;  // keys <= MAXSEG * SEG = 65536
 constexpr uint32_t F_SIGN = 1u << 31;
 
-__global__ void __launch_bounds__(64) k_reg_lists(const uint32_t* __restrict__ hA,
-                                                  const uint32_t* __restrict__ hB, int m, int p,
-                                                  int d, int lbs, int64_t cap_a, int64_t cap_b,
-                                                  uint32_t* __restrict__ LA,
-                                                  uint32_t* __restrict__ LB,
-                                                  int* __restrict__ ngroups) {
+__host__ __device__ __forceinline__ int64_t list_cap(int keys) {
+  const int64_t segs = (int64_t(keys) + SEG - 1) / SEG;
+  return int64_t(keys) + segs * 64 * RCAP;
+}
+
+struct ListLayout {
+  uint32_t* LA; uint32_t* LB; int* ngroups; int64_t cap_a, cap_b; int lists;
+};
+
+__host__ __device__ __forceinline__ ListLayout list_layout(const RegJob& j) {
+  ListLayout o;
+  o.lists = j.d << j.lbs;
+  o.cap_a = list_cap(j.m);
+  o.cap_b = list_cap(j.p);
+  o.LA = static_cast<uint32_t*>(j.lists);
+  o.LB = o.LA + o.lists * o.cap_a;
+  o.ngroups = reinterpret_cast<int*>(o.LB + o.lists * o.cap_b);
+  return o;
+}
+
+// schedule of list g: g < lists covers A's row for (t, slice) = g, the rest B's
+__device__ __forceinline__ void reg_lists_body(const RegJob& j, int g) {
+  const ListLayout lay = list_layout(j);
+  const int lbs = j.lbs, m = j.m, p = j.p;
+  const uint32_t* __restrict__ hA = j.hA;
+  const uint32_t* __restrict__ hB = j.hB;
+  const int64_t cap_a = lay.cap_a, cap_b = lay.cap_b;
+  uint32_t* __restrict__ LA = lay.LA;
+  uint32_t* __restrict__ LB = lay.LB;
+  int* __restrict__ ngroups = lay.ngroups;
   __shared__ uint32_t cnt[B2];
   __shared__ uint8_t rnd[SEG];
   __shared__ int nr[RCAP];
   const int b1 = 1 << lbs;
-  const int lists = d * b1;
-  const int g = blockIdx.x;
+  const int lists = lay.lists;
   const bool side_b = g >= lists;
   const int ts = side_b ? g - lists : g;
   const int t = ts >> lbs;
@@ -140,6 +163,14 @@ __global__ void __launch_bounds__(64) k_reg_lists(const uint32_t* __restrict__ h
     out = acc;
     __syncthreads();
   }
+}
+
+__global__ void __launch_bounds__(64) k_reg_lists(RegJob j) { reg_lists_body(j, blockIdx.x); }
+
+__global__ void __launch_bounds__(64) k_reg_lists_batch(const RegJob* __restrict__ jobs,
+                                                        const int2* __restrict__ blocks) {
+  const int2 e = blocks[blockIdx.x];
+  reg_lists_body(jobs[e.x], e.y);
 }
 
 __device__ __forceinline__ void wht_reg(float (&x)[R]) {
@@ -265,42 +296,29 @@ __device__ __forceinline__ int f_of(int L, int r) {
   return ((L & 31) << 6) | ((r & 1) << 5) | ((r & ~1) + (L >> 5));
 }
 
-__global__ void __launch_bounds__(64) k_compress_fwht_reg(CompressArgs a,
-                                                          const uint32_t* __restrict__ LA,
-                                                          const uint32_t* __restrict__ LB,
-                                                          int64_t cap_a, int64_t cap_b,
-                                                          const int* __restrict__ ngroups,
-                                                          int units) {
-  __shared__ __align__(16) float S[SLEN];
-  __shared__ __align__(16) float Rb[SEG];
-  const int lbs = a.lb - LB2;
-  const int b1 = 1 << lbs;
-  // XCD-aware numbering: workgroup g runs on XCD g % 8; give each XCD a
-  // contiguous range of units so the units of one chunk share an L2
-  const int G = gridDim.x;
-  const int g = blockIdx.x;
-  const int u = (g & 7) * (G >> 3) + (g >> 3);
-  if (u >= units) return;
-  const int per_chunk = a.d * b1;
+// One unit: (chunk of k, t, slice) = u of job j, over k in [chunk * kper, ...).
+__device__ __forceinline__ void fwht_unit(const RegJob& j, int u, float* S, float* Rb) {
+  const ListLayout lay = list_layout(j);
+  const int per_chunk = lay.lists;
   const int chunk = u / per_chunk;
   const int ts = u - chunk * per_chunk;  // t * b1 + slice
   const int lane = threadIdx.x;
-  const int k0 = chunk * a.kper;
-  const int k1 = min(a.n, k0 + a.kper);
-  const float* __restrict__ AT = static_cast<const float*>(a.AT);
-  const float* __restrict__ B = static_cast<const float*>(a.B);
-  const uint32_t* __restrict__ la = LA + ts * cap_a;
-  const uint32_t* __restrict__ lb = LB + ts * cap_b;
-  const int* __restrict__ sca = ngroups + int64_t(ts) * MAXSEG * RCAP;
-  const int* __restrict__ scb = ngroups + int64_t(per_chunk + ts) * MAXSEG * RCAP;
+  const int k0 = chunk * j.kper;
+  const int k1 = min(j.n, k0 + j.kper);
+  const float* __restrict__ AT = j.AT;
+  const float* __restrict__ B = j.B;
+  const uint32_t* __restrict__ la = lay.LA + ts * lay.cap_a;
+  const uint32_t* __restrict__ lb = lay.LB + ts * lay.cap_b;
+  const int* __restrict__ sca = lay.ngroups + int64_t(ts) * MAXSEG * RCAP;
+  const int* __restrict__ scb = lay.ngroups + int64_t(per_chunk + ts) * MAXSEG * RCAP;
 
   float acc[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) acc[r] = 0.f;
   for (int k = k0; k < k1; ++k) {
     float xa[R], xb[R];
-    sketch_transform(AT + int64_t(k) * a.ld_at, a.m, la, sca, S, Rb, lane, xa);
-    sketch_transform(B + int64_t(k) * a.ld_b, a.p, lb, scb, S, Rb, lane, xb);
+    sketch_transform(AT + int64_t(k) * j.ld_at, j.m, la, sca, S, Rb, lane, xa);
+    sketch_transform(B + int64_t(k) * j.ld_b, j.p, lb, scb, S, Rb, lane, xb);
 #pragma unroll
     for (int r = 0; r < R; ++r) acc[r] = fmaf(xa[r], xb[r], acc[r]);
   }
@@ -309,25 +327,60 @@ __global__ void __launch_bounds__(64) k_compress_fwht_reg(CompressArgs a,
 #pragma unroll
   for (int r = 0; r < R; ++r) S[f_of(lane, r)] = acc[r];
   __syncthreads();
-  float* out = static_cast<float*>(a.partial) + (int64_t(chunk) * per_chunk + ts) * B2;
+  float* out = j.partial + (int64_t(chunk) * per_chunk + ts) * B2;
 #pragma unroll
   for (int r = 0; r < R; ++r) out[r * 64 + lane] = S[r * 64 + lane];
 }
 
-int64_t list_cap(int keys) {
-  const int64_t segs = (int64_t(keys) + SEG - 1) / SEG;
-  return int64_t(keys) + segs * 64 * RCAP;
+__global__ void __launch_bounds__(64) k_compress_fwht_reg(RegJob j, int units) {
+  __shared__ __align__(16) float S[SLEN];
+  __shared__ __align__(16) float Rb[SEG];
+  // XCD-aware numbering: workgroup g runs on XCD g % 8; give each XCD a
+  // contiguous range of units so the units of one chunk share an L2
+  const int G = gridDim.x;
+  const int g = blockIdx.x;
+  const int u = (g & 7) * (G >> 3) + (g >> 3);
+  if (u >= units) return;
+  fwht_unit(j, u, S, Rb);
+}
+
+// Persistent form for many products: every workgroup takes items from the
+// queue of its XCD until it is empty, then from the other queues in turn.
+__global__ void __launch_bounds__(64) k_compress_fwht_batch(const RegJob* __restrict__ jobs,
+                                                            const int2* __restrict__ items,
+                                                            const int* __restrict__ qstart,
+                                                            unsigned* __restrict__ counters) {
+  __shared__ __align__(16) float S[SLEN];
+  __shared__ __align__(16) float Rb[SEG];
+  const int home = blockIdx.x & 7;
+  for (int qi = 0; qi < 8; ++qi) {
+    const int q = (home + qi) & 7;
+    const int base = qstart[q];
+    const int len = qstart[q + 1] - base;
+    for (;;) {
+      unsigned it = 0;
+      if (threadIdx.x == 0) it = atomicAdd(&counters[q], 1u);
+      it = __builtin_amdgcn_readfirstlane(it);
+      if (int(it) >= len) break;
+      const int2 e = items[base + int(it)];
+      fwht_unit(jobs[e.x], e.y, S, Rb);
+      __syncthreads();  // the next unit rewrites S and Rb
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256) k_zero_u32(unsigned* __restrict__ p, int n) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) p[i] = 0u;
 }
 
 }  // namespace
 
-int compress_fwht_reg_supported(int lb, int m, int p) {
-  return lb >= LB2 && lb <= 30 && m <= MAXSEG * SEG && p <= MAXSEG * SEG;
+int fwht_reg_supported(int lb, int m, int p) {
+  return lb >= 1 && lb <= 30 && m <= MAXSEG * SEG && p <= MAXSEG * SEG;
 }
 
-int compress_fwht_reg_lb2() { return LB2; }
-
-int compress_fwht_reg_slots() {
+int fwht_reg_slots() {
   int dev = 0, cus = 0, per_cu = 0;
   if (hipGetDevice(&dev) != hipSuccess ||
       hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
@@ -337,24 +390,38 @@ int compress_fwht_reg_slots() {
   return cus * per_cu;
 }
 
-size_t compress_fwht_reg_entries_bytes(int d, int lb, int m, int p) {
-  const size_t lists = size_t(d) << (lb - LB2);
+size_t fwht_reg_lists_bytes(int d, int lb, int m, int p) {
+  const size_t lists = size_t(d) << reg_lbs(lb);
   return lists * size_t(list_cap(m) + list_cap(p)) * 4 + 2 * lists * MAXSEG * RCAP * sizeof(int) + 256;
 }
 
-hipError_t launch_compress_fwht_reg(const CompressArgs& a, void* entries, hipStream_t s) {
-  if (a.nchunks == 0) return hipSuccess;
-  const int lbs = a.lb - LB2;
-  const int lists = a.d << lbs;
-  const int64_t cap_a = list_cap(a.m), cap_b = list_cap(a.p);
-  uint32_t* LA = static_cast<uint32_t*>(entries);
-  uint32_t* LB = LA + lists * cap_a;
-  int* ng = reinterpret_cast<int*>(LB + lists * cap_b);
-  k_reg_lists<<<dim3(unsigned(2 * lists)), dim3(64), 0, s>>>(a.hA, a.hB, a.m, a.p, a.d, lbs, cap_a,
-                                                             cap_b, LA, LB, ng);
-  const int units = a.nchunks * lists;
+hipError_t launch_fwht_reg(const RegJob& j, hipStream_t s) {
+  if (j.nchunks == 0) return hipSuccess;
+  const int lists = j.d << j.lbs;
+  k_reg_lists<<<dim3(unsigned(2 * lists)), dim3(64), 0, s>>>(j);
+  const int units = j.nchunks * lists;
   const int grid = (units + 7) / 8 * 8;
-  k_compress_fwht_reg<<<dim3(unsigned(grid)), dim3(64), 0, s>>>(a, LA, LB, cap_a, cap_b, ng, units);
+  k_compress_fwht_reg<<<dim3(unsigned(grid)), dim3(64), 0, s>>>(j, units);
+  return hipGetLastError();
+}
+
+hipError_t launch_fwht_reg_lists_batch(const RegJob* jobs, const int2* blocks, int nblocks,
+                                       hipStream_t s) {
+  if (nblocks <= 0) return hipSuccess;
+  k_reg_lists_batch<<<dim3(unsigned(nblocks)), dim3(64), 0, s>>>(jobs, blocks);
+  return hipGetLastError();
+}
+
+hipError_t launch_fwht_reg_batch(const RegJob* jobs, const int2* items, const int* qstart,
+                                 unsigned* counters, int grid, hipStream_t s) {
+  if (grid <= 0) return hipSuccess;
+  k_compress_fwht_batch<<<dim3(unsigned(grid)), dim3(64), 0, s>>>(jobs, items, qstart, counters);
+  return hipGetLastError();
+}
+
+hipError_t launch_zero_u32(unsigned* p, int n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  k_zero_u32<<<dim3(unsigned((n + 255) / 256)), dim3(256), 0, s>>>(p, n);
   return hipGetLastError();
 }
 

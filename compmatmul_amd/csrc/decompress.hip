@@ -65,10 +65,10 @@ __device__ __forceinline__ void emit(const DecompressArgs& a, int i, int j, T me
 }
 
 template <typename T, bool FWHT, int D>
-__global__ void __launch_bounds__(NT) k_decompress(DecompressArgs a) {
+__device__ __forceinline__ void decompress_body(const DecompressArgs& a, int blk) {
   const int tiles = (a.p + NT - 1) / NT;
-  const int j = int(blockIdx.x % tiles) * NT + threadIdx.x;
-  const int i0 = int(blockIdx.x / tiles) * ROWS;
+  const int j = int(blk % tiles) * NT + threadIdx.x;
+  const int i0 = int(blk / tiles) * ROWS;
   if (j >= a.p) return;
   const T* __restrict__ P = static_cast<const T*>(a.P);
   const uint32_t mask = uint32_t(a.b - 1);
@@ -85,6 +85,11 @@ __global__ void __launch_bounds__(NT) k_decompress(DecompressArgs a) {
   }
 }
 
+template <typename T, bool FWHT, int D>
+__global__ void __launch_bounds__(NT) k_decompress(DecompressArgs a) {
+  decompress_body<T, FWHT, D>(a, blockIdx.x);
+}
+
 // LDS-staged variant for sketch rows of at most 64 KB: the workgroup (512
 // threads, one column each, up to 12 rows per thread) streams the d rows of the
 // sketch through two LDS buffers (row t+1 is fetched while row t is
@@ -97,15 +102,15 @@ constexpr int rows_per_thread() {
 }
 
 template <typename T, bool FWHT, int D>
-__global__ void __launch_bounds__(LNT) k_decompress_lds(DecompressArgs a) {
+__device__ __forceinline__ void decompress_lds_body(const DecompressArgs& a, int blk) {
   constexpr int ER = rows_per_thread<T, D>();
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int b = a.b;
   T* const buf0 = reinterpret_cast<T*>(smem_raw);
   const int tiles = (a.p + LNT - 1) / LNT;
   const int tid = threadIdx.x;
-  const int j = int(blockIdx.x % tiles) * LNT + tid;
-  const int i0 = int(blockIdx.x / tiles) * ER;
+  const int j = int(blk % tiles) * LNT + tid;
+  const int i0 = int(blk / tiles) * ER;
   const bool jok = j < a.p;
   const uint32_t mask = uint32_t(b - 1);
   const T* __restrict__ P = static_cast<const T*>(a.P);
@@ -151,6 +156,25 @@ __global__ void __launch_bounds__(LNT) k_decompress_lds(DecompressArgs a) {
     if (i0 + e >= a.m) break;
     emit<T>(a, i0 + e, j, median_network<T, D>(v[e]));
   }
+}
+
+template <typename T, bool FWHT, int D>
+__global__ void __launch_bounds__(LNT) k_decompress_lds(DecompressArgs a) {
+  decompress_lds_body<T, FWHT, D>(a, blockIdx.x);
+}
+
+template <bool FWHT, int D>
+__global__ void __launch_bounds__(NT) k_decompress_batch(const DecompressArgs* __restrict__ jobs,
+                                                         const int2* __restrict__ blocks) {
+  const int2 e = blocks[blockIdx.x];
+  decompress_body<float, FWHT, D>(jobs[e.x], e.y);
+}
+
+template <bool FWHT, int D>
+__global__ void __launch_bounds__(LNT) k_decompress_lds_batch(const DecompressArgs* __restrict__ jobs,
+                                                              const int2* __restrict__ blocks) {
+  const int2 e = blocks[blockIdx.x];
+  decompress_lds_body<float, FWHT, D>(jobs[e.x], e.y);
 }
 
 // any d: the value of rank d/2 by counting, O(d^2) gathers per entry
@@ -218,6 +242,44 @@ hipError_t launch_t(const DecompressArgs& a, hipStream_t s) {
 }
 
 }  // namespace
+
+int decompress_batch_blocks(int d, int staged, int m, int p) {
+  if (m <= 0 || p <= 0) return 0;
+  if (!staged) return int(int64_t((p + NT - 1) / NT) * ((m + ROWS - 1) / ROWS));
+  int er = 1;
+  switch (d) {
+#define CM_D(N) case N: er = rows_per_thread<float, N>(); break;
+    CM_D(1) CM_D(2) CM_D(3) CM_D(4) CM_D(5) CM_D(6) CM_D(7) CM_D(8)
+    CM_D(9) CM_D(10) CM_D(11) CM_D(12) CM_D(13) CM_D(14) CM_D(15) CM_D(16)
+#undef CM_D
+    default: return -1;
+  }
+  return int(int64_t((p + LNT - 1) / LNT) * ((m + er - 1) / er));
+}
+
+hipError_t launch_decompress_batch(int fwht, int d, int staged, size_t lds,
+                                   const DecompressArgs* jobs, const int2* blocks, int nblocks,
+                                   hipStream_t s) {
+  if (nblocks <= 0) return hipSuccess;
+  if (d < 1 || d > 16 || (staged && lds > 65536)) return hipErrorInvalidValue;
+  const dim3 grid = dim3(unsigned(nblocks));
+  switch (d) {
+#define CM_D(N)                                                                               \
+  case N:                                                                                     \
+    if (staged) {                                                                             \
+      if (fwht) k_decompress_lds_batch<true, N><<<grid, dim3(LNT), lds, s>>>(jobs, blocks);   \
+      else k_decompress_lds_batch<false, N><<<grid, dim3(LNT), lds, s>>>(jobs, blocks);       \
+    } else {                                                                                  \
+      if (fwht) k_decompress_batch<true, N><<<grid, dim3(NT), 0, s>>>(jobs, blocks);          \
+      else k_decompress_batch<false, N><<<grid, dim3(NT), 0, s>>>(jobs, blocks);              \
+    }                                                                                         \
+    break;
+    CM_D(1) CM_D(2) CM_D(3) CM_D(4) CM_D(5) CM_D(6) CM_D(7) CM_D(8)
+    CM_D(9) CM_D(10) CM_D(11) CM_D(12) CM_D(13) CM_D(14) CM_D(15) CM_D(16)
+#undef CM_D
+  }
+  return hipGetLastError();
+}
 
 hipError_t launch_decompress(const DecompressArgs& a, hipStream_t s) {
   if (a.m == 0 || a.p == 0) return hipSuccess;

@@ -49,8 +49,8 @@ __device__ __forceinline__ uint32_t eval_packed(const HashArgs& a, int fh, int f
 }
 
 // functions are stored h1[0..d), h2[0..d), s1[0..d), s2[0..d)
-__global__ void __launch_bounds__(256) k_hash(HashArgs a) {
-  const int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void hash_body(const HashArgs& a, int blk) {
+  const int64_t idx = int64_t(blk) * 256 + threadIdx.x;
   const int64_t na = int64_t(a.d) * a.m;
   const int64_t total = na + int64_t(a.d) * a.p;
   if (idx >= total) return;
@@ -66,7 +66,25 @@ __global__ void __launch_bounds__(256) k_hash(HashArgs a) {
   }
 }
 
+__global__ void __launch_bounds__(256) k_hash(HashArgs a) { hash_body(a, blockIdx.x); }
+
+__global__ void __launch_bounds__(256) k_hash_batch(const HashArgs* __restrict__ jobs,
+                                                    const int2* __restrict__ blocks) {
+  const int2 e = blocks[blockIdx.x];
+  hash_body(jobs[e.x], e.y);
+}
+
 }  // namespace
+
+int hash_batch_blocks(const HashArgs& a) {
+  return int((int64_t(a.d) * (int64_t(a.m) + a.p) + 255) / 256);
+}
+
+hipError_t launch_hash_batch(const HashArgs* jobs, const int2* blocks, int nblocks, hipStream_t s) {
+  if (nblocks <= 0) return hipSuccess;
+  k_hash_batch<<<dim3(unsigned(nblocks)), dim3(256), 0, s>>>(jobs, blocks);
+  return hipGetLastError();
+}
 
 hipError_t launch_hash(const HashArgs& a, hipStream_t s) {
   const int64_t total = int64_t(a.d) * (int64_t(a.m) + a.p);

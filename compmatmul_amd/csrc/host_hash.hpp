@@ -8,7 +8,7 @@
 //               :145-160 (multiply-add-shift, a then b), :277-295
 //               (tabulation tables filled row-major, low 32 bits per draw)
 //   sketch      matmul_sketch.hpp:15-18 (h1, h2, s1, s2 from seed+0..3)
-// The device evaluates the drawn functions (hash_kernels.hip).
+// The device evaluates the drawn functions (hash.hip).
 #pragma once
 #include <cstdint>
 #include <vector>

@@ -23,6 +23,10 @@ struct HashArgs {
   uint32_t* outB;      // d x p
 };
 hipError_t launch_hash(const HashArgs& a, hipStream_t s);
+// Many products (device table of HashArgs, use_inline = 0): blocks[g] =
+// (job, workgroup within the job), hash_batch_blocks() workgroups per job.
+int hash_batch_blocks(const HashArgs& a);
+hipError_t launch_hash_batch(const HashArgs* jobs, const int2* blocks, int nblocks, hipStream_t s);
 
 // ------------------------------------------------------------- compress
 // Sliced fused compress: the spectrum of length b = b1*b2 is cut into b1
@@ -42,25 +46,52 @@ struct CompressArgs {
 int compress_lds_max_lb2(int fwht, int f64);
 hipError_t launch_compress_lds(const CompressArgs& a, hipStream_t s);
 
-// Register-resident single-wave FP32 FWHT compress (fast path): slices of
-// 2^compress_fwht_reg_lb2() frequencies; `entries` is a scratch buffer of
-// compress_fwht_reg_entries_bytes() bytes for the packed scatter entries.
-int compress_fwht_reg_supported(int lb, int m, int p);
-int compress_fwht_reg_lb2();
-int compress_fwht_reg_slots();  // resident workgroups on the device
-size_t compress_fwht_reg_entries_bytes(int d, int lb, int m, int p);
-hipError_t launch_compress_fwht_reg(const CompressArgs& a, void* entries, hipStream_t s);
+// ------------------------------------------------- register-resident compress
+// FP32 fast paths (compress_reg.hip: FWHT, compress_fft_reg.hip: FFT).  One
+// 64-lane workgroup transforms 2048-frequency slices in registers; the
+// spectrum of length b = b1 * 2048 is cut into b1 = 2^lbs slices.  A product
+// with b < 2048 runs as one zero-padded 2048-point slice.  Padding is exact
+// for both variants: the padded XOR convolution repeats the length-b one,
+// and the padded cyclic convolution is the linear one, which the combine
+// kernel folds back to length b.
+constexpr int kRegLb2 = 11;
+struct RegJob {
+  const float* AT; int64_t ld_at;
+  const float* B; int64_t ld_b;
+  const uint32_t* hA; const uint32_t* hB;
+  int m, n, p, d, lb, lbs;  // lbs = reg_lbs(lb)
+  int kper, nchunks;        // inner indices per unit; units = nchunks * d << lbs
+  void* lists;              // scatter schedules (*_reg_lists_bytes())
+  const void* tw;           // FFT: W_b^j (float2), j < max(b, 2048)
+  const void* t2;           // FFT: W_2048^(L * bitrev5(q)) at [q * 64 + L]
+  float* partial;           // [nchunks][d][b1][2048] float (float2 for FFT)
+};
+inline int reg_lbs(int lb) { return lb > kRegLb2 ? lb - kRegLb2 : 0; }
 
-// Register-resident single-wave FP32 FFT compress (fast path): slices of
-// 2^compress_fft_reg_lb2() frequencies; `scratch` holds the scatter
-// schedules (compress_fft_reg_scratch_bytes()); `t2` is the table
-// W_2048^(L * bitrev5(q)) at [q * 64 + L] (float2, 2048 entries).
-int compress_fft_reg_supported(int lb, int m, int p);
-int compress_fft_reg_lb2();
-int compress_fft_reg_slots();
-size_t compress_fft_reg_scratch_bytes(int d, int lb, int m, int p);
-hipError_t launch_compress_fft_reg(const CompressArgs& a, void* scratch, const void* t2,
-                                   hipStream_t s);
+int fwht_reg_supported(int lb, int m, int p);
+int fwht_reg_slots();  // resident workgroups of the compress kernel
+size_t fwht_reg_lists_bytes(int d, int lb, int m, int p);
+hipError_t launch_fwht_reg(const RegJob& j, hipStream_t s);  // schedules + compress
+int fft_reg_supported(int lb, int m, int p);
+int fft_reg_slots();
+size_t fft_reg_lists_bytes(int d, int lb, int m, int p);
+hipError_t launch_fft_reg(const RegJob& j, hipStream_t s);
+
+// Many products.  Schedules: `blocks[g]` = (job, list) of workgroup g.
+// Compress: one persistent launch drains eight work queues, queue x holding
+// items[qstart[x] .. qstart[x + 1]) = (job, unit); the workgroups of XCD x
+// take queue x first, then the others.  counters[0..8) must be zero.
+hipError_t launch_fwht_reg_lists_batch(const RegJob* jobs, const int2* blocks, int nblocks,
+                                       hipStream_t s);
+hipError_t launch_fwht_reg_batch(const RegJob* jobs, const int2* items, const int* qstart,
+                                 unsigned* counters, int grid, hipStream_t s);
+hipError_t launch_fft_reg_lists_batch(const RegJob* jobs, const int2* blocks, int nblocks,
+                                      hipStream_t s);
+hipError_t launch_fft_reg_batch(const RegJob* jobs, const int2* items, const int* qstart,
+                                unsigned* counters, int grid, hipStream_t s);
+// lists per job for the schedule launch (2 per (t, slice) for FWHT: A and B)
+inline int reg_lists_count(int fwht, int d, int lb) { return (fwht ? 2 : 1) * (d << reg_lbs(lb)); }
+hipError_t launch_zero_u32(unsigned* p, int n, hipStream_t s);
 
 // Sum of the per-chunk partial spectra, then the per-slice inverse transform.
 struct ReduceArgs {
@@ -72,14 +103,25 @@ struct ReduceArgs {
 };
 hipError_t launch_reduce_inverse(const ReduceArgs& a, hipStream_t s);
 
-// Cross-slice combination and scaling into the sketch p (d x b).
+// Cross-slice combination and scaling into the sketch p (d x 2^lb_out).
+// lb_out < lb folds a zero-padded product (lb = 11) back to its length.
 struct CombineArgs {
   int fwht, f64;
   const void* V; int d, lb, lb2;
   const void* tw;
   void* P;
+  int lb_out;
 };
 hipError_t launch_combine(const CombineArgs& a, hipStream_t s);
+
+// Many FP32 products: chunk sum, per-slice inverse and combine of every job
+// in one launch each; `*_blocks[g]` = (job, workgroup within the job), with
+// the per-job counts from finish_batch_blocks() (returns the combine count).
+struct FinishJob { ReduceArgs r; CombineArgs c; };
+int finish_batch_blocks(const FinishJob& j, int* sum_blocks, int* red_blocks);
+hipError_t launch_finish_batch(const FinishJob* jobs, const int2* sum_blocks, int n_sum,
+                               const int2* red_blocks, int n_red, size_t red_lds,
+                               const int2* comb_blocks, int n_comb, hipStream_t s);
 
 // ------------------------------------------------------------- decompress
 struct DecompressArgs {
@@ -92,6 +134,14 @@ struct DecompressArgs {
   int64_t max_out; int32_t* oi; int32_t* oj; void* ov;
 };
 hipError_t launch_decompress(const DecompressArgs& a, hipStream_t s);
+// Many FP32 products of one class (variant, d <= 16, staged through LDS or
+// gathered from L2): blocks[g] = (job, workgroup within the job), with
+// decompress_batch_blocks() workgroups per job; `lds` = 2 * max b * 4 bytes
+// for the staged form (at most 64 KB).
+int decompress_batch_blocks(int d, int staged, int m, int p);
+hipError_t launch_decompress_batch(int fwht, int d, int staged, size_t lds,
+                                   const DecompressArgs* jobs, const int2* blocks, int nblocks,
+                                   hipStream_t s);
 
 // ------------------------------------------------------------- plain path
 // Stage-by-stage pipeline that materialises per-k sketches and spectra.

@@ -13,12 +13,20 @@ import numpy as np
 DATA_SEED = 20260001
 
 
-def planted(m, n, p, plants, magnitude, seed, dtype=np.float32):
+def planted(m, n, p, plants, magnitude, seed, dtype=np.float32, direct=False):
+    """direct=True draws the noise in `dtype` itself (faster for float32;
+    a different stream than the default float64 draw)."""
     rng = np.random.default_rng(seed)
-    scale = 1.0 / np.sqrt(n)
-    AT = (rng.standard_normal((n, m)) * scale).astype(dtype)
-    B = (rng.standard_normal((n, p)) * scale).astype(dtype)
-    plants = min(plants, m * p)
+    scale = 1.0 / np.sqrt(max(n, 1))
+    if direct:
+        AT = rng.standard_normal((n, m), dtype=dtype)
+        AT *= dtype(scale)
+        B = rng.standard_normal((n, p), dtype=dtype)
+        B *= dtype(scale)
+    else:
+        AT = (rng.standard_normal((n, m)) * scale).astype(dtype)
+        B = (rng.standard_normal((n, p)) * scale).astype(dtype)
+    plants = min(plants, m * p) if n > 0 else 0
     flat = rng.choice(m * p, size=plants, replace=False)
     ii, jj = flat // p, flat % p
     kk = rng.integers(0, n, size=plants)
@@ -39,6 +47,28 @@ CONFIGS = {
                                variant="fft", b=1 << 16, d=11, tau=12.5,
                                dtype=np.float32),
 }
+
+
+# many products of uneven shape: m, p log-uniform in [64, 4096], n in
+# [128, 8192], b = 2^10 .. 2^16, d in {5, 7, 9, 11, 15}, FFT / FWHT 50/50,
+# 16-256 plants of magnitude 5.0 each, recovery threshold 12.5
+BATCH_SEED = 20260004
+
+
+def batch_specs(count=512, seed=BATCH_SEED):
+    rng = np.random.default_rng(seed)
+
+    def logu(lo, hi, size):
+        return np.exp(rng.uniform(np.log(lo), np.log(hi), size)).round().astype(int)
+
+    m, p, n = logu(64, 4096, count), logu(64, 4096, count), logu(128, 8192, count)
+    b = 1 << rng.integers(10, 17, count)
+    d = rng.choice([5, 7, 9, 11, 15], count)
+    variant = np.where(rng.random(count) < 0.5, "fft", "fwht")
+    plants = rng.integers(16, 257, count)
+    return [dict(variant=str(variant[i]), m=int(m[i]), n=int(n[i]), p=int(p[i]), b=int(b[i]),
+                 d=int(d[i]), plants=int(plants[i]), magnitude=5.0, tau=12.5)
+            for i in range(count)]
 
 
 def make(name, seed_offset=0):

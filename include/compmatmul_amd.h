@@ -155,6 +155,59 @@ int cm_timing_enable(int on);
 int cm_timing_reset(void);
 int cm_timing_get(const char* name, double* total_ms, int64_t* launches);
 
+/* ------------------------------------------------------------------ batches
+ * Many independent FP32 products of uneven shape in one pass.  The reference
+ * has no counterpart: it computes one product per call
+ * (compmatmul.cpp:107-186, matmul_fft.hpp:155-178).  Each product's result
+ * is the one cm_compress + cm_decompress would give for the same operands
+ * (same hashes; the chunking of the inner dimension and so the FP32
+ * summation order may differ).  Operands and outputs are device pointers.
+ *
+ * A plan owns every device buffer of the batch.  Running it launches one
+ * schedule, hash and finishing launch each for all products, one persistent
+ * compress launch per transform variant (workgroups drain XCD-local work
+ * queues), and one decompression launch per (variant, d) class.  With
+ * CM_BATCH_GRAPH the launch sequence is captured once into a HIP graph and
+ * replayed. */
+typedef struct cm_product {
+  int variant;              /* CM_FFT or CM_FWHT */
+  int m, n, p;              /* C is m x p; inner dimension n */
+  const float* AT;          /* device, n x m, row stride ld_at >= m */
+  int64_t ld_at;
+  const float* B;           /* device, n x p, row stride ld_b >= p */
+  int64_t ld_b;
+  float* C;                 /* device, m x p, row stride ldc >= p; NULL: not written */
+  int64_t ldc;
+  int d, b;                 /* 1 <= d <= 16; b a power of two, 2 <= b <= 2^30 */
+  uint64_t seed;            /* h1, h2, s1, s2 from seed + 0..3 */
+  int family;               /* CM_MULTIPLY_ADD_SHIFT ... */
+  double tau;               /* recover entries > tau; NaN: no recovery */
+  int64_t max_out;          /* capacity of the product's recovered list */
+} cm_product;
+
+typedef struct cm_batch cm_batch;
+
+#define CM_BATCH_GRAPH 1u /* capture the launches into a HIP graph on the first run */
+
+/* Plan `count` products.  `grain` is the target work per compress work item
+ * in k-steps of a product with m + p = 0 (the per-product inner-dimension
+ * chunk is grain / (1 + (m + p) / 1700), at least 1); 0 = the autotuned value
+ * for this batch's shapes (tuned on first use by timing the candidate
+ * grains, then cached for the process). */
+int cm_batch_plan(int count, const cm_product* products, unsigned flags, int grain,
+                  cm_batch** out);
+/* Run the plan (asynchronous on the library stream; cm_synchronize waits). */
+int cm_batch_run(cm_batch* batch);
+/* The grain the plan uses. */
+int cm_batch_grain(const cm_batch* batch);
+/* Recovered entries of product i, sorted by (row, column); *count is the
+ * number found (may exceed max_out; only max_out are stored). */
+int cm_batch_recovered(cm_batch* batch, int i, int64_t max_out, int32_t* out_i, int32_t* out_j,
+                       float* out_val, int64_t* count);
+/* Sketch values (d x b, FP32 on the host) of product i. */
+int cm_batch_values(cm_batch* batch, int i, float* host_out);
+void cm_batch_free(cm_batch* batch);
+
 #ifdef __cplusplus
 }
 #endif

@@ -179,7 +179,8 @@ def test_errors_raise():
 
 @pytest.mark.parametrize("m,n,p,b", [(2048, 96, 2048, 1 << 14), (100, 70, 37, 1 << 12),
                                      (1, 5, 3, 1 << 13), (300, 33, 0, 1 << 12),
-                                     (60000, 3, 50, 1 << 12), (5000, 20, 4100, 1 << 15)])
+                                     (60000, 3, 50, 1 << 12), (5000, 20, 4100, 1 << 15),
+                                     (300, 40, 200, 1 << 10), (64, 9, 70, 1 << 10)])
 def test_fwht_register_kernel_matches_lds_kernel(m, n, p, b):
     print(f"reg vs lds m={m} n={n} p={p} b={b}", flush=True)
     AT, B, _ = workloads.planted(m, n, max(p, 1), 4, 5.0, m + n, np.float32)
@@ -203,7 +204,8 @@ def test_fwht_register_kernel_matches_lds_kernel(m, n, p, b):
 
 @pytest.mark.parametrize("m,n,p,b", [(2048, 40, 2048, 1 << 14), (100, 70, 37, 1 << 11),
                                      (1, 5, 3, 1 << 13), (300, 33, 0, 1 << 12),
-                                     (60000, 3, 50, 1 << 11), (5000, 20, 4100, 1 << 16)])
+                                     (60000, 3, 50, 1 << 11), (5000, 20, 4100, 1 << 16),
+                                     (300, 40, 200, 1 << 10), (64, 9, 70, 1 << 10)])
 def test_fft_register_kernel_matches_lds_kernel(m, n, p, b):
     print(f"fft reg vs lds m={m} n={n} p={p} b={b}", flush=True)
     AT, B, _ = workloads.planted(m, n, max(p, 1), 4, 5.0, m + n + 1, np.float32)
