@@ -174,6 +174,151 @@ def bench_batch(args):
     }
 
 
+def serve_datasets(count, seed):
+    """`count` request templates from the batch shape distribution; 30% hold
+    CSR operands at 1% density (N(0,1) values plus the planted entries),
+    the rest dense operands in pinned host memory."""
+    import scipy.sparse as sparse
+    specs = workloads.batch_specs(count, seed)
+    rng = np.random.default_rng(seed + 1)
+    out = []
+    for i, s in enumerate(specs):
+        m, n, p = s["m"], s["n"], s["p"]
+        if rng.random() < 0.3:
+            A = sparse.random(n, m, density=0.01, format="lil", random_state=seed + 2 * i,
+                              dtype=np.float32, data_rvs=lambda k: rng.standard_normal(k))
+            Bs = sparse.random(n, p, density=0.01, format="lil", random_state=seed + 2 * i + 1,
+                               dtype=np.float32, data_rvs=lambda k: rng.standard_normal(k))
+            for _ in range(s["plants"]):
+                k = rng.integers(n)
+                A[k, rng.integers(m)] = s["magnitude"]
+                Bs[k, rng.integers(p)] = s["magnitude"]
+            out.append(dict(s, csr=True, AT=A.tocsr(), B=Bs.tocsr()))
+        else:
+            AT, B, _ = workloads.planted(m, n, p, s["plants"], s["magnitude"], seed + 2 * i,
+                                         np.float32, direct=True)
+            out.append(dict(s, csr=False, AT=cma.HostBuffer.from_numpy(AT),
+                            B=cma.HostBuffer.from_numpy(B)))
+    return out
+
+
+def bench_serve(args):
+    """Poisson arrivals at a fixed QPS into the continuous-batching server;
+    the same arrivals replayed through one product per launch sequence."""
+    templates = serve_datasets(64, workloads.BATCH_SEED + 100)
+    rng = np.random.default_rng(7)
+    nreq = args.requests
+    arrivals = np.cumsum(rng.exponential(1.0 / args.qps, nreq))
+    picks = rng.integers(0, len(templates), nreq)
+    seeds = HASH_SEED + np.arange(nreq)
+
+    def operands(t):
+        if t["csr"]:
+            return t["AT"], t["B"]
+        return t["AT"], t["B"]
+
+    sv = cma.Server(hbm_bytes=args.hbm_gb << 30, max_waves=args.waves,
+                    max_wave_products=args.wave_products, grain=args.grain or 64)
+    # warm-up: one request of every template
+    for k, t in enumerate(templates):
+        a, b = operands(t)
+        tk = sv.submit(t["variant"], a, b, t["d"], t["b"], 1 + k, tau=t["tau"], max_out=1 << 16)
+        sv.wait(tk)
+        sv.release(tk)
+    import threading
+    tickets = [None] * nreq
+    t0 = time.perf_counter() + 0.05
+
+    def feeder():
+        for r in range(nreq):
+            dt = t0 + arrivals[r] - time.perf_counter()
+            if dt > 0:
+                time.sleep(dt)
+            t = templates[picks[r]]
+            a, b = operands(t)
+            tickets[r] = sv.submit(t["variant"], a, b, t["d"], t["b"], int(seeds[r]),
+                                   tau=t["tau"], max_out=1 << 16)
+
+    th = threading.Thread(target=feeder)
+    th.start()
+    lat = np.zeros(nreq)
+    results = {}
+    for r in range(nreq):
+        while tickets[r] is None:
+            time.sleep(1e-4)
+        sv.wait(tickets[r])
+        ii, jj, _, lat[r], _ = sv.result(tickets[r])
+        if picks[r] < 16 and picks[r] not in results:
+            results[picks[r]] = set(zip(ii.tolist(), jj.tolist()))
+        sv.release(tickets[r])
+    th.join()
+    t_end = time.perf_counter()
+    st = sv.stats()
+    sv.close()
+    served_s = t_end - t0
+
+    # one product per launch sequence: the same arrivals, served in order
+    base_lat = np.zeros(nreq)
+    dev = {}
+    for k, t in enumerate(templates):  # operands resident on the device, as for the server
+        if t["csr"]:
+            a, b = t["AT"].toarray(), t["B"].toarray()
+        else:
+            a, b = t["AT"].array, t["B"].array
+        dev[k] = (cma.DeviceArray.from_numpy(a), cma.DeviceArray.from_numpy(b))
+    t1 = time.perf_counter() + 0.05
+    sk = None
+    for r in range(nreq):
+        dt = t1 + arrivals[r] - time.perf_counter()
+        if dt > 0:
+            time.sleep(dt)
+        t = templates[picks[r]]
+        a, b = dev[picks[r]]
+        sk = cma.compress(t["variant"], a, b, t["d"], t["b"], seed=int(seeds[r]), dtype=np.float32)
+        sk.recover(t["tau"])
+        sk.free()
+        base_lat[r] = time.perf_counter() - (t1 + arrivals[r])
+    base_s = time.perf_counter() - t1
+
+    # recall of the served results on the first templates
+    hits = truth_n = got_n = 0
+    for k, got in results.items():
+        t = templates[k]
+        a = t["AT"].toarray() if t["csr"] else t["AT"].array
+        b = t["B"].toarray() if t["csr"] else t["B"].array
+        truth = set(zip(*np.nonzero(a.astype(np.float64).T @ b.astype(np.float64) > t["tau"])))
+        hits += len(got & truth)
+        truth_n += len(truth)
+        got_n += len(got)
+    q = lambda x, f: float(np.quantile(x, f))
+    return {
+        "metric": "products/s",
+        "value": nreq / served_s,
+        "unit": "products/s",
+        "n_gpus": 1,
+        "steps": nreq,
+        "warmup": len(templates),
+        "ms_per_step": 1e3 * served_s / nreq,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: batch shape distribution, 30% CSR at 1% density, planted entries (seeded)",
+        "config": {"workload": f"serving: Poisson arrivals at {args.qps} QPS, {nreq} requests over "
+                               f"{len(templates)} request templates",
+                   "qps": args.qps, "max_waves": args.waves,
+                   "max_wave_products": args.wave_products, "grain": args.grain or 64,
+                   "parallelism": "single GPU"},
+        "detail": {"latency_ms": {"p50": 1e3 * q(lat, 0.5), "p99": 1e3 * q(lat, 0.99),
+                                  "max": 1e3 * float(lat.max())},
+                   "one_product_per_launch": {"products_per_s": nreq / base_s,
+                                              "latency_ms": {"p50": 1e3 * q(base_lat, 0.5),
+                                                             "p99": 1e3 * q(base_lat, 0.99)}},
+                   "server": st, "recall": hits / max(1, truth_n),
+                   "precision": hits / max(1, got_n)},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -182,10 +327,15 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--plain", action="store_true", help="time the unfused path instead")
     ap.add_argument("--workload", default=DEFAULT_WORKLOAD,
-                    choices=sorted(workloads.CONFIGS) + ["batch512"],
+                    choices=sorted(workloads.CONFIGS) + ["batch512", "serve"],
                     help="named workload (default: the metric's configuration)")
     ap.add_argument("--grain", type=int, default=0,
-                    help="batch512: work-item grain (0 = autotuned)")
+                    help="batch512: work-item grain (0 = autotuned); serve: 0 = 64")
+    ap.add_argument("--qps", type=float, default=300.0, help="serve: arrival rate")
+    ap.add_argument("--requests", type=int, default=600, help="serve: requests")
+    ap.add_argument("--waves", type=int, default=3, help="serve: waves on the device at once")
+    ap.add_argument("--wave-products", type=int, default=64, help="serve: requests per wave")
+    ap.add_argument("--hbm-gb", type=int, default=64, help="serve: server HBM budget (GiB)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copies of MIB MiB (the known-size "
                          "kernel that scales FETCH_SIZE / WRITE_SIZE counter readings)")
@@ -202,6 +352,9 @@ def main():
     name = args.workload
     if name == "batch512":
         print(json.dumps(bench_batch(args)), flush=True)
+        return
+    if name == "serve":
+        print(json.dumps(bench_serve(args)), flush=True)
         return
     c = workloads.CONFIGS[name]
     m, n, p, d, b = c["m"], c["n"], c["p"], c["d"], c["b"]

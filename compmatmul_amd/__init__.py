@@ -5,8 +5,9 @@ Python host layer over the C-ABI library ``libcompmatmul_amd.so``
 reference's interface is ``compmatmul_amd.compmatmul`` (importable as plain
 ``compmatmul`` with this directory on ``sys.path``); this module adds the
 extended interface: float32 inputs, device-resident inputs and sketches,
-sketch values, hash arrays, large-entry recovery, kernel timing, and
-batches of many products (Batch).
+sketch values, hash arrays, large-entry recovery, kernel timing, batches
+of many products (Batch) and a request server with continuous batching
+(Server).
 
 There is no CPU fallback: if the native library is missing or no gfx950
 device is present, the calls raise.
@@ -83,6 +84,18 @@ def lib():
     L.cm_batch_values.argtypes = [_vp, ctypes.c_int, _vp]
     L.cm_batch_free.argtypes = [_vp]
     L.cm_batch_free.restype = None
+    L.cm_server_create.argtypes = [ctypes.c_size_t, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                   ctypes.POINTER(_vp)]
+    L.cm_server_submit.argtypes = [_vp, _vp, ctypes.POINTER(ctypes.c_uint64)]
+    L.cm_server_wait.argtypes = [_vp, ctypes.c_uint64, ctypes.c_double]
+    L.cm_server_result.argtypes = [_vp, ctypes.c_uint64, ctypes.c_int64, _i32p, _i32p, _vp,
+                                   _i64p, ctypes.POINTER(ctypes.c_double)]
+    L.cm_server_decompress.argtypes = [_vp, ctypes.c_uint64, _vp]
+    L.cm_server_release.argtypes = [_vp, ctypes.c_uint64]
+    L.cm_server_stats_get.argtypes = [_vp, _vp]
+    L.cm_server_destroy.argtypes = [_vp]
+    L.cm_host_alloc.argtypes = [ctypes.c_size_t, ctypes.POINTER(_vp)]
+    L.cm_host_free.argtypes = [_vp]
     _LIB = L
     return L
 
@@ -375,6 +388,180 @@ class Batch:
     def __del__(self):
         try:
             self.free()
+        except Exception:
+            pass
+
+
+class HostBuffer:
+    """Pinned host memory viewed as a numpy array (`.array`), for operands
+    that the server uploads asynchronously."""
+
+    def __init__(self, shape, dtype):
+        dtype = np.dtype(dtype)
+        shape = tuple(int(x) for x in np.atleast_1d(shape))
+        nbytes = max(1, int(np.prod(shape)) * dtype.itemsize)
+        self.ptr = _vp()
+        check(lib().cm_host_alloc(nbytes, ctypes.byref(self.ptr)))
+        raw = (ctypes.c_char * nbytes).from_address(self.ptr.value)
+        self.array = np.frombuffer(raw, dtype=np.uint8)[:int(np.prod(shape)) * dtype.itemsize] \
+            .view(dtype).reshape(shape)
+
+    @classmethod
+    def from_numpy(cls, a):
+        out = cls(a.shape, a.dtype)
+        out.array[...] = a
+        return out
+
+    def __del__(self):
+        try:
+            if self.ptr:
+                lib().cm_host_free(self.ptr)
+                self.ptr = _vp()
+        except Exception:
+            pass
+
+
+class _Request(ctypes.Structure):
+    _fields_ = [("variant", ctypes.c_int), ("m", ctypes.c_int), ("n", ctypes.c_int),
+                ("p", ctypes.c_int), ("d", ctypes.c_int), ("b", ctypes.c_int),
+                ("seed", ctypes.c_uint64), ("family", ctypes.c_int), ("tau", ctypes.c_double),
+                ("max_out", ctypes.c_int64), ("format", ctypes.c_int), ("AT", _vp), ("B", _vp),
+                ("at_indptr", _vp), ("at_indices", _vp), ("at_values", _vp), ("b_indptr", _vp),
+                ("b_indices", _vp), ("b_values", _vp), ("C", _vp)]
+
+
+class _ServerStats(ctypes.Structure):
+    _fields_ = [("waves", ctypes.c_int64), ("requests_done", ctypes.c_int64),
+                ("requests_failed", ctypes.c_int64), ("max_waves_in_flight", ctypes.c_int),
+                ("hbm_bytes", ctypes.c_size_t), ("peak_bytes", ctypes.c_size_t)]
+
+
+DENSE, CSR = 0, 1
+
+
+def _host_f32(x):
+    if isinstance(x, HostBuffer):
+        return x.array
+    return np.ascontiguousarray(x, np.float32)
+
+
+def _csr_parts(x):
+    """(indptr, indices, values) of a CSR operand: a scipy.sparse matrix or a
+    tuple of arrays."""
+    if hasattr(x, "indptr"):
+        ip, ix, v = x.indptr, x.indices, x.data
+    else:
+        ip, ix, v = x
+    return (np.ascontiguousarray(ip, np.int32), np.ascontiguousarray(ix, np.int32),
+            _host_f32(v))
+
+
+class Server:
+    """Request server with continuous batching on one GPU (cm_server_*).
+
+    Requests queue in arrival order; a dispatcher thread admits them while
+    their device footprint fits `hbm_bytes` and runs them as waves (batches)
+    on up to `max_waves` streams at once.  Results: the entries above tau
+    (and C when asked for), plus the request's latency.  The sketch stays on
+    the device until release(ticket)."""
+
+    def __init__(self, hbm_bytes=8 << 30, max_waves=3, max_wave_products=64, grain=64):
+        self.handle = _vp()
+        self._keep = {}
+        check(lib().cm_server_create(int(hbm_bytes), int(max_waves), int(max_wave_products),
+                                     int(grain), ctypes.byref(self.handle)))
+
+    def submit(self, variant, AT, B, d, b, seed, hash=None, tau=None, max_out=4096,
+               want_C=False, shape=None):
+        """AT (n x m) and B (n x p): float32 arrays / HostBuffers, or CSR
+        operands (scipy.sparse CSR or (indptr, indices, values); then pass
+        shape=(m, n, p)).  Returns the ticket."""
+        if hash not in FAMILIES:
+            raise ValueError(f"Invalid hash `{hash}'")
+        r = _Request()
+        keep = []
+        sparse = hasattr(AT, "indptr") or isinstance(AT, tuple)
+        if sparse:
+            if shape is None:
+                if not hasattr(AT, "shape"):
+                    raise ValueError("pass shape=(m, n, p) with tuple CSR operands")
+                shape = (AT.shape[1], AT.shape[0], B.shape[1])
+            m, n, p = shape
+            a, bb = _csr_parts(AT), _csr_parts(B)
+            keep += [a, bb]
+            r.format = CSR
+            r.at_indptr, r.at_indices, r.at_values = (x.ctypes.data for x in a)
+            r.b_indptr, r.b_indices, r.b_values = (x.ctypes.data for x in bb)
+        else:
+            a, bb = _host_f32(AT), _host_f32(B)
+            n, m = a.shape
+            if bb.shape[0] != n:
+                raise ValueError("Shape mismatch! Note that AT is assumed to be transposed")
+            p = bb.shape[1]
+            keep += [a, bb, AT, B]
+            r.format = DENSE
+            r.AT, r.B = a.ctypes.data, bb.ctypes.data
+        r.variant, r.m, r.n, r.p, r.d, r.b = VARIANTS[variant], m, n, p, int(d), int(b)
+        r.seed, r.family = int(seed), FAMILIES[hash]
+        r.tau = float("nan") if tau is None else float(tau)
+        r.max_out = int(max_out)
+        C = None
+        if want_C:
+            C = np.empty((m, p), np.float32)
+            keep.append(C)
+            r.C = C.ctypes.data
+        t = ctypes.c_uint64()
+        check(lib().cm_server_submit(self.handle, ctypes.byref(r), ctypes.byref(t)))
+        self._keep[t.value] = (keep, C, int(max_out), (m, p))
+        return t.value
+
+    def wait(self, ticket, timeout=None):
+        """True when done (raises on a failed request), False on timeout."""
+        rc = lib().cm_server_wait(self.handle, ticket, -1.0 if timeout is None else float(timeout))
+        if rc == 1:
+            return False
+        check(rc)
+        return True
+
+    def result(self, ticket):
+        """(rows, cols, values, latency_s, C or None) of a finished request."""
+        keep, C, cap, _ = self._keep[ticket]
+        oi = np.empty(max(1, cap), np.int32)
+        oj = np.empty(max(1, cap), np.int32)
+        ov = np.empty(max(1, cap), np.float32)
+        cnt = ctypes.c_int64()
+        lat = ctypes.c_double()
+        check(lib().cm_server_result(self.handle, ticket, cap, oi.ctypes.data_as(_i32p),
+                                     oj.ctypes.data_as(_i32p), ov.ctypes.data,
+                                     ctypes.byref(cnt), ctypes.byref(lat)))
+        k = min(cnt.value, cap)
+        return oi[:k].copy(), oj[:k].copy(), ov[:k].copy(), lat.value, C
+
+    def decompress(self, ticket):
+        """All of C from the request's device-resident sketch."""
+        m, p = self._keep[ticket][3]
+        C = np.empty((m, p), np.float32)
+        check(lib().cm_server_decompress(self.handle, ticket, C.ctypes.data))
+        return C
+
+    def release(self, ticket):
+        check(lib().cm_server_release(self.handle, ticket))
+        self._keep.pop(ticket, None)
+
+    def stats(self):
+        st = _ServerStats()
+        check(lib().cm_server_stats_get(self.handle, ctypes.byref(st)))
+        return {k: getattr(st, k) for k, _ in _ServerStats._fields_}
+
+    def close(self):
+        if self.handle:
+            check(lib().cm_server_destroy(self.handle))
+            self.handle = _vp()
+            self._keep.clear()
+
+    def __del__(self):
+        try:
+            self.close()
         except Exception:
             pass
 

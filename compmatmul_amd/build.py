@@ -30,7 +30,7 @@ HIP_FLAGS = [
 ]
 
 SOURCES = ["hash.hip", "compress.hip", "compress_reg.hip", "compress_fft_reg.hip", "decompress.hip",
-           "plain.hip", "api.cpp", "batch.cpp", "host_hash.cpp"]
+           "plain.hip", "sparse.hip", "api.cpp", "batch.cpp", "serve.cpp", "host_hash.cpp"]
 
 
 def _run(cmd):

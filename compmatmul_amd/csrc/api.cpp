@@ -830,6 +830,21 @@ int cm_dev_free(void* ptr) {
   return CM_OK;
 }
 
+int cm_host_alloc(size_t bytes, void** ptr) {
+  if (!ptr) return fail(CM_EINVAL, "null pointer");
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  int rc = init_locked(-1);
+  if (rc) return rc;
+  CM_HIP(hipHostMalloc(ptr, std::max<size_t>(bytes, 1), hipHostMallocDefault));
+  return CM_OK;
+}
+
+int cm_host_free(void* ptr) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  if (ptr) CM_HIP(hipHostFree(ptr));
+  return CM_OK;
+}
+
 int cm_memcpy(void* dst, const void* src, size_t bytes, int dst_mem, int src_mem) {
   std::lock_guard<std::recursive_mutex> lk(g_mu);
   int rc = init_locked(-1);

@@ -33,6 +33,7 @@
 #include <vector>
 
 #include "../../include/compmatmul_amd.h"
+#include "batch_internal.hpp"
 #include "host_hash.hpp"
 #include "kernels.hpp"
 #include "runtime.hpp"
@@ -42,6 +43,8 @@ struct cm_batch {
   unsigned flags = 0;
   int grain = 0;
   void* arena = nullptr;
+  bool owns_arena = true;
+  hipStream_t stream = nullptr;
   struct Prod {
     uint32_t* hA;
     uint32_t* hB;
@@ -159,8 +162,11 @@ struct HostPlan {
   std::vector<int> tab_len;
 };
 
-int build_plan(int count, const cm_product* ps, unsigned flags, int grain, cm_batch** out) {
-  if (count < 0 || (count > 0 && !ps) || !out) return error(CM_EINVAL, "null or negative argument");
+int build_plan(int count, const cm_product* ps, unsigned flags, int grain, const cm::PlanEnv& env,
+               cm_batch** out) {
+  if (count < 0 || (count > 0 && !ps) || (!out && !env.size_only))
+    return error(CM_EINVAL, "null or negative argument");
+  const bool size_only = env.size_only != nullptr;
   for (int i = 0; i < count; ++i) {
     int rc = validate(i, ps[i]);
     if (rc) return rc;
@@ -171,6 +177,7 @@ int build_plan(int count, const cm_product* ps, unsigned flags, int grain, cm_ba
   bt->prods.assign(ps, ps + count);
   bt->flags = flags;
   bt->grain = grain;
+  bt->stream = env.stream ? env.stream : cm::rt::stream();
   HostPlan hp;
   hp.lbv.resize(count);
   hp.kper.resize(count);
@@ -193,7 +200,17 @@ int build_plan(int count, const cm_product* ps, unsigned flags, int grain, cm_ba
     hp.kper[i] = kper;
     hp.nchunks[i] = q.n > 0 ? (q.n + kper - 1) / kper : 0;
     hp.jobidx[i] = njobs[fw]++;
-    if (q.family == CM_MULTIPLY_ADD_SHIFT || q.family == CM_MULTIPLY_SHIFT) {
+    if (size_only) {  // sizes of the parameter blocks only
+      if (q.family == CM_MULTIPLY_ADD_SHIFT || q.family == CM_MULTIPLY_SHIFT) {
+        hp.mul[i].resize(size_t(4) * q.d * (q.family == CM_MULTIPLY_ADD_SHIFT ? 2 : 1));
+      } else {
+        const int chars = q.family == CM_TABULATION8 ? 4 : 2;
+        const int64_t maxkey = std::max(1, std::max(q.m, q.p)) - 1;
+        const int64_t tl = q.family == CM_TABULATION8 ? 256 : std::min<int64_t>(65536, maxkey + 1);
+        hp.tab_len[i] = int(tl);
+        hp.tabs[i].resize(size_t(4) * q.d * chars * tl);
+      }
+    } else if (q.family == CM_MULTIPLY_ADD_SHIFT || q.family == CM_MULTIPLY_SHIFT) {
       std::vector<uint64_t> one;
       for (int f = 0; f < 4; ++f) {
         cm::draw_multiply(q.family, q.d, q.seed + uint64_t(f), one);
@@ -229,16 +246,18 @@ int build_plan(int count, const cm_product* ps, unsigned flags, int grain, cm_ba
   std::vector<cm::DecompressArgs> dec(count);
 
   const void* t2 = nullptr;
-  if (njobs[0] && (rc = cm::rt::fft_t2_table(&t2))) {
-    delete bt;
-    return rc;
-  }
   std::vector<const void*> tw(count, nullptr);
-  for (int i = 0; i < count; ++i)
-    if (ps[i].variant == CM_FFT && (rc = cm::rt::twiddles(hp.lbv[i], 0, &tw[i]))) {
+  if (!size_only) {
+    if (njobs[0] && (rc = cm::rt::fft_t2_table(&t2))) {
       delete bt;
       return rc;
     }
+    for (int i = 0; i < count; ++i)
+      if (ps[i].variant == CM_FFT && (rc = cm::rt::twiddles(hp.lbv[i], 0, &tw[i]))) {
+        delete bt;
+        return rc;
+      }
+  }
 
   // two passes over the arena: sizes (base == nullptr), then addresses
   Carver cv;
@@ -254,9 +273,15 @@ int build_plan(int count, const cm_product* ps, unsigned flags, int grain, cm_ba
       const int fw = q.variant == CM_FWHT;
       const int lb = ilog2(q.b);
       auto& P = bt->pp[i];
-      P.hA = cv.take<uint32_t>(size_t(q.d) * q.m);
-      P.hB = cv.take<uint32_t>(size_t(q.d) * q.p);
-      P.P = cv.take<float>(size_t(q.d) * q.b);
+      if (env.keep) {
+        P.hA = env.keep[i].hA;
+        P.hB = env.keep[i].hB;
+        P.P = env.keep[i].P;
+      } else {
+        P.hA = cv.take<uint32_t>(size_t(q.d) * q.m);
+        P.hB = cv.take<uint32_t>(size_t(q.d) * q.p);
+        P.P = cv.take<float>(size_t(q.d) * q.b);
+      }
       const int64_t mo = std::isnan(q.tau) ? 0 : q.max_out;
       P.oi = cv.take<int32_t>(size_t(mo));
       P.oj = cv.take<int32_t>(size_t(mo));
@@ -446,12 +471,27 @@ int build_plan(int count, const cm_product* ps, unsigned flags, int grain, cm_ba
     const size_t tables_end = cv.off;
 
     if (pass == 0) {
-      const hipError_t e = hipMalloc(&bt->arena, std::max<size_t>(cv.off, 256));
-      if (e != hipSuccess) {
+      if (size_only) {
+        *env.size_only = std::max<size_t>(cv.off, 256);
         delete bt;
-        return error(e == hipErrorOutOfMemory ? CM_ENOMEM : CM_EDEVICE,
-                     std::string("batch arena of ") + std::to_string(cv.off) + " bytes: " +
-                         hipGetErrorString(e));
+        return CM_OK;
+      }
+      if (env.arena) {
+        if (env.arena_bytes < cv.off) {
+          delete bt;
+          return error(CM_ENOMEM, "batch arena too small: " + std::to_string(cv.off) + " > " +
+                                      std::to_string(env.arena_bytes) + " bytes");
+        }
+        bt->arena = env.arena;
+        bt->owns_arena = false;
+      } else {
+        const hipError_t e = hipMalloc(&bt->arena, std::max<size_t>(cv.off, 256));
+        if (e != hipSuccess) {
+          delete bt;
+          return error(e == hipErrorOutOfMemory ? CM_ENOMEM : CM_EDEVICE,
+                       std::string("batch arena of ") + std::to_string(cv.off) + " bytes: " +
+                           hipGetErrorString(e));
+        }
       }
       cv.base = static_cast<char*>(bt->arena);
       continue;
@@ -479,17 +519,21 @@ int build_plan(int count, const cm_product* ps, unsigned flags, int grain, cm_ba
     put(bt->comb_blocks, comb_blocks.data(), comb_blocks.size() * sizeof(int2));
     for (size_t c = 0; c < bt->classes.size(); ++c)
       put(bt->classes[c].blocks, cls_host[c]->data(), cls_host[c]->size() * sizeof(int2));
-    hipError_t e = hipMemcpy(cv.base + tables_begin, img.data(), img.size(), hipMemcpyHostToDevice);
+    // stream-ordered uploads, complete when the calls return (pageable
+    // host memory), so the host images may go out of scope
+    hipError_t e = hipMemcpyAsync(cv.base + tables_begin, img.data(), img.size(),
+                                  hipMemcpyHostToDevice, bt->stream);
     for (int i = 0; i < count && e == hipSuccess; ++i) {
       if (hp.mul[i].size() > 256)
-        e = hipMemcpy(const_cast<uint64_t*>(mulp[i]), hp.mul[i].data(), hp.mul[i].size() * 8,
-                      hipMemcpyHostToDevice);
+        e = hipMemcpyAsync(const_cast<uint64_t*>(mulp[i]), hp.mul[i].data(), hp.mul[i].size() * 8,
+                           hipMemcpyHostToDevice, bt->stream);
       if (e == hipSuccess && !hp.tabs[i].empty())
-        e = hipMemcpy(const_cast<uint32_t*>(tabp[i]), hp.tabs[i].data(), hp.tabs[i].size() * 4,
-                      hipMemcpyHostToDevice);
+        e = hipMemcpyAsync(const_cast<uint32_t*>(tabp[i]), hp.tabs[i].data(), hp.tabs[i].size() * 4,
+                           hipMemcpyHostToDevice, bt->stream);
     }
+    if (e == hipSuccess) e = hipStreamSynchronize(bt->stream);
     if (e != hipSuccess) {
-      hipFree(bt->arena);
+      if (bt->owns_arena) hipFree(bt->arena);
       delete bt;
       return error(CM_EDEVICE, std::string("batch table upload: ") + hipGetErrorString(e));
     }
@@ -501,7 +545,7 @@ int build_plan(int count, const cm_product* ps, unsigned flags, int grain, cm_ba
 // the launch sequence of one run; `untimed` skips the per-kernel timing
 // events (graph capture, autotuning)
 int enqueue(cm_batch* bt, bool untimed) {
-  hipStream_t s = cm::rt::stream();
+  hipStream_t s = bt->stream;
   auto run = [&](const char* name, const std::function<hipError_t()>& f) -> int {
     if (untimed) {
       const hipError_t e = f();
@@ -545,7 +589,7 @@ int enqueue(cm_batch* bt, bool untimed) {
 
 int run_locked(cm_batch* bt) {
   if (!(bt->flags & CM_BATCH_GRAPH)) return enqueue(bt, false);
-  hipStream_t s = cm::rt::stream();
+  hipStream_t s = bt->stream;
   if (!bt->exec) {
     hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
     if (e != hipSuccess) return error(CM_EDEVICE, std::string("begin capture: ") + hipGetErrorString(e));
@@ -567,10 +611,10 @@ int run_locked(cm_batch* bt) {
 
 void free_locked(cm_batch* bt) {
   if (!bt) return;
-  hipStreamSynchronize(cm::rt::stream());
+  hipStreamSynchronize(bt->stream);
   if (bt->exec) hipGraphExecDestroy(bt->exec);
   if (bt->graph) hipGraphDestroy(bt->graph);
-  if (bt->arena) hipFree(bt->arena);
+  if (bt->arena && bt->owns_arena) hipFree(bt->arena);
   delete bt;
 }
 
@@ -586,7 +630,7 @@ int tune(int count, const cm_product* ps, unsigned flags, int* best) {
   int rc = CM_OK;
   for (int grain : kGrains) {
     cm_batch* bt = nullptr;
-    if ((rc = build_plan(count, ps, flags & ~CM_BATCH_GRAPH, grain, &bt))) break;
+    if ((rc = build_plan(count, ps, flags & ~CM_BATCH_GRAPH, grain, cm::PlanEnv{}, &bt))) break;
     double ms_min = 1e300;
     for (int r = 0; r < 3 && rc == CM_OK; ++r) {
       hipEventRecord(e0, s);
@@ -611,6 +655,29 @@ int tune(int count, const cm_product* ps, unsigned flags, int* best) {
 
 }  // namespace
 
+namespace cm {
+
+int batch_plan_env(int count, const cm_product* ps, unsigned flags, int grain, const PlanEnv& env,
+                   cm_batch** out) {
+  if (grain <= 0) return error(CM_EINVAL, "grain must be positive");
+  return build_plan(count, ps, flags & ~CM_BATCH_GRAPH, grain, env, out);
+}
+
+int batch_enqueue(cm_batch* bt) { return enqueue(bt, true); }
+
+ProductView batch_product(const cm_batch* bt, int i) {
+  const auto& P = bt->pp[i];
+  return ProductView{P.counter, P.oi, P.oj, P.ov};
+}
+
+void batch_destroy(cm_batch* bt) {
+  if (!bt) return;
+  if (bt->arena && bt->owns_arena) hipFree(bt->arena);
+  delete bt;
+}
+
+}  // namespace cm
+
 extern "C" {
 
 int cm_batch_plan(int count, const cm_product* products, unsigned flags, int grain, cm_batch** out) {
@@ -632,7 +699,7 @@ int cm_batch_plan(int count, const cm_product* products, unsigned flags, int gra
       g_tuned[sig] = grain;
     }
   }
-  return build_plan(count, products, flags, grain, out);
+  return build_plan(count, products, flags, grain, cm::PlanEnv{}, out);
 }
 
 int cm_batch_run(cm_batch* batch) {
@@ -652,7 +719,7 @@ int cm_batch_recovered(cm_batch* batch, int i, int64_t max_out, int32_t* out_i, 
   const cm_product& q = batch->prods[i];
   if (std::isnan(q.tau)) return error(CM_EINVAL, "the product was planned without recovery");
   const auto& P = batch->pp[i];
-  hipStream_t s = cm::rt::stream();
+  hipStream_t s = batch->stream;
   unsigned long long found = 0;
   if (hipMemcpyAsync(&found, P.counter, 8, hipMemcpyDeviceToHost, s) != hipSuccess ||
       hipStreamSynchronize(s) != hipSuccess)
@@ -684,7 +751,7 @@ int cm_batch_values(cm_batch* batch, int i, float* host_out) {
   if (i < 0 || i >= int(batch->prods.size())) return error(CM_EINVAL, "product index out of range");
   std::lock_guard<std::recursive_mutex> lk(cm::rt::mutex());
   const cm_product& q = batch->prods[i];
-  hipStream_t s = cm::rt::stream();
+  hipStream_t s = batch->stream;
   if (hipStreamSynchronize(s) != hipSuccess ||
       hipMemcpy(host_out, batch->pp[i].P, size_t(q.d) * q.b * 4, hipMemcpyDeviceToHost) != hipSuccess)
     return error(CM_EDEVICE, "reading the sketch failed");

@@ -160,6 +160,11 @@ hipError_t launch_plain_sketch(const PlainArgs& a, hipStream_t s);
 hipError_t launch_plain_transform(const PlainArgs& a, hipStream_t s);
 hipError_t launch_plain_mac(const PlainArgs& a, hipStream_t s);
 
+// ------------------------------------------------------------- sparse operands
+// CSR (rows x cols, indptr[rows + 1], indices, values) to dense row-major.
+hipError_t launch_csr_dense(const int32_t* indptr, const int32_t* indices, const float* values,
+                            int rows, int cols, int64_t ld, float* out, hipStream_t s);
+
 // ------------------------------------------------------------- gemm
 hipError_t launch_gemm_f64(int m, int n, int p, const double* AT, const double* B,
                            double* C, hipStream_t s);

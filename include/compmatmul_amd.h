@@ -208,6 +208,75 @@ int cm_batch_recovered(cm_batch* batch, int i, int64_t max_out, int32_t* out_i, 
 int cm_batch_values(cm_batch* batch, int i, float* host_out);
 void cm_batch_free(cm_batch* batch);
 
+/* ------------------------------------------------------------------ serving
+ * A request server on one GPU (no counterpart in the reference, which has
+ * one blocking product per call).  A dispatcher thread admits queued
+ * requests in arrival order while their device footprint fits the server's
+ * HBM budget, and runs each admitted group as a batch ("wave") on a stream
+ * of its own.  Up to max_waves waves are on the device at once, so requests
+ * that arrive while others run are admitted as wave slots and memory free
+ * up, and their work overlaps.  A request's hash arrays and sketch stay in
+ * HBM until cm_server_release (cm_server_decompress reads them), and count
+ * against the budget until then.  With a fixed grain a request's result
+ * does not depend on which other requests share its wave. */
+#define CM_DENSE 0
+#define CM_CSR 1
+
+typedef struct cm_request {
+  int variant;                  /* CM_FFT or CM_FWHT */
+  int m, n, p, d, b;
+  uint64_t seed;
+  int family;
+  double tau;                   /* recover entries > tau (NaN: none) */
+  int64_t max_out;              /* recovered entries kept */
+  int format;                   /* CM_DENSE or CM_CSR, for both operands */
+  /* CM_DENSE: host FP32 row-major AT (n x m) and B (n x p) */
+  const float* AT;
+  const float* B;
+  /* CM_CSR: AT and B as n-row CSR matrices (indptr[n + 1], indices, values) */
+  const int32_t* at_indptr;
+  const int32_t* at_indices;
+  const float* at_values;
+  const int32_t* b_indptr;
+  const int32_t* b_indices;
+  const float* b_values;
+  float* C;                     /* host m x p output, or NULL */
+} cm_request;
+
+typedef struct cm_server cm_server;
+
+/* hbm_bytes: device memory the server may hold; max_waves: waves on the
+ * device at once; max_wave_products: requests per wave; grain as for
+ * cm_batch_plan (must be > 0: results then depend on the request only). */
+int cm_server_create(size_t hbm_bytes, int max_waves, int max_wave_products, int grain,
+                     cm_server** out);
+/* Queue a request (the descriptor is copied; the host buffers it points to
+ * must stay valid until the request completes).  Returns its ticket. */
+int cm_server_submit(cm_server* server, const cm_request* request, uint64_t* ticket);
+/* Wait up to timeout_s seconds (< 0: forever): CM_OK when the request is
+ * done, 1 on timeout, or the request's error status. */
+int cm_server_wait(cm_server* server, uint64_t ticket, double timeout_s);
+/* Recovered entries (sorted by row, column), the number found, and the
+ * request's submit-to-done latency in seconds. */
+int cm_server_result(cm_server* server, uint64_t ticket, int64_t max_out, int32_t* out_i,
+                     int32_t* out_j, float* out_val, int64_t* count, double* latency_s);
+/* Median-of-d estimate of all of C from the request's retained sketch. */
+int cm_server_decompress(cm_server* server, uint64_t ticket, float* host_C);
+/* Drop the request's retained sketch and its record. */
+int cm_server_release(cm_server* server, uint64_t ticket);
+typedef struct cm_server_stats {
+  int64_t waves, requests_done, requests_failed;
+  int max_waves_in_flight;
+  size_t hbm_bytes, peak_bytes;
+} cm_server_stats;
+int cm_server_stats_get(cm_server* server, cm_server_stats* out);
+/* Finish the waves in flight, fail the queued requests, free everything. */
+int cm_server_destroy(cm_server* server);
+
+/* Pinned host memory (fast, truly asynchronous uploads for server clients). */
+int cm_host_alloc(size_t bytes, void** ptr);
+int cm_host_free(void* ptr);
+
 #ifdef __cplusplus
 }
 #endif
