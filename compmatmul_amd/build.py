@@ -4,6 +4,8 @@
   (hipcc, one object per source, compiled in parallel).
 * ``compmatmul<ext>``: the pybind11 drop-in module ``compmatmul``, linked
   against the C-ABI library with an ``$ORIGIN`` rpath.
+* ``multiply``: the command-line tool (cli/), and ``tests/cpp/tests_runner``,
+  the C++ known-answer test executable, both linked against the library.
 
 Usage: ``python -m compmatmul_amd.build`` (or ``build()`` from Python).
 """
@@ -21,6 +23,8 @@ ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")
 LIB = os.path.join(HERE, "libcompmatmul_amd.so")
 EXT = os.path.join(HERE, "compmatmul" + sysconfig.get_config_var("EXT_SUFFIX"))
+MULTIPLY = os.path.join(HERE, "multiply")
+RUNNER = os.path.join(ROOT, "tests", "cpp", "tests_runner")
 
 HIP_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
@@ -63,8 +67,17 @@ def build(verbose=False):
           "-L", HERE, "-lcompmatmul_amd", "-Wl,-rpath,$ORIGIN",
           "-o", EXT + ".tmp"])
     os.replace(EXT + ".tmp", EXT)
+    cli = os.path.join(HERE, "cli")
+    _run(["g++", "-O2", "-std=c++17", "-I", ROOT, os.path.join(cli, "multiply.cpp"),
+          os.path.join(cli, "npy.cpp"), "-L", HERE, "-lcompmatmul_amd", "-Wl,-rpath,$ORIGIN",
+          "-o", MULTIPLY + ".tmp"])
+    os.replace(MULTIPLY + ".tmp", MULTIPLY)
+    _run(["g++", "-O2", "-std=c++17", "-I", ROOT, os.path.join(ROOT, "tests", "cpp", "runner.cpp"),
+          os.path.join(cli, "npy.cpp"), "-L", HERE, "-lcompmatmul_amd",
+          "-Wl,-rpath,$ORIGIN/../../compmatmul_amd", "-o", RUNNER + ".tmp"])
+    os.replace(RUNNER + ".tmp", RUNNER)
     if verbose:
-        print("built", LIB, "and", EXT)
+        print("built", LIB, EXT, MULTIPLY, "and", RUNNER)
     return LIB
 
 
