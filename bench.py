@@ -33,6 +33,9 @@ HASH_SEED = 13794
 PEAK_FP32_TFLOPS = 157.3
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
+# largest sketch length of the unfused (plain) FP32 path: one LDS-resident
+# transform of at most 64 KB (FFT: complex)
+PLAIN_MAX_B = {"fft": 1 << 13, "fwht": 1 << 14}
 
 
 def compress_flops(variant, m, n, p, d, b):
@@ -174,6 +177,81 @@ def bench_batch(args):
     }
 
 
+def bench_precision(args):
+    """The precision form (FFT, 4096 x 16384 x 4096, b = 2^16, d = 11) in three
+    arithmetic modes: FP32 butterflies, FP64 cross-chunk spectral
+    accumulation, BF16 matrix-core DFT stages with hi/lo split correction.
+    Sketch errors against the device FP64 path with the same hashes (itself
+    within 1e-10 of the oracle in the tests); estimate errors against the
+    dense FP64 product; recovered set against {C > tau}."""
+    name = "fft4096x16384_fp32"
+    c = workloads.CONFIGS[name]
+    m, n, p, d, b, tau = c["m"], c["n"], c["p"], c["d"], c["b"], c["tau"]
+    AT, B, _ = workloads.make(name)
+    C_true = AT.astype(np.float64).T @ B.astype(np.float64)
+    truth = set(zip(*np.nonzero(C_true > tau)))
+    ref = cma.compress("fft", cma.DeviceArray.from_numpy(AT.astype(np.float64)),
+                       cma.DeviceArray.from_numpy(B.astype(np.float64)), d, b, seed=HASH_SEED,
+                       dtype=np.float64)
+    ref_vals = ref.values()
+    C_ref = ref.decompress()
+    ref.free()
+    dAT, dB = cma.DeviceArray.from_numpy(AT), cma.DeviceArray.from_numpy(B)
+    modes = {"fp32": 0, "fp64_cross_chunk": cma.FLAG_F64_REDUCE, "bf16_mfma_split": cma.FLAG_BF16_MFMA}
+    out = {}
+    for mode, flags in modes.items():
+        sk = None
+        for _ in range(max(1, args.warmup)):
+            sk = cma.compress("fft", dAT, dB, d, b, seed=HASH_SEED, dtype=np.float32, flags=flags,
+                              sketch=sk)
+        cma.synchronize()
+        cma.timing_enable(True)
+        cma.timing_reset()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            sk = cma.compress("fft", dAT, dB, d, b, seed=HASH_SEED, dtype=np.float32, flags=flags,
+                              sketch=sk)
+        cma.synchronize()
+        step_s = (time.perf_counter() - t0) / args.steps
+        k_ms = cma.timing_get("compress")
+        cma.timing_enable(False)
+        v = sk.values()
+        e = v - ref_vals
+        C_est = sk.decompress().astype(np.float64)
+        ii, jj, _ = sk.recover(tau)
+        got = set(zip(ii.tolist(), jj.tolist()))
+        out[mode] = {
+            "sketch": {"max_abs": float(np.abs(e).max()), "rms": float(np.sqrt(np.mean(e * e))),
+                       "rel_l2": float(np.linalg.norm(e) / np.linalg.norm(ref_vals))},
+            "C_vs_fp64_sketch_estimate": {"max_abs": float(np.abs(C_est - C_ref).max()),
+                                          "rms": float(np.sqrt(np.mean((C_est - C_ref) ** 2)))},
+            "C_vs_dense_fp64": {"max_abs": float(np.abs(C_est - C_true).max()),
+                                "rms": float(np.sqrt(np.mean((C_est - C_true) ** 2)))},
+            "recall": len(got & truth) / max(1, len(truth)),
+            "precision": len(got & truth) / max(1, len(got)),
+            "recovered": len(got), "truth": len(truth),
+            "compress_ms": k_ms[0] / max(1, k_ms[1]), "compress_products_per_s": 1.0 / step_s,
+        }
+        print(f"[bench] {mode}: {json.dumps(out[mode])}", file=sys.stderr, flush=True)
+        sk.free()
+    flops = compress_flops("fft", m, n, p, d, b)
+    k_s = out["fp32"]["compress_ms"] * 1e-3
+    return {
+        "metric": "products/s", "value": out["fp32"]["compress_products_per_s"],
+        "unit": "products/s (compress)", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 / out["fp32"]["compress_products_per_s"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": f"synthetic: N(0,1/{n}) noise + {c['plants']} planted entries of {c['magnitude']}",
+        "config": {"workload": f"precision form: fft {m}x{n}x{p}, b={b}, d={d}, tau={tau}, "
+                               "three arithmetic modes", "parallelism": "single GPU"},
+        "roofline": {"bound": "mfma", "achieved": flops / k_s / 1e12, "peak": PEAK_FP32_TFLOPS,
+                     "unit": "TFLOP/s", "frac": flops / k_s / 1e12 / PEAK_FP32_TFLOPS,
+                     "traffic": None, "kernel": "compress (fp32 mode)",
+                     "kernel_ms": k_s * 1e3, "flops_per_launch": flops},
+        "detail": {"modes": out},
+    }
+
+
 def serve_datasets(count, seed):
     """`count` request templates from the batch shape distribution; 30% hold
     CSR operands at 1% density (N(0,1) values plus the planted entries),
@@ -213,8 +291,6 @@ def bench_serve(args):
     seeds = HASH_SEED + np.arange(nreq)
 
     def operands(t):
-        if t["csr"]:
-            return t["AT"], t["B"]
         return t["AT"], t["B"]
 
     sv = cma.Server(hbm_bytes=args.hbm_gb << 30, max_waves=args.waves,
@@ -257,8 +333,9 @@ def bench_serve(args):
     sv.close()
     served_s = t_end - t0
 
-    # one product per launch sequence: the same arrivals, served in order
-    base_lat = np.zeros(nreq)
+    # one product per launch sequence: the same arrivals, served in order,
+    # by the fused single-product path and by the unfused (plain) path where
+    # its LDS limit allows the sketch length (else fused)
     dev = {}
     for k, t in enumerate(templates):  # operands resident on the device, as for the server
         if t["csr"]:
@@ -266,19 +343,26 @@ def bench_serve(args):
         else:
             a, b = t["AT"].array, t["B"].array
         dev[k] = (cma.DeviceArray.from_numpy(a), cma.DeviceArray.from_numpy(b))
-    t1 = time.perf_counter() + 0.05
-    sk = None
-    for r in range(nreq):
-        dt = t1 + arrivals[r] - time.perf_counter()
-        if dt > 0:
-            time.sleep(dt)
-        t = templates[picks[r]]
-        a, b = dev[picks[r]]
-        sk = cma.compress(t["variant"], a, b, t["d"], t["b"], seed=int(seeds[r]), dtype=np.float32)
-        sk.recover(t["tau"])
-        sk.free()
-        base_lat[r] = time.perf_counter() - (t1 + arrivals[r])
-    base_s = time.perf_counter() - t1
+
+    def one_at_a_time(plain):
+        lat_ = np.zeros(nreq)
+        t1 = time.perf_counter() + 0.05
+        for r in range(nreq):
+            dt = t1 + arrivals[r] - time.perf_counter()
+            if dt > 0:
+                time.sleep(dt)
+            t = templates[picks[r]]
+            a, b = dev[picks[r]]
+            fl = cma.FLAG_PLAIN if plain and t["b"] <= PLAIN_MAX_B[t["variant"]] else 0
+            sk = cma.compress(t["variant"], a, b, t["d"], t["b"], seed=int(seeds[r]),
+                              dtype=np.float32, flags=fl)
+            sk.recover(t["tau"])
+            sk.free()
+            lat_[r] = time.perf_counter() - (t1 + arrivals[r])
+        return lat_, time.perf_counter() - t1
+
+    base_lat, base_s = one_at_a_time(False)
+    plain_lat, plain_s = one_at_a_time(True) if args.plain_baseline else (None, None)
 
     # recall of the served results on the first templates
     hits = truth_n = got_n = 0
@@ -314,6 +398,11 @@ def bench_serve(args):
                    "one_product_per_launch": {"products_per_s": nreq / base_s,
                                               "latency_ms": {"p50": 1e3 * q(base_lat, 0.5),
                                                              "p99": 1e3 * q(base_lat, 0.99)}},
+                   "plain_one_product_per_launch": None if plain_lat is None else {
+                       "products_per_s": nreq / plain_s,
+                       "latency_ms": {"p50": 1e3 * q(plain_lat, 0.5),
+                                      "p99": 1e3 * q(plain_lat, 0.99)},
+                       "note": "unfused path for b within its LDS limit, fused otherwise"},
                    "server": st, "recall": hits / max(1, truth_n),
                    "precision": hits / max(1, got_n)},
     }
@@ -327,7 +416,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--plain", action="store_true", help="time the unfused path instead")
     ap.add_argument("--workload", default=DEFAULT_WORKLOAD,
-                    choices=sorted(workloads.CONFIGS) + ["batch512", "serve"],
+                    choices=sorted(workloads.CONFIGS) + ["batch512", "serve", "precision"],
                     help="named workload (default: the metric's configuration)")
     ap.add_argument("--grain", type=int, default=0,
                     help="batch512: work-item grain (0 = autotuned); serve: 0 = 64")
@@ -336,6 +425,8 @@ def main():
     ap.add_argument("--waves", type=int, default=3, help="serve: waves on the device at once")
     ap.add_argument("--wave-products", type=int, default=64, help="serve: requests per wave")
     ap.add_argument("--hbm-gb", type=int, default=64, help="serve: server HBM budget (GiB)")
+    ap.add_argument("--plain-baseline", action="store_true",
+                    help="serve: also replay the arrivals through the unfused path")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copies of MIB MiB (the known-size "
                          "kernel that scales FETCH_SIZE / WRITE_SIZE counter readings)")
@@ -355,6 +446,9 @@ def main():
         return
     if name == "serve":
         print(json.dumps(bench_serve(args)), flush=True)
+        return
+    if name == "precision":
+        print(json.dumps(bench_precision(args)), flush=True)
         return
     c = workloads.CONFIGS[name]
     m, n, p, d, b = c["m"], c["n"], c["p"], c["d"], c["b"]

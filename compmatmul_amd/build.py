@@ -34,7 +34,7 @@ HIP_FLAGS = [
 ]
 
 SOURCES = ["hash.hip", "compress.hip", "compress_reg.hip", "compress_fft_reg.hip", "decompress.hip",
-           "plain.hip", "sparse.hip", "api.cpp", "batch.cpp", "serve.cpp", "host_hash.cpp"]
+           "plain.hip", "sparse.hip", "fft_mfma.hip", "api.cpp", "batch.cpp", "serve.cpp", "host_hash.cpp"]
 
 
 def _run(cmd):

@@ -418,6 +418,48 @@ int compress_device(cm_sketch* sk, const void* AT, int64_t ld_at, const void* B,
     return timed("combine", [&] { return cm::launch_combine(ca, g.stream); });
   }
 
+  if (flags & CM_FLAG_BF16_MFMA) {
+    if (fwht || f64 || !cm::fft_bf16_supported(lb))
+      return fail(CM_ENOTSUP, "the BF16 matrix-core mode takes the FFT variant, float32 and "
+                              "b = 16^k with 2^12 <= b <= 2^28");
+    // two row buffers of at most 512 MiB each
+    const int rows = int(std::max<int64_t>(1, std::min<int64_t>(std::max(n, 1), (int64_t(1) << 29) / (b * 8))));
+    const int nchunks = n > 0 ? (n + rows - 1) / rows : 0;
+    const void* twb = nullptr;
+    int rc;
+    if ((rc = twiddles(lb, 0, &twb))) return rc;
+    const size_t row_bytes = size_t(rows) * b * 8;
+    const size_t chunk_bytes = size_t(d) * b * 8;
+    if ((rc = ensure_buf(g.plainA, row_bytes)) || (rc = ensure_buf(g.plainB, row_bytes)) ||
+        (rc = ensure_buf(g.partial, std::max<size_t>(1, size_t(nchunks) * chunk_bytes))) ||
+        (rc = ensure_buf(g.V, chunk_bytes)))
+      return rc;
+    cm::Bf16FftArgs fa{};
+    fa.AT = static_cast<const float*>(AT);
+    fa.ld_at = ld_at;
+    fa.B = static_cast<const float*>(B);
+    fa.ld_b = ld_b;
+    fa.hA = sk->hA;
+    fa.hB = sk->hB;
+    fa.m = m;
+    fa.n = n;
+    fa.p = p;
+    fa.d = d;
+    fa.lb = lb;
+    fa.rows = rows;
+    fa.nchunks = nchunks;
+    fa.tw = twb;
+    fa.Z0 = static_cast<float2*>(g.plainA.p);
+    fa.Z1 = static_cast<float2*>(g.plainB.p);
+    fa.partial = static_cast<float*>(g.partial.p);
+    if ((rc = timed("compress", [&] { return cm::launch_fft_bf16_compress(fa, g.stream); }))) return rc;
+    const int f64_reduce = (flags & CM_FLAG_F64_REDUCE) || n > 16384;
+    cm::ReduceArgs ra{0, 0, f64_reduce, g.partial.p, nchunks, d, lb, cm::kRegLb2, twb, g.V.p};
+    if ((rc = timed("reduce", [&] { return cm::launch_reduce_inverse(ra, g.stream); }))) return rc;
+    cm::CombineArgs ca{0, 0, g.V.p, d, lb, cm::kRegLb2, twb, sk->P, lb};
+    return timed("combine", [&] { return cm::launch_combine(ca, g.stream); });
+  }
+
   // fast path: FP32 with b >= 1024 runs the register-resident kernels
   // (2048-frequency slices; smaller b zero-padded to one slice)
   const bool reg = !f64 && !(flags & CM_FLAG_LDS_KERNEL) && lb >= kRegMinLb &&

@@ -93,6 +93,23 @@ hipError_t launch_fft_reg_batch(const RegJob* jobs, const int2* items, const int
 inline int reg_lists_count(int fwht, int d, int lb) { return (fwht ? 2 : 1) * (d << reg_lbs(lb)); }
 hipError_t launch_zero_u32(unsigned* p, int n, hipStream_t s);
 
+// BF16 matrix-core FFT compress (fft_mfma.hip): radix-16 Stockham stages
+// as hi/lo-split BF16 MFMA products, b = 16^S with 2^12 <= b <= 2^28.
+// Writes the per-chunk spectra in the sliced layout the reduce kernel reads
+// (slices of 2^kRegLb2).
+struct Bf16FftArgs {
+  const float* AT; int64_t ld_at;
+  const float* B; int64_t ld_b;
+  const uint32_t* hA; const uint32_t* hB;
+  int m, n, p, d, lb;
+  int rows, nchunks;       // inner indices per chunk, chunks
+  const void* tw;          // W_b^j (float2)
+  float2* Z0; float2* Z1;  // rows x b complex each
+  float* partial;          // [nchunks][d][b1][2048] complex
+};
+int fft_bf16_supported(int lb);
+hipError_t launch_fft_bf16_compress(const Bf16FftArgs& a, hipStream_t s);
+
 // Sum of the per-chunk partial spectra, then the per-slice inverse transform.
 struct ReduceArgs {
   int fwht, f64, f64_reduce;
