@@ -1,0 +1,381 @@
+// Slice-group FP32 FWHT compress: one scatter per row for G slices.
+//
+// The single-wave kernel (compress_reg.hip) gives every slice of the
+// spectrum its own modulated count-sketch, so each row is scattered b1
+// times.  Here a workgroup of G = 2^LOG_G waves owns G consecutive slices
+// s = group*G + w of one repetition:
+//  * the row is staged once into LDS (2048 keys at a time) by all waves;
+//  * it is scattered once into a G x 2048-bucket sketch S[hw][l] (h = hash,
+//    l = h mod 2048, hw = (h >> 11) mod G), modulated by the group bits
+//    (-1)^popc((h >> (11 + LOG_G)) & group).  Wave w scatters only the keys
+//    with hw = w, into its own partition, following a conflict-free schedule
+//    (rounds by occurrence within the bucket, as in compress_reg.hip);
+//  * a G-point Walsh-Hadamard transform across the partitions,
+//      x_s[l] = sum_hw (-1)^popc(hw & w) S[hw][l],
+//    turns partition w into the modulated sketch of slice s (the sign
+//    (-1)^popc((h >> 11) & s) splits into the hw and group parts);
+//  * each wave then runs the 2048-point register WHT of compress_reg.hip on
+//    its partition and keeps its slice's accumulator in 32 VGPRs.
+// Per (t, k) this scatters each key once per G slices instead of G times,
+// and reads each row once per group; the partial spectra keep the
+// [chunk][t][slice][2048] layout, so reduction and combine are unchanged.
+#include "device_common.hpp"
+#include "kernels.hpp"
+
+namespace cm {
+
+namespace {
+
+constexpr int LB2 = 11;          // slice length 2048
+constexpr int B2 = 1 << LB2;
+constexpr int R = B2 / 64;       // values per lane (32)
+constexpr int TROW = 33;         // padded transpose row
+constexpr int SLEN = 64 * TROW;  // LDS floats per partition
+constexpr int SEG = 2048;        // keys staged at a time
+constexpr int RCAP = 8;
+constexpr int MAXSEG = 32;       // keys <= MAXSEG * SEG = 65536
+constexpr uint32_t F_SIGN = 1u << 31;
+constexpr int GB = 16;           // schedule groups per batch
+
+__device__ __forceinline__ float flip(float v, uint32_t sgn) {
+  return __int_as_float(__float_as_int(v) ^ sgn);
+}
+
+__host__ __device__ __forceinline__ int64_t list_cap(int keys) {
+  const int64_t segs = (int64_t(keys) + SEG - 1) / SEG;
+  return int64_t(keys) + segs * 64 * RCAP;
+}
+
+struct Layout {
+  uint32_t* LA;
+  uint32_t* LB;
+  int* ngroups;
+  int64_t cap_a, cap_b;
+  int lists;  // d * b1: one per (t, slice) and operand
+};
+
+__host__ __device__ __forceinline__ Layout layout(const RegJob& j) {
+  Layout o;
+  o.lists = j.d << j.lbs;
+  o.cap_a = list_cap(j.m);
+  o.cap_b = list_cap(j.p);
+  o.LA = static_cast<uint32_t*>(j.lists);
+  o.LB = o.LA + o.lists * o.cap_a;
+  o.ngroups = reinterpret_cast<int*>(o.LB + o.lists * o.cap_b);
+  return o;
+}
+
+// Schedule of list g (g < lists: A's row for slice ts = g, else B's): the
+// keys of partition w = slice mod G, bucket l = h mod 2048, sign modulated
+// by the group bits.  Entry: l << 2 (or a dummy slot 2048 + lane) | key
+// offset in the 2048-key segment << 18 | sign << 31.
+template <int LOG_G>
+__global__ void __launch_bounds__(64) k_grp_lists(RegJob j) {
+  constexpr int G = 1 << LOG_G;
+  __shared__ uint32_t cnt[B2];
+  __shared__ uint8_t rnd[SEG];
+  __shared__ int nr[RCAP];
+  const Layout lay = layout(j);
+  const int g = blockIdx.x;
+  const bool side_b = g >= lay.lists;
+  const int ts = side_b ? g - lay.lists : g;
+  const int t = ts >> j.lbs;
+  const uint32_t slice = uint32_t(ts & ((1 << j.lbs) - 1));
+  const uint32_t w = slice & (G - 1), group = slice >> LOG_G;
+  const int keys = side_b ? j.p : j.m;
+  const uint32_t* __restrict__ hh = (side_b ? j.hB : j.hA) + int64_t(t) * keys;
+  uint32_t* __restrict__ L = side_b ? lay.LB + ts * lay.cap_b : lay.LA + ts * lay.cap_a;
+  const int lane = threadIdx.x;
+  for (int e = lane; e < B2; e += 64) cnt[e] = 0;
+  int out = 0;
+  for (int s0 = 0; s0 < keys; s0 += SEG) {
+    const int sl = min(SEG, keys - s0);
+    if (lane < RCAP) nr[lane] = 0;
+    __syncthreads();
+    for (int c = 0; c < sl; c += 64) {
+      if (c + lane < sl) {
+        const uint32_t h = hbucket(hh[s0 + c + lane]);
+        int r = -1;
+        if (((h >> LB2) & (G - 1)) == w) {
+          r = min(int(atomicAdd(&cnt[h & (B2 - 1)], 1u)), RCAP - 1);
+          atomicAdd(&nr[r], 1);
+        }
+        rnd[c + lane] = uint8_t(r < 0 ? 255 : r);
+      }
+      __syncthreads();
+    }
+    int base[RCAP], run[RCAP];
+    int acc = out;
+#pragma unroll
+    for (int r = 0; r < RCAP; ++r) {
+      base[r] = acc;
+      acc += (nr[r] + 63) / 64;
+      run[r] = 0;
+    }
+    for (int c = 0; c < sl; c += 64) {
+      const bool ok = c + lane < sl;
+      const int rv = ok ? int(rnd[c + lane]) : 255;
+      const int r = rv == 255 ? -1 : rv;
+      uint32_t ent = 0;
+      if (r >= 0) {
+        const uint32_t e = hh[s0 + c + lane];
+        const uint32_t h = hbucket(e);
+        const uint32_t neg = hneg(e) ^ (__popc((h >> (LB2 + LOG_G)) & group) & 1u);
+        ent = ((h & (B2 - 1)) << 2) | (uint32_t(c + lane) << 18) | (neg ? F_SIGN : 0u);
+      }
+#pragma unroll
+      for (int rr = 0; rr < RCAP; ++rr) {
+        const uint64_t mask = __ballot(r == rr);
+        if (r == rr) {
+          const int pos = run[rr] + __popcll(mask & ((1ull << lane) - 1ull));
+          L[int64_t(base[rr]) * 64 + pos] = ent;
+        }
+        run[rr] += __popcll(mask);
+      }
+    }
+#pragma unroll
+    for (int rr = 0; rr < RCAP; ++rr) {  // pad rounds to whole groups (dummy slots)
+      const int full = (nr[rr] + 63) / 64 * 64;
+      for (int slot = nr[rr] + lane; slot < full; slot += 64)
+        L[int64_t(base[rr]) * 64 + slot] = uint32_t(B2 + (slot & 63)) << 2;
+    }
+    if (lane < RCAP) {
+      const int64_t row = int64_t(g) * MAXSEG + s0 / SEG;
+      lay.ngroups[row * RCAP + lane] = (nr[lane] + 63) / 64;
+    }
+    out = acc;
+    __syncthreads();
+  }
+}
+
+__device__ __forceinline__ void wht_reg(float (&x)[R]) {
+#pragma unroll
+  for (int s = 1; s < R; s <<= 1) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      if (r & s) continue;
+      const float a = x[r], b = x[r | s];
+      x[r] = a + b;
+      x[r | s] = a - b;
+    }
+  }
+}
+
+// KIND 0 plain writes, 1 read-add-writes, 2 LDS atomics (see compress_reg.hip)
+template <int KIND>
+__device__ __forceinline__ void scatter_section(const uint32_t* __restrict__ lp, int ng, float* S,
+                                                const float* Rb, int lane) {
+  const char* Rbb = reinterpret_cast<const char*>(Rb);
+  char* Sb = reinterpret_cast<char*>(S);
+  for (int g0 = 0; g0 < ng; g0 += GB) {
+    const int nb = min(GB, ng - g0);
+    uint32_t ent[GB];
+    float v[GB];
+#pragma unroll
+    for (int q = 0; q < GB; ++q) ent[q] = lp[int64_t(g0 + min(q, nb - 1)) * 64 + lane];
+#pragma unroll
+    for (int q = 0; q < GB; ++q)
+      v[q] = flip(*reinterpret_cast<const float*>(Rbb + ((ent[q] >> 16) & 0x1ffcu)), ent[q] & F_SIGN);
+#pragma unroll
+    for (int q = 0; q < GB; ++q) {
+      if (q < nb) {
+        float* d = reinterpret_cast<float*>(Sb + (ent[q] & 0xfffcu));
+        if (KIND == 2)
+          __hip_atomic_fetch_add(d, v[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        else if (KIND == 1)
+          *d += v[q];
+        else
+          *d = v[q];
+      }
+    }
+  }
+}
+
+// Scatter one row for the G slices of the group, cross-partition WHT, then
+// the register WHT of this wave's slice.  On return lane L, register r holds
+// frequency f_of(L, r) of slice group*G + w.
+template <int LOG_G>
+__device__ __forceinline__ void sketch_transform_grp(const float* __restrict__ row, int keys,
+                                                     const uint32_t* __restrict__ list,
+                                                     const int* __restrict__ segcnt, float* S,
+                                                     float* Rb, int w, int lane, float (&x)[R]) {
+  constexpr int G = 1 << LOG_G;
+  float* Sw = S + w * SLEN;
+  const int tid = w * 64 + lane;
+  {
+    float4* S4 = reinterpret_cast<float4*>(Sw);
+#pragma unroll
+    for (int q = 0; q < B2 / 4 / 64; ++q) S4[q * 64 + lane] = make_float4(0.f, 0.f, 0.f, 0.f);
+  }
+  const int nseg = (keys + SEG - 1) / SEG;
+  const uint32_t* __restrict__ lp = list;
+  for (int seg = 0; seg < nseg; ++seg) {
+    const int s0 = seg * SEG;
+    const int sl = min(SEG, keys - s0);
+    float v[SEG / (64 * G)];
+#pragma unroll
+    for (int q = 0; q < SEG / (64 * G); ++q) v[q] = row[s0 + min(q * 64 * G + tid, sl - 1)];
+    __syncthreads();  // every wave is done with the previous segment
+#pragma unroll
+    for (int q = 0; q < SEG / (64 * G); ++q) Rb[q * 64 * G + tid] = v[q];
+    __syncthreads();
+    const int* __restrict__ nr = segcnt + seg * RCAP;
+    int nw = 0;
+    for (int r = 1; r < RCAP - 1; ++r) nw += nr[r];
+    scatter_section<0>(lp, nr[0], Sw, Rb, lane);
+    lp += int64_t(nr[0]) * 64;
+    scatter_section<1>(lp, nw, Sw, Rb, lane);
+    lp += int64_t(nw) * 64;
+    scatter_section<2>(lp, nr[RCAP - 1], Sw, Rb, lane);
+    lp += int64_t(nr[RCAP - 1]) * 64;
+  }
+  __syncthreads();  // all partitions complete
+  if constexpr (G > 1) {
+    // wave w transforms positions l in [w 2048/G, (w+1) 2048/G) across the
+    // G partitions (natural-order WHT over hw)
+    constexpr int PER = R / G;
+    float c[G][PER];
+    const int l0 = w * (B2 / G);
+#pragma unroll
+    for (int hw = 0; hw < G; ++hw)
+#pragma unroll
+      for (int q = 0; q < PER; ++q) c[hw][q] = S[hw * SLEN + l0 + q * 64 + lane];
+#pragma unroll
+    for (int st = 1; st < G; st <<= 1)
+#pragma unroll
+      for (int hw = 0; hw < G; ++hw) {
+        if (hw & st) continue;
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+          const float a = c[hw][q], b = c[hw | st][q];
+          c[hw][q] = a + b;
+          c[hw | st][q] = a - b;
+        }
+      }
+#pragma unroll
+    for (int hw = 0; hw < G; ++hw)
+#pragma unroll
+      for (int q = 0; q < PER; ++q) S[hw * SLEN + l0 + q * 64 + lane] = c[hw][q];
+    __syncthreads();
+  }
+#pragma unroll
+  for (int r = 0; r < R; ++r) x[r] = Sw[r * 64 + lane];
+  __syncthreads();
+  wht_reg(x);  // index bits 6..10
+  {
+    float* wbase = Sw + (lane >> 5) * TROW + (lane & 31);
+#pragma unroll
+    for (int r = 0; r < R; ++r) wbase[r * 2 * TROW] = x[r];
+  }
+  __syncthreads();
+  {
+    const float* rbase = Sw + (((lane & 31) << 1) | (lane >> 5)) * TROW;
+#pragma unroll
+    for (int r = 0; r < R; ++r) x[r] = rbase[r];
+  }
+  __syncthreads();
+  wht_reg(x);  // index bits 0..4
+#pragma unroll
+  for (int jj = 0; jj < R / 2; ++jj) {  // index bit 5 across lane halves
+    const auto pq = __builtin_amdgcn_permlane32_swap(__float_as_uint(x[2 * jj]),
+                                                     __float_as_uint(x[2 * jj + 1]), false, false);
+    const float p = __uint_as_float(pq[0]), q = __uint_as_float(pq[1]);
+    x[2 * jj] = p + q;
+    x[2 * jj + 1] = p - q;
+  }
+}
+
+__device__ __forceinline__ int f_of(int L, int r) {
+  return ((L & 31) << 6) | ((r & 1) << 5) | ((r & ~1) + (L >> 5));
+}
+
+// unit u = (chunk, t, group); wave w owns slice group * G + w
+template <int LOG_G>
+__device__ __forceinline__ void grp_unit(const RegJob& j, int u, float* S, float* Rb) {
+  constexpr int G = 1 << LOG_G;
+  const Layout lay = layout(j);
+  const int per_chunk = lay.lists >> LOG_G;
+  const int chunk = u / per_chunk;
+  const int tg = u - chunk * per_chunk;  // t * (b1 / G) + group
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int ts = tg * G + w;              // t * b1 + slice
+  const int k0 = chunk * j.kper;
+  const int k1 = min(j.n, k0 + j.kper);
+  const uint32_t* __restrict__ la = lay.LA + ts * lay.cap_a;
+  const uint32_t* __restrict__ lb = lay.LB + ts * lay.cap_b;
+  const int* __restrict__ sca = lay.ngroups + int64_t(ts) * MAXSEG * RCAP;
+  const int* __restrict__ scb = lay.ngroups + int64_t(lay.lists + ts) * MAXSEG * RCAP;
+  float acc[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) acc[r] = 0.f;
+  for (int k = k0; k < k1; ++k) {
+    float xa[R], xb[R];
+    sketch_transform_grp<LOG_G>(j.AT + int64_t(k) * j.ld_at, j.m, la, sca, S, Rb, w, lane, xa);
+    sketch_transform_grp<LOG_G>(j.B + int64_t(k) * j.ld_b, j.p, lb, scb, S, Rb, w, lane, xb);
+#pragma unroll
+    for (int r = 0; r < R; ++r) acc[r] = fmaf(xa[r], xb[r], acc[r]);
+  }
+  float* Sw = S + w * SLEN;
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < R; ++r) Sw[f_of(lane, r)] = acc[r];
+  __syncthreads();
+  float* out = j.partial + (int64_t(chunk) * lay.lists + ts) * B2;
+#pragma unroll
+  for (int r = 0; r < R; ++r) out[r * 64 + lane] = Sw[r * 64 + lane];
+}
+
+template <int LOG_G>
+__global__ void __launch_bounds__(64 << LOG_G) k_compress_fwht_grp(RegJob j, int units) {
+  __shared__ __align__(16) float S[SLEN << LOG_G];
+  __shared__ __align__(16) float Rb[SEG];
+  const int G = gridDim.x;
+  const int g = blockIdx.x;
+  const int u = (g & 7) * (G >> 3) + (g >> 3);  // XCD-aware numbering
+  if (u >= units) return;
+  grp_unit<LOG_G>(j, u, S, Rb);
+}
+
+template <int LOG_G>
+int slots_t() {
+  int dev = 0, cus = 0, per_cu = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_compress_fwht_grp<LOG_G>, 64 << LOG_G, 0) !=
+          hipSuccess ||
+      cus <= 0 || per_cu <= 0)
+    return 512;
+  return cus * per_cu;
+}
+
+template <int LOG_G>
+hipError_t launch_t(const RegJob& j, hipStream_t s) {
+  const int lists = j.d << j.lbs;
+  k_grp_lists<LOG_G><<<dim3(unsigned(2 * lists)), dim3(64), 0, s>>>(j);
+  const int units = j.nchunks * (lists >> LOG_G);
+  const int grid = (units + 7) / 8 * 8;
+  k_compress_fwht_grp<LOG_G><<<dim3(unsigned(grid)), dim3(64 << LOG_G), 0, s>>>(j, units);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+int fwht_grp_log_g(int lb) { return lb >= 13 ? 2 : (lb == 12 ? 1 : 0); }
+
+int fwht_grp_supported(int lb, int m, int p) {
+  return lb >= 12 && lb <= 30 && m <= MAXSEG * SEG && p <= MAXSEG * SEG;
+}
+
+int fwht_grp_slots(int lb) { return fwht_grp_log_g(lb) == 2 ? slots_t<2>() : slots_t<1>(); }
+
+size_t fwht_grp_lists_bytes(int d, int lb, int m, int p) {
+  const size_t lists = size_t(d) << reg_lbs(lb);
+  return lists * size_t(list_cap(m) + list_cap(p)) * 4 + 2 * lists * MAXSEG * RCAP * sizeof(int) + 256;
+}
+
+hipError_t launch_fwht_grp(const RegJob& j, hipStream_t s) {
+  if (j.nchunks == 0) return hipSuccess;
+  return fwht_grp_log_g(j.lb) == 2 ? launch_t<2>(j, s) : launch_t<1>(j, s);
+}
+
+}  // namespace cm

@@ -77,6 +77,16 @@ int fft_reg_slots();
 size_t fft_reg_lists_bytes(int d, int lb, int m, int p);
 hipError_t launch_fft_reg(const RegJob& j, hipStream_t s);
 
+// Slice-group FP32 FWHT compress (compress_grp.hip): a workgroup of
+// 2^fwht_grp_log_g(lb) waves scatters each row once for that many slices.
+// Same RegJob and partial layout as the single-wave kernel; lists from
+// fwht_grp_lists_bytes().
+int fwht_grp_supported(int lb, int m, int p);
+int fwht_grp_log_g(int lb);
+int fwht_grp_slots(int lb);  // resident workgroups
+size_t fwht_grp_lists_bytes(int d, int lb, int m, int p);
+hipError_t launch_fwht_grp(const RegJob& j, hipStream_t s);
+
 // Many products.  Schedules: `blocks[g]` = (job, list) of workgroup g.
 // Compress: one persistent launch drains eight work queues, queue x holding
 // items[qstart[x] .. qstart[x + 1]) = (job, unit); the workgroups of XCD x

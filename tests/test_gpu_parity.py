@@ -180,7 +180,8 @@ def test_errors_raise():
 @pytest.mark.parametrize("m,n,p,b", [(2048, 96, 2048, 1 << 14), (100, 70, 37, 1 << 12),
                                      (1, 5, 3, 1 << 13), (300, 33, 0, 1 << 12),
                                      (60000, 3, 50, 1 << 12), (5000, 20, 4100, 1 << 15),
-                                     (300, 40, 200, 1 << 10), (64, 9, 70, 1 << 10)])
+                                     (300, 40, 200, 1 << 10), (64, 9, 70, 1 << 10),
+                                     (700, 50, 65536, 1 << 16), (4096, 30, 4096, 1 << 13)])
 def test_fwht_register_kernel_matches_lds_kernel(m, n, p, b):
     print(f"reg vs lds m={m} n={n} p={p} b={b}", flush=True)
     AT, B, _ = workloads.planted(m, n, max(p, 1), 4, 5.0, m + n, np.float32)
@@ -190,12 +191,15 @@ def test_fwht_register_kernel_matches_lds_kernel(m, n, p, b):
     lds = cma.compress("fwht", AT, B, d, b, seed=3, dtype=np.float32,
                        flags=cma.FLAG_LDS_KERNEL)
     again = cma.compress("fwht", AT, B, d, b, seed=3, dtype=np.float32)
-    v1, v2 = reg.values(), lds.values()
-    assert np.array_equal(v1, again.values())  # single-wave scatter: deterministic
+    single = cma.compress("fwht", AT, B, d, b, seed=3, dtype=np.float32,
+                          flags=cma.FLAG_SINGLE_WAVE)
+    v1, v2, v3 = reg.values(), lds.values(), single.values()
+    assert np.array_equal(v1, again.values())  # scheduled scatter: deterministic
     if p == 0 or m == 0:
-        assert not v1.any() and not v2.any()
+        assert not v1.any() and not v2.any() and not v3.any()
         return
     assert rel_l2(v1, v2) <= 2e-6
+    assert rel_l2(v3, v2) <= 2e-6  # slice-group and single-wave kernels
     hA, hB = reg.hashes()
     want = orc.sketch_from_product("fwht", AT.astype(np.float64).T @ B.astype(np.float64),
                                    hA, hB, b)
