@@ -18,7 +18,8 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libcompmatmul_amd.so")
+# COMPMATMUL_AMD_LIB selects another build of the C-ABI library
+LIB_PATH = os.environ.get("COMPMATMUL_AMD_LIB") or os.path.join(HERE, "libcompmatmul_amd.so")
 
 FFT, FWHT = 0, 1
 F64, F32 = 0, 1

@@ -35,7 +35,7 @@ constexpr int SEG = 2048;        // keys staged at a time
 constexpr int RCAP = 8;
 constexpr int MAXSEG = 32;       // keys <= MAXSEG * SEG = 65536
 constexpr uint32_t F_SIGN = 1u << 31;
-constexpr int GB = 16;           // schedule groups per batch
+constexpr int GB = 8;            // schedule groups per batch
 
 __device__ __forceinline__ float flip(float v, uint32_t sgn) {
   return __int_as_float(__float_as_int(v) ^ sgn);
@@ -260,20 +260,23 @@ __device__ __forceinline__ void sketch_transform_grp(const float* __restrict__ r
   }
 #pragma unroll
   for (int r = 0; r < R; ++r) x[r] = Sw[r * 64 + lane];
-  __syncthreads();
+  // the register WHT and transposes touch only this wave's partition: LDS
+  // operations of one wave complete in order, so only the compiler must be
+  // kept from reordering them
+  asm volatile("" ::: "memory");
   wht_reg(x);  // index bits 6..10
   {
     float* wbase = Sw + (lane >> 5) * TROW + (lane & 31);
 #pragma unroll
     for (int r = 0; r < R; ++r) wbase[r * 2 * TROW] = x[r];
   }
-  __syncthreads();
+  asm volatile("" ::: "memory");
   {
     const float* rbase = Sw + (((lane & 31) << 1) | (lane >> 5)) * TROW;
 #pragma unroll
     for (int r = 0; r < R; ++r) x[r] = rbase[r];
   }
-  __syncthreads();
+  asm volatile("" ::: "memory");
   wht_reg(x);  // index bits 0..4
 #pragma unroll
   for (int jj = 0; jj < R / 2; ++jj) {  // index bit 5 across lane halves
