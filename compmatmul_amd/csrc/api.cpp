@@ -467,11 +467,13 @@ int compress_device(cm_sketch* sk, const void* AT, int64_t ld_at, const void* B,
   if (reg) {
     const int lbv = std::max(lb, cm::kRegLb2);  // transform length actually run
     const int lbs = cm::reg_lbs(lb);
-    // FWHT with >= 2 slices: workgroups of 2^logg waves share one scatter
-    const bool grp = fwht && !(flags & CM_FLAG_SINGLE_WAVE) && cm::fwht_grp_supported(lb, m, p);
-    const int logg = grp ? cm::fwht_grp_log_g(lb) : 0;
+    // with >= 2 slices, workgroups of 2^logg waves share one scatter
+    const bool grp = !(flags & CM_FLAG_SINGLE_WAVE) &&
+                     (fwht ? cm::fwht_grp_supported(lb, m, p) : cm::fft_grp_supported(lb, m, p));
+    const int logg = grp ? (fwht ? cm::fwht_grp_log_g(lb) : cm::fft_grp_log_g(lb)) : 0;
     const int64_t per_chunk = (int64_t(d) << lbs) >> logg;  // workgroups per chunk of k
-    const int64_t slots = grp ? cm::fwht_grp_slots(lb) : fwht ? cm::fwht_reg_slots() : cm::fft_reg_slots();
+    const int64_t slots = grp ? (fwht ? cm::fwht_grp_slots(lb) : cm::fft_grp_slots(lb))
+                              : fwht ? cm::fwht_reg_slots() : cm::fft_reg_slots();
     const int64_t rounds = std::max<int64_t>(1, (int64_t(n) * per_chunk + slots * 64 - 1) / (slots * 64));
     int64_t nchunks = std::max<int64_t>(1, slots * rounds / per_chunk);
     nchunks = std::min<int64_t>(nchunks, std::max(1, n));
@@ -487,7 +489,8 @@ int compress_device(cm_sketch* sk, const void* AT, int64_t ld_at, const void* B,
     }
     if ((rc = ensure_buf(g.partial, std::max<size_t>(1, size_t(nchunks) * chunk_bytes)))) return rc;
     if ((rc = ensure_buf(g.V, chunk_bytes))) return rc;
-    const size_t lists_bytes = grp ? cm::fwht_grp_lists_bytes(d, lb, m, p)
+    const size_t lists_bytes = grp ? (fwht ? cm::fwht_grp_lists_bytes(d, lb, m, p)
+                                           : cm::fft_grp_lists_bytes(d, lb, m, p))
                                : fwht ? cm::fwht_reg_lists_bytes(d, lb, m, p)
                                       : cm::fft_reg_lists_bytes(d, lb, m, p);
     if ((rc = ensure_buf(g.entries, lists_bytes))) return rc;
@@ -511,8 +514,8 @@ int compress_device(cm_sketch* sk, const void* AT, int64_t ld_at, const void* B,
     j.t2 = t2;
     j.partial = static_cast<float*>(g.partial.p);
     if ((rc = timed("compress", [&] {
-           return grp ? cm::launch_fwht_grp(j, g.stream)
-                      : fwht ? cm::launch_fwht_reg(j, g.stream) : cm::launch_fft_reg(j, g.stream);
+           if (grp) return fwht ? cm::launch_fwht_grp(j, g.stream) : cm::launch_fft_grp(j, g.stream);
+           return fwht ? cm::launch_fwht_reg(j, g.stream) : cm::launch_fft_reg(j, g.stream);
          })))
       return rc;
     const int f64_reduce = (flags & CM_FLAG_F64_REDUCE) || n > 16384;

@@ -1,0 +1,382 @@
+// Slice-group FP32 FFT compress: one scatter of the pair-packed row for G
+// slices (the FFT counterpart of compress_grp.hip).
+//
+// Slice s of the spectrum X[s + b1 f] (b1 = b / 2048 slices of 2048) is the
+// 2048-point DFT of the modulated sketch z_s[l] = sum_{h mod 2048 = l}
+// c_h W_b^(h s).  A workgroup of G = 2^LOG_G waves owns the strided slices
+// s_j = s0 + j b1/G (j < G) of one repetition.  With h = l + 2048 q and
+// q = hw + G hg (hw = q mod G):
+//   W_b^(h s_j) = W_b^(h s0) W_G^(hw j) W_{2048 G}^(l j),
+// so every slice of the group follows from ONE scatter with the slice-s0
+// coefficients c_h W_b^(h s0) into G partitions P[hw][l]:
+//   z_{s_j}[l] = W_{2048 G}^(l j) sum_hw W_G^(hw j) P[hw][l],
+// a G-point DFT across the partitions and a twiddle per (l, j).  Wave w
+// scatters the keys of partition hw = w (conflict-free schedule, as in the
+// single-wave kernel), the waves then run the cross-partition step on a
+// share of the positions l each, and wave j runs the 2048-point register
+// DFT (fft2048_reg.hpp) of slice s_j and accumulates Q += z^2.  The partial
+// spectra keep the [chunk][t][slice][2048] layout.
+#include "device_common.hpp"
+#include "fft2048_reg.hpp"
+#include "kernels.hpp"
+
+namespace cm {
+
+namespace {
+
+using f2k::R;
+using f2k::fq_of;
+
+constexpr int LB2 = 11;
+constexpr int B2 = 1 << LB2;
+constexpr int SLEN = 2112;   // complex slots per partition (transpose + dummies)
+constexpr int SEG = 2048;    // keys staged at a time
+constexpr int RCAP = 8;
+constexpr int MAXSEG = 64;   // m + p <= MAXSEG * SEG = 131072
+constexpr int GB = 8;
+
+__host__ __device__ __forceinline__ int64_t list_cap(int keys) {
+  const int64_t segs = (int64_t(keys) + SEG - 1) / SEG;
+  return int64_t(keys) + segs * 64 * RCAP;
+}
+
+struct Layout {
+  float2* C;      // per-item coefficients
+  uint32_t* L;    // entries
+  int* segcnt;    // [list][MAXSEG][RCAP] group counts
+  int64_t cap;
+  int lists;      // d * b1, one per (t, s0, partition)
+};
+
+__host__ __device__ __forceinline__ Layout layout(const RegJob& j) {
+  Layout o;
+  o.lists = j.d << j.lbs;
+  o.cap = list_cap(j.m + j.p);
+  o.C = static_cast<float2*>(j.lists);
+  o.L = reinterpret_cast<uint32_t*>(o.C + o.lists * o.cap);
+  o.segcnt = reinterpret_cast<int*>(o.L + o.lists * o.cap);
+  return o;
+}
+
+// Schedule of list g = (t * (b1/G) + s0) * G + w over the concatenated keys
+// [0, m + p): the keys of partition hw = w, bucket l = h mod 2048,
+// coefficient s W_b^(h s0) (times i for B).  Entry: l << 3 (byte offset of
+// a complex, or a dummy slot 2048 + lane) | key offset in the segment << 18.
+template <int LOG_G>
+__global__ void __launch_bounds__(64) k_fft_grp_lists(RegJob j) {
+  constexpr int G = 1 << LOG_G;
+  __shared__ uint32_t cnt[B2];
+  __shared__ uint8_t rnd[SEG];
+  __shared__ int nr[RCAP];
+  const Layout lay = layout(j);
+  const int g = blockIdx.x;
+  const int b1 = 1 << j.lbs;
+  const int t = g >> j.lbs;
+  const int rest = g & (b1 - 1);
+  const uint64_t s0 = uint64_t(rest >> LOG_G);
+  const uint32_t w = uint32_t(rest & (G - 1));
+  const uint64_t bmask = (uint64_t(1) << j.lb) - 1;
+  const float2* __restrict__ tw = static_cast<const float2*>(j.tw);
+  const int m = j.m, p = j.p, keys = m + p;
+  const uint32_t* __restrict__ hA = j.hA + int64_t(t) * m;
+  const uint32_t* __restrict__ hB = j.hB + int64_t(t) * p;
+  uint32_t* __restrict__ Ls = lay.L + g * lay.cap;
+  float2* __restrict__ Cs = lay.C + g * lay.cap;
+  const int lane = threadIdx.x;
+  for (int e = lane; e < B2; e += 64) cnt[e] = 0;
+  int out = 0;
+  for (int base0 = 0; base0 < keys; base0 += SEG) {
+    const int sl = min(SEG, keys - base0);
+    if (lane < RCAP) nr[lane] = 0;
+    __syncthreads();
+    for (int c = 0; c < sl; c += 64) {
+      if (c + lane < sl) {
+        const int key = base0 + c + lane;
+        const uint32_t h = hbucket(key < m ? hA[key] : hB[key - m]);
+        int r = -1;
+        if (((h >> LB2) & (G - 1)) == w) {
+          r = min(int(atomicAdd(&cnt[h & (B2 - 1)], 1u)), RCAP - 1);
+          atomicAdd(&nr[r], 1);
+        }
+        rnd[c + lane] = uint8_t(r < 0 ? 255 : r);
+      }
+      __syncthreads();
+    }
+    int base[RCAP], run[RCAP];
+    int acc = out;
+#pragma unroll
+    for (int r = 0; r < RCAP; ++r) {
+      base[r] = acc;
+      acc += (nr[r] + 63) / 64;
+      run[r] = 0;
+    }
+    for (int c = 0; c < sl; c += 64) {
+      const bool ok = c + lane < sl;
+      const int rv = ok ? int(rnd[c + lane]) : 255;
+      const int r = rv == 255 ? -1 : rv;
+      uint32_t ent = 0;
+      float2 coef = make_float2(0.f, 0.f);
+      if (r >= 0) {
+        const int key = base0 + c + lane;
+        const bool isb = key >= m;
+        const uint32_t e = isb ? hB[key - m] : hA[key];
+        const uint32_t h = hbucket(e);
+        const float2 wv = tw[(uint64_t(h) * s0) & bmask];
+        const float sg = hneg(e) ? -1.f : 1.f;
+        coef = isb ? make_float2(-sg * wv.y, sg * wv.x) : make_float2(sg * wv.x, sg * wv.y);
+        ent = ((h & (B2 - 1)) << 3) | (uint32_t(c + lane) << 18);
+      }
+#pragma unroll
+      for (int rr = 0; rr < RCAP; ++rr) {
+        const uint64_t mask = __ballot(r == rr);
+        if (r == rr) {
+          const int64_t pos = int64_t(base[rr]) * 64 + run[rr] + __popcll(mask & ((1ull << lane) - 1ull));
+          Ls[pos] = ent;
+          Cs[pos] = coef;
+        }
+        run[rr] += __popcll(mask);
+      }
+    }
+#pragma unroll
+    for (int rr = 0; rr < RCAP; ++rr) {  // pad rounds to whole groups (dummy slots)
+      const int full = (nr[rr] + 63) / 64 * 64;
+      for (int slot = nr[rr] + lane; slot < full; slot += 64) {
+        const int64_t pos = int64_t(base[rr]) * 64 + slot;
+        Ls[pos] = uint32_t(B2 + (slot & 63)) << 3;
+        Cs[pos] = make_float2(0.f, 0.f);
+      }
+    }
+    if (lane < RCAP) {
+      const int64_t row = int64_t(g) * MAXSEG + base0 / SEG;
+      lay.segcnt[row * RCAP + lane] = (nr[lane] + 63) / 64;
+    }
+    out = acc;
+    __syncthreads();
+  }
+}
+
+// KIND 0 plain writes, 1 read-add-writes, 2 LDS atomics (as in the
+// single-wave kernel)
+template <int KIND>
+__device__ __forceinline__ void scatter_section(const uint32_t* __restrict__ lp,
+                                                const float2* __restrict__ cp, int ng, float2* S,
+                                                const float* Rb, int lane) {
+  const char* Rbb = reinterpret_cast<const char*>(Rb);
+  char* Sb = reinterpret_cast<char*>(S);
+  for (int g0 = 0; g0 < ng; g0 += GB) {
+    const int nb = min(GB, ng - g0);
+    uint32_t ent[GB];
+    float2 cf[GB];
+    float v[GB];
+#pragma unroll
+    for (int q = 0; q < GB; ++q) {
+      const int64_t pos = int64_t(g0 + min(q, nb - 1)) * 64 + lane;
+      ent[q] = lp[pos];
+      cf[q] = cp[pos];
+    }
+#pragma unroll
+    for (int q = 0; q < GB; ++q) v[q] = *reinterpret_cast<const float*>(Rbb + ((ent[q] >> 16) & 0x1ffcu));
+#pragma unroll
+    for (int q = 0; q < GB; ++q) {
+      if (q < nb) {
+        const float2 c = make_float2(v[q] * cf[q].x, v[q] * cf[q].y);
+        float2* d = reinterpret_cast<float2*>(Sb + (ent[q] & 0xfff8u));
+        if (KIND == 2) {
+          float* df = reinterpret_cast<float*>(d);
+          __hip_atomic_fetch_add(df, c.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          __hip_atomic_fetch_add(df + 1, c.y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        } else if (KIND == 1) {
+          const float2 o = *d;
+          *d = make_float2(o.x + c.x, o.y + c.y);
+        } else {
+          *d = c;
+        }
+      }
+    }
+  }
+}
+
+// G-point DFT (natural order, W_G = exp(-2 pi i / G)), G in {2, 4}
+template <int G>
+__device__ __forceinline__ void dft_g(float2 (&y)[G]) {
+  if constexpr (G == 2) {
+    const float2 a = y[0], b = y[1];
+    y[0] = make_float2(a.x + b.x, a.y + b.y);
+    y[1] = make_float2(a.x - b.x, a.y - b.y);
+  } else {
+    const float2 a = y[0], b = y[1], c = y[2], d = y[3];
+    const float2 ac = make_float2(a.x + c.x, a.y + c.y), am = make_float2(a.x - c.x, a.y - c.y);
+    const float2 bd = make_float2(b.x + d.x, b.y + d.y), bm = make_float2(b.x - d.x, b.y - d.y);
+    y[0] = make_float2(ac.x + bd.x, ac.y + bd.y);
+    y[2] = make_float2(ac.x - bd.x, ac.y - bd.y);
+    y[1] = make_float2(am.x + bm.y, am.y - bm.x);  // am - i bm
+    y[3] = make_float2(am.x - bm.y, am.y + bm.x);  // am + i bm
+  }
+}
+
+// unit u = (chunk, t, s0): wave j owns slice s0 + j b1/G
+template <int LOG_G>
+__device__ __forceinline__ void fft_grp_unit(const RegJob& j, int u, float2* S, float* Rb) {
+  constexpr int G = 1 << LOG_G;
+  const Layout lay = layout(j);
+  const int b1 = 1 << j.lbs;
+  const int per_chunk = lay.lists >> LOG_G;       // (t, s0) pairs
+  const int chunk = u / per_chunk;
+  const int ts0 = u - chunk * per_chunk;            // t * (b1/G) + s0
+  const int t = ts0 / (b1 >> LOG_G);
+  const int s0 = ts0 - t * (b1 >> LOG_G);
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int tid = threadIdx.x;
+  const int slice = s0 + w * (b1 >> LOG_G);
+  const int g = ts0 * G + w;                        // this wave's list
+  const int k0 = chunk * j.kper;
+  const int k1 = min(j.n, k0 + j.kper);
+  const uint32_t* __restrict__ ls = lay.L + g * lay.cap;
+  const float2* __restrict__ cs = lay.C + g * lay.cap;
+  const int* __restrict__ sc = lay.segcnt + int64_t(g) * MAXSEG * RCAP;
+  const float2* __restrict__ t2 = static_cast<const float2*>(j.t2);
+  const float2* __restrict__ tw = static_cast<const float2*>(j.tw);
+  const int keys = j.m + j.p;
+  const int nseg = (keys + SEG - 1) / SEG;
+  // twiddle W_{2048 G}^(l w) for this wave's output partition = W_b^(l w b/(2048 G))
+  const int tw_shift = j.lbs - LOG_G;
+  float2* Sw = S + w * SLEN;
+
+  float qr[R], qi[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) qr[r] = qi[r] = 0.f;
+  for (int k = k0; k < k1; ++k) {
+    const float* ar = j.AT + int64_t(k) * j.ld_at;
+    const float* br = j.B + int64_t(k) * j.ld_b;
+    {
+      float4* S4 = reinterpret_cast<float4*>(Sw);
+#pragma unroll
+      for (int q = 0; q < B2 / 2 / 64; ++q) S4[q * 64 + lane] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    int64_t off = 0;
+    for (int seg = 0; seg < nseg; ++seg) {
+      const int base0 = seg * SEG;
+      const int sl = min(SEG, keys - base0);
+      float v[SEG / (64 * G)];
+#pragma unroll
+      for (int q = 0; q < SEG / (64 * G); ++q) {
+        const int key = base0 + min(q * 64 * G + tid, sl - 1);
+        v[q] = key < j.m ? ar[key] : br[key - j.m];
+      }
+      __syncthreads();  // every wave is done with the previous segment
+#pragma unroll
+      for (int q = 0; q < SEG / (64 * G); ++q) Rb[q * 64 * G + tid] = v[q];
+      __syncthreads();
+      const int* __restrict__ nr = sc + seg * RCAP;
+      int nw = 0;
+      for (int r = 1; r < RCAP - 1; ++r) nw += nr[r];
+      scatter_section<0>(ls + off, cs + off, nr[0], Sw, Rb, lane);
+      off += int64_t(nr[0]) * 64;
+      scatter_section<1>(ls + off, cs + off, nw, Sw, Rb, lane);
+      off += int64_t(nw) * 64;
+      scatter_section<2>(ls + off, cs + off, nr[RCAP - 1], Sw, Rb, lane);
+      off += int64_t(nr[RCAP - 1]) * 64;
+    }
+    __syncthreads();  // all partitions complete
+    {
+      // positions l of this wave: [w 2048/G, (w+1) 2048/G); output
+      // partition jj gets the DFT over hw times W_{2048 G}^(l jj)
+      constexpr int PER = R / G;
+      const int l0 = w * (B2 / G);
+      // the twiddles do not depend on k: the laundered table address makes
+      // the compiler reload them (L1-resident) instead of holding 48 VGPRs
+      // across the loop
+      const float2* twl = tw;
+      asm volatile("" : "+v"(twl));
+#pragma unroll 1  // unrolled, all positions' loads would be live at once
+      for (int q = 0; q < PER; ++q) {
+        const int l = l0 + q * 64 + lane;
+        float2 y[G];
+#pragma unroll
+        for (int hw = 0; hw < G; ++hw) y[hw] = S[hw * SLEN + l];
+        dft_g<G>(y);
+#pragma unroll
+        for (int jj = 1; jj < G; ++jj) {
+          const float2 wv = twl[int64_t(l * jj) << tw_shift];
+          y[jj] = make_float2(y[jj].x * wv.x - y[jj].y * wv.y, y[jj].x * wv.y + y[jj].y * wv.x);
+        }
+#pragma unroll
+        for (int jj = 0; jj < G; ++jj) S[jj * SLEN + l] = y[jj];
+      }
+    }
+    __syncthreads();
+    float re[R], im[R];
+    f2k::dft2048_lds<true>(Sw, t2, lane, re, im);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {  // Q += z^2
+      qr[r] += re[r] * re[r] - im[r] * im[r];
+      qi[r] = fmaf(2.f * re[r], im[r], qi[r]);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < R; ++r) Sw[fq_of(lane, r)] = make_float2(qr[r], qi[r]);
+  __syncthreads();
+  float2* out = reinterpret_cast<float2*>(j.partial) +
+                (int64_t(chunk) * lay.lists + int64_t(t) * b1 + slice) * B2;
+#pragma unroll
+  for (int r = 0; r < R; ++r) out[r * 64 + lane] = Sw[r * 64 + lane];
+}
+
+// Two waves per SIMD is worth a small spill: measured 2.04 ms against
+// 3.34 ms at one wave per SIMD without the spill (fft 2048^3, b = 2^14).
+template <int LOG_G>
+__global__ void __launch_bounds__(64 << LOG_G) __attribute__((amdgpu_waves_per_eu(2, 2)))
+k_compress_fft_grp(RegJob j, int units) {
+  __shared__ __align__(16) float2 S[SLEN << LOG_G];
+  __shared__ __align__(16) float Rb[SEG];
+  const int G = gridDim.x;
+  const int g = blockIdx.x;
+  const int u = (g & 7) * (G >> 3) + (g >> 3);  // XCD-aware numbering
+  if (u >= units) return;
+  fft_grp_unit<LOG_G>(j, u, S, Rb);
+}
+
+template <int LOG_G>
+int slots_t() {
+  int dev = 0, cus = 0, per_cu = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_compress_fft_grp<LOG_G>, 64 << LOG_G, 0) !=
+          hipSuccess ||
+      cus <= 0 || per_cu <= 0)
+    return 512;
+  return cus * per_cu;
+}
+
+template <int LOG_G>
+hipError_t launch_t(const RegJob& j, hipStream_t s) {
+  const int lists = j.d << j.lbs;
+  k_fft_grp_lists<LOG_G><<<dim3(unsigned(lists)), dim3(64), 0, s>>>(j);
+  const int units = j.nchunks * (lists >> LOG_G);
+  const int grid = (units + 7) / 8 * 8;
+  k_compress_fft_grp<LOG_G><<<dim3(unsigned(grid)), dim3(64 << LOG_G), 0, s>>>(j, units);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+int fft_grp_log_g(int lb) { return lb >= 13 ? 2 : (lb == 12 ? 1 : 0); }
+
+int fft_grp_supported(int lb, int m, int p) {
+  return lb >= 12 && lb <= 30 && int64_t(m) + p <= int64_t(MAXSEG) * SEG;
+}
+
+int fft_grp_slots(int lb) { return fft_grp_log_g(lb) == 2 ? slots_t<2>() : slots_t<1>(); }
+
+size_t fft_grp_lists_bytes(int d, int lb, int m, int p) {
+  const size_t lists = size_t(d) << reg_lbs(lb);
+  return lists * size_t(list_cap(m + p)) * 12 + lists * MAXSEG * RCAP * sizeof(int) + 256;
+}
+
+hipError_t launch_fft_grp(const RegJob& j, hipStream_t s) {
+  if (j.nchunks == 0) return hipSuccess;
+  return fft_grp_log_g(j.lb) == 2 ? launch_t<2>(j, s) : launch_t<1>(j, s);
+}
+
+}  // namespace cm

@@ -87,6 +87,15 @@ int fwht_grp_slots(int lb);  // resident workgroups
 size_t fwht_grp_lists_bytes(int d, int lb, int m, int p);
 hipError_t launch_fwht_grp(const RegJob& j, hipStream_t s);
 
+// Slice-group FP32 FFT compress (compress_fft_grp.hip): a workgroup of
+// 2^fft_grp_log_g(lb) waves owns the strided slices s0 + j b1/G and
+// scatters each pair-packed row once for all of them.
+int fft_grp_supported(int lb, int m, int p);
+int fft_grp_log_g(int lb);
+int fft_grp_slots(int lb);
+size_t fft_grp_lists_bytes(int d, int lb, int m, int p);
+hipError_t launch_fft_grp(const RegJob& j, hipStream_t s);
+
 // Many products.  Schedules: `blocks[g]` = (job, list) of workgroup g.
 // Compress: one persistent launch drains eight work queues, queue x holding
 // items[qstart[x] .. qstart[x + 1]) = (job, unit); the workgroups of XCD x

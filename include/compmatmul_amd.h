@@ -53,8 +53,8 @@ extern "C" {
                                  spectra materialised in HBM) */
 #define CM_FLAG_LDS_KERNEL 4u /* FP32: use the general LDS-resident fused
                                  kernel instead of the register-resident ones */
-#define CM_FLAG_SINGLE_WAVE 16u /* FP32 FWHT: one wave per slice instead of
-                                   the slice-group kernel (comparison) */
+#define CM_FLAG_SINGLE_WAVE 16u /* FP32: one wave per slice instead of the
+                                   slice-group kernels (comparison) */
 #define CM_FLAG_BF16_MFMA 8u  /* FP32 FFT, b = 16^k (2^12 .. 2^28): forward
                                  transforms as radix-16 DFT stages on BF16
                                  matrix cores with hi/lo split correction and

@@ -209,7 +209,8 @@ def test_fwht_register_kernel_matches_lds_kernel(m, n, p, b):
 @pytest.mark.parametrize("m,n,p,b", [(2048, 40, 2048, 1 << 14), (100, 70, 37, 1 << 11),
                                      (1, 5, 3, 1 << 13), (300, 33, 0, 1 << 12),
                                      (60000, 3, 50, 1 << 11), (5000, 20, 4100, 1 << 16),
-                                     (300, 40, 200, 1 << 10), (64, 9, 70, 1 << 10)])
+                                     (300, 40, 200, 1 << 10), (64, 9, 70, 1 << 10),
+                                     (700, 50, 65000, 1 << 16), (4096, 30, 4096, 1 << 12)])
 def test_fft_register_kernel_matches_lds_kernel(m, n, p, b):
     print(f"fft reg vs lds m={m} n={n} p={p} b={b}", flush=True)
     AT, B, _ = workloads.planted(m, n, max(p, 1), 4, 5.0, m + n + 1, np.float32)
@@ -219,7 +220,9 @@ def test_fft_register_kernel_matches_lds_kernel(m, n, p, b):
     lds = cma.compress("fft", AT, B, d, b, seed=11, dtype=np.float32,
                        flags=cma.FLAG_LDS_KERNEL)
     again = cma.compress("fft", AT, B, d, b, seed=11, dtype=np.float32)
-    v1, v2 = reg.values(), lds.values()
+    single = cma.compress("fft", AT, B, d, b, seed=11, dtype=np.float32,
+                          flags=cma.FLAG_SINGLE_WAVE)
+    v1, v2, v3 = reg.values(), lds.values(), single.values()
     assert np.array_equal(v1, again.values())
     hA, hB = reg.hashes()
     want = orc.sketch_from_product("fft", AT.astype(np.float64).T @ B.astype(np.float64),
@@ -229,6 +232,8 @@ def test_fft_register_kernel_matches_lds_kernel(m, n, p, b):
         # residue is rounding of |A|^2 (~1e-7 relative), not exactly zero
         scale = float((AT.astype(np.float64) ** 2).sum(axis=1).max())
         assert np.abs(v1).max() < 1e-6 * scale and np.abs(v2).max() < 1e-6 * scale
+        assert np.abs(v3).max() < 1e-6 * scale
         return
     assert rel_l2(v1, want) <= 1e-5
     assert rel_l2(v2, want) <= 1e-5
+    assert rel_l2(v3, want) <= 1e-5  # single-wave kernel
