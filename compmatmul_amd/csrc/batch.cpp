@@ -58,14 +58,16 @@ struct cm_batch {
   cm::HashArgs* hash = nullptr;
   int2* hash_blocks = nullptr;
   int n_hash_blocks = 0;
-  cm::RegJob* jobs[2] = {nullptr, nullptr};  // [fwht]
-  int2* list_blocks[2] = {nullptr, nullptr};
-  int n_list_blocks[2] = {0, 0};
-  int2* items[2] = {nullptr, nullptr};
-  int* qstart[2] = {nullptr, nullptr};
-  int n_items[2] = {0, 0};
-  int grid[2] = {0, 0};
-  unsigned* counters = nullptr;  // 8 queue counters per variant, then a u64 per product
+  // per compress kernel kind (kind_of): FFT then FWHT, 1, 2 or 4 waves
+  static constexpr int kKinds = 6;
+  cm::RegJob* jobs[kKinds] = {};
+  int2* list_blocks[kKinds] = {};
+  int n_list_blocks[kKinds] = {};
+  int2* items[kKinds] = {};
+  int* qstart[kKinds] = {};
+  int n_items[kKinds] = {};
+  int grid[kKinds] = {};
+  unsigned* counters = nullptr;  // 8 queue counters per kind, then a u64 per product
   int counter_words = 0;
   cm::FinishJob* finish = nullptr;
   int2* sum_blocks = nullptr;
@@ -100,6 +102,35 @@ int ilog2(int64_t b) {
 }
 
 double step_cost(const cm_product& q) { return 1.0 + double(q.m + q.p) / 1700.0; }
+
+constexpr int kKinds = cm_batch::kKinds;
+
+// compress kernel of a product: 3 * fwht + log2(waves per workgroup), the
+// slice-group kernels for sketches of at least two 2048-point slices
+int kind_of(const cm_product& q) {
+  const int lb = ilog2(q.b);
+  const bool fw = q.variant == CM_FWHT;
+  int logg = 0;
+  if (fw ? cm::fwht_grp_supported(lb, q.m, q.p) : cm::fft_grp_supported(lb, q.m, q.p))
+    logg = fw ? cm::fwht_grp_log_g(lb) : cm::fft_grp_log_g(lb);
+  return (fw ? 3 : 0) + logg;
+}
+
+size_t kind_lists_bytes(int kind, const cm_product& q) {
+  const int lb = ilog2(q.b);
+  switch (kind) {
+    case 0: return cm::fft_reg_lists_bytes(q.d, lb, q.m, q.p);
+    case 3: return cm::fwht_reg_lists_bytes(q.d, lb, q.m, q.p);
+    default:
+      return kind < 3 ? cm::fft_grp_lists_bytes(q.d, lb, q.m, q.p) : cm::fwht_grp_lists_bytes(q.d, lb, q.m, q.p);
+  }
+}
+
+int kind_slots(int kind) {
+  const int logg = kind % 3, lb_of = logg == 1 ? 12 : 13;
+  if (kind < 3) return logg ? cm::fft_grp_slots(lb_of) : cm::fft_reg_slots();
+  return logg ? cm::fwht_grp_slots(lb_of) : cm::fwht_reg_slots();
+}
 
 uint64_t signature(int count, const cm_product* ps) {
   uint64_t h = 1469598103934665603ull;
@@ -156,7 +187,7 @@ struct Carver {
 
 struct HostPlan {
   // per product
-  std::vector<int> lbv, kper, nchunks, jobidx;
+  std::vector<int> lbv, kper, nchunks, jobidx, kind;
   std::vector<std::vector<uint64_t>> mul;    // multiply-family parameters
   std::vector<std::vector<uint32_t>> tabs;   // tabulation table prefixes
   std::vector<int> tab_len;
@@ -183,10 +214,11 @@ int build_plan(int count, const cm_product* ps, unsigned flags, int grain, const
   hp.kper.resize(count);
   hp.nchunks.resize(count);
   hp.jobidx.resize(count);
+  hp.kind.resize(count);
   hp.mul.resize(count);
   hp.tabs.resize(count);
   hp.tab_len.assign(count, 0);
-  int njobs[2] = {0, 0};
+  int njobs[kKinds] = {};
   for (int i = 0; i < count; ++i) {
     const cm_product& q = ps[i];
     const int fw = q.variant == CM_FWHT;
@@ -199,7 +231,8 @@ int build_plan(int count, const cm_product* ps, unsigned flags, int grain, const
     if (q.n > 0 && (q.n + kper - 1) / kper > max_chunks) kper = int((q.n + max_chunks - 1) / max_chunks);
     hp.kper[i] = kper;
     hp.nchunks[i] = q.n > 0 ? (q.n + kper - 1) / kper : 0;
-    hp.jobidx[i] = njobs[fw]++;
+    hp.kind[i] = kind_of(q);
+    hp.jobidx[i] = njobs[hp.kind[i]]++;
     if (size_only) {  // sizes of the parameter blocks only
       if (q.family == CM_MULTIPLY_ADD_SHIFT || q.family == CM_MULTIPLY_SHIFT) {
         hp.mul[i].resize(size_t(4) * q.d * (q.family == CM_MULTIPLY_ADD_SHIFT ? 2 : 1));
@@ -237,10 +270,9 @@ int build_plan(int count, const cm_product* ps, unsigned flags, int grain, const
   // host tables
   std::vector<cm::HashArgs> hash(count);
   std::vector<int2> hash_blocks;
-  std::vector<cm::RegJob> jobs[2];
-  jobs[0].resize(njobs[0]);
-  jobs[1].resize(njobs[1]);
-  std::vector<int2> list_blocks[2];
+  std::vector<cm::RegJob> jobs[kKinds];
+  for (int k = 0; k < kKinds; ++k) jobs[k].resize(njobs[k]);
+  std::vector<int2> list_blocks[kKinds];
   std::vector<cm::FinishJob> fin(count);
   std::vector<int2> sum_blocks, red_blocks, comb_blocks;
   std::vector<cm::DecompressArgs> dec(count);
@@ -248,7 +280,7 @@ int build_plan(int count, const cm_product* ps, unsigned flags, int grain, const
   const void* t2 = nullptr;
   std::vector<const void*> tw(count, nullptr);
   if (!size_only) {
-    if (njobs[0] && (rc = cm::rt::fft_t2_table(&t2))) {
+    if (njobs[0] + njobs[1] + njobs[2] && (rc = cm::rt::fft_t2_table(&t2))) {
       delete bt;
       return rc;
     }
@@ -271,7 +303,6 @@ int build_plan(int count, const cm_product* ps, unsigned flags, int grain, const
     for (int i = 0; i < count; ++i) {
       const cm_product& q = ps[i];
       const int fw = q.variant == CM_FWHT;
-      const int lb = ilog2(q.b);
       auto& P = bt->pp[i];
       if (env.keep) {
         P.hA = env.keep[i].hA;
@@ -286,29 +317,28 @@ int build_plan(int count, const cm_product* ps, unsigned flags, int grain, const
       P.oi = cv.take<int32_t>(size_t(mo));
       P.oj = cv.take<int32_t>(size_t(mo));
       P.ov = cv.take<float>(size_t(mo));
-      lists[i] = cv.take<char>(fw ? cm::fwht_reg_lists_bytes(q.d, lb, q.m, q.p)
-                                  : cm::fft_reg_lists_bytes(q.d, lb, q.m, q.p));
+      lists[i] = cv.take<char>(kind_lists_bytes(hp.kind[i], q));
       const size_t chunk_floats = (size_t(q.d) << hp.lbv[i]) * (fw ? 1 : 2);
       partial[i] = cv.take<float>(size_t(std::max(1, hp.nchunks[i])) * chunk_floats);
       V[i] = cv.take<float>(chunk_floats);
       if (!hp.mul[i].empty() && hp.mul[i].size() > 256) mulp[i] = cv.take<uint64_t>(hp.mul[i].size());
       if (!hp.tabs[i].empty()) tabp[i] = cv.take<uint32_t>(hp.tabs[i].size());
     }
-    bt->counter_words = 16 + 2 * count;
+    bt->counter_words = 8 * kKinds + 2 * count;
     bt->counters = cv.take<unsigned>(size_t(bt->counter_words));
     // tables (filled below on the second pass, uploaded from one host image)
     const size_t tables_begin = (cv.off + 255) & ~size_t(255);
     bt->hash = cv.take<cm::HashArgs>(size_t(count));
     bt->finish = cv.take<cm::FinishJob>(size_t(count));
     bt->dec = cv.take<cm::DecompressArgs>(size_t(count));
-    for (int fw = 0; fw < 2; ++fw) bt->jobs[fw] = cv.take<cm::RegJob>(size_t(njobs[fw]));
+    for (int k = 0; k < kKinds; ++k) bt->jobs[k] = cv.take<cm::RegJob>(size_t(njobs[k]));
 
     // per-product tables and block maps
     hash_blocks.clear();
     sum_blocks.clear();
     red_blocks.clear();
     comb_blocks.clear();
-    for (int fw = 0; fw < 2; ++fw) list_blocks[fw].clear();
+    for (int k = 0; k < kKinds; ++k) list_blocks[k].clear();
     for (int i = 0; i < count; ++i) {
       const cm_product& q = ps[i];
       const int fw = q.variant == CM_FWHT;
@@ -337,7 +367,7 @@ int build_plan(int count, const cm_product* ps, unsigned flags, int grain, const
       const int hb = cm::hash_batch_blocks(ha);
       for (int g = 0; g < hb; ++g) hash_blocks.push_back(make_int2(i, g));
 
-      cm::RegJob& j = jobs[fw][hp.jobidx[i]];
+      cm::RegJob& j = jobs[hp.kind[i]][hp.jobidx[i]];
       j = cm::RegJob{};
       j.AT = q.AT;
       j.ld_at = q.ld_at;
@@ -358,7 +388,7 @@ int build_plan(int count, const cm_product* ps, unsigned flags, int grain, const
       j.t2 = t2;
       j.partial = static_cast<float*>(partial[i]);
       const int nl = cm::reg_lists_count(fw, q.d, lb);
-      for (int g = 0; g < nl; ++g) list_blocks[fw].push_back(make_int2(hp.jobidx[i], g));
+      for (int g = 0; g < nl; ++g) list_blocks[hp.kind[i]].push_back(make_int2(hp.jobidx[i], g));
 
       cm::FinishJob& f = fin[i];
       const int f64r = q.n > 16384;
@@ -383,7 +413,7 @@ int build_plan(int count, const cm_product* ps, unsigned flags, int grain, const
       da.b = q.b;
       da.C = q.C;
       da.ldc = q.ldc;
-      P.counter = reinterpret_cast<unsigned long long*>(bt->counters + 16 + 2 * i);
+      P.counter = reinterpret_cast<unsigned long long*>(bt->counters + 8 * kKinds + 2 * i);
       if (!std::isnan(q.tau)) {
         da.recover = 1;
         da.tau = q.tau;
@@ -396,9 +426,9 @@ int build_plan(int count, const cm_product* ps, unsigned flags, int grain, const
     }
     bt->hash_blocks = cv.take<int2>(hash_blocks.size());
     bt->n_hash_blocks = int(hash_blocks.size());
-    for (int fw = 0; fw < 2; ++fw) {
-      bt->list_blocks[fw] = cv.take<int2>(list_blocks[fw].size());
-      bt->n_list_blocks[fw] = int(list_blocks[fw].size());
+    for (int k = 0; k < kKinds; ++k) {
+      bt->list_blocks[k] = cv.take<int2>(list_blocks[k].size());
+      bt->n_list_blocks[k] = int(list_blocks[k].size());
     }
     bt->sum_blocks = cv.take<int2>(sum_blocks.size());
     bt->red_blocks = cv.take<int2>(red_blocks.size());
@@ -411,17 +441,19 @@ int build_plan(int count, const cm_product* ps, unsigned flags, int grain, const
       bt->red_lds = std::max(bt->red_lds, size_t(ps[i].variant == CM_FWHT ? 1 : 2) * sizeof(float) << cm::kRegLb2);
 
     // compress work items: chunks to the least-loaded of eight queues
-    std::vector<int2> items_host[2];
-    std::vector<int> qstart_host[2];
-    for (int fw = 0; fw < 2; ++fw) {
+    std::vector<int2> items_host[kKinds];
+    std::vector<int> qstart_host[kKinds];
+    for (int kd = 0; kd < kKinds; ++kd) {
       struct Chunk {
         int job, chunk, units;
         double unit_cost;
       };
       std::vector<Chunk> chunks;
       for (int i = 0; i < count; ++i) {
-        if ((ps[i].variant == CM_FWHT) != bool(fw)) continue;
-        const int per_chunk = ps[i].d << cm::reg_lbs(ilog2(ps[i].b));
+        if (hp.kind[i] != kd) continue;
+        // workgroups per chunk of k: one per (t, slice), or per group of
+        // 2^(kd % 3) slices
+        const int per_chunk = (ps[i].d << cm::reg_lbs(ilog2(ps[i].b))) >> (kd % 3);
         for (int c = 0; c < hp.nchunks[i]; ++c) {
           const int k = std::min(hp.kper[i], ps[i].n - c * hp.kper[i]);
           chunks.push_back({hp.jobidx[i], c, per_chunk, k * step_cost(ps[i])});
@@ -436,16 +468,16 @@ int build_plan(int count, const cm_product* ps, unsigned flags, int grain, const
         load[x] += c.units * c.unit_cost;
         for (int u = 0; u < c.units; ++u) q[x].push_back(make_int2(c.job, c.chunk * c.units + u));
       }
-      qstart_host[fw].assign(9, 0);
+      qstart_host[kd].assign(9, 0);
       for (int x = 0; x < 8; ++x) {
-        qstart_host[fw][x + 1] = qstart_host[fw][x] + int(q[x].size());
-        items_host[fw].insert(items_host[fw].end(), q[x].begin(), q[x].end());
+        qstart_host[kd][x + 1] = qstart_host[kd][x] + int(q[x].size());
+        items_host[kd].insert(items_host[kd].end(), q[x].begin(), q[x].end());
       }
-      bt->items[fw] = cv.take<int2>(items_host[fw].size());
-      bt->qstart[fw] = cv.take<int>(9);
-      bt->n_items[fw] = int(items_host[fw].size());
-      const int slots = fw ? cm::fwht_reg_slots() : cm::fft_reg_slots();
-      bt->grid[fw] = items_host[fw].empty() ? 0 : std::max(8, std::min(slots, (bt->n_items[fw] + 7) / 8 * 8) / 8 * 8);
+      bt->items[kd] = cv.take<int2>(items_host[kd].size());
+      bt->qstart[kd] = cv.take<int>(9);
+      bt->n_items[kd] = int(items_host[kd].size());
+      const int slots = items_host[kd].empty() ? 8 : kind_slots(kd);
+      bt->grid[kd] = items_host[kd].empty() ? 0 : std::max(8, std::min(slots, (bt->n_items[kd] + 7) / 8 * 8) / 8 * 8);
     }
 
     // decompress classes
@@ -507,11 +539,11 @@ int build_plan(int count, const cm_product* ps, unsigned flags, int grain, const
     put(bt->hash, hash.data(), hash.size() * sizeof(cm::HashArgs));
     put(bt->finish, fin.data(), fin.size() * sizeof(cm::FinishJob));
     put(bt->dec, dec.data(), dec.size() * sizeof(cm::DecompressArgs));
-    for (int fw = 0; fw < 2; ++fw) {
-      put(bt->jobs[fw], jobs[fw].data(), jobs[fw].size() * sizeof(cm::RegJob));
-      put(bt->list_blocks[fw], list_blocks[fw].data(), list_blocks[fw].size() * sizeof(int2));
-      put(bt->items[fw], items_host[fw].data(), items_host[fw].size() * sizeof(int2));
-      put(bt->qstart[fw], qstart_host[fw].data(), 9 * sizeof(int));
+    for (int k = 0; k < kKinds; ++k) {
+      put(bt->jobs[k], jobs[k].data(), jobs[k].size() * sizeof(cm::RegJob));
+      put(bt->list_blocks[k], list_blocks[k].data(), list_blocks[k].size() * sizeof(int2));
+      put(bt->items[k], items_host[k].data(), items_host[k].size() * sizeof(int2));
+      put(bt->qstart[k], qstart_host[k].data(), 9 * sizeof(int));
     }
     put(bt->hash_blocks, hash_blocks.data(), hash_blocks.size() * sizeof(int2));
     put(bt->sum_blocks, sum_blocks.data(), sum_blocks.size() * sizeof(int2));
@@ -533,7 +565,7 @@ int build_plan(int count, const cm_product* ps, unsigned flags, int grain, const
     }
     if (e == hipSuccess) e = hipStreamSynchronize(bt->stream);
     if (e != hipSuccess) {
-      if (bt->owns_arena) hipFree(bt->arena);
+      if (bt->owns_arena) (void)hipFree(bt->arena);
       delete bt;
       return error(CM_EDEVICE, std::string("batch table upload: ") + hipGetErrorString(e));
     }
@@ -559,21 +591,33 @@ int enqueue(cm_batch* bt, bool untimed) {
     return rc;
   if ((rc = run("zero", [&] { return cm::launch_zero_u32(bt->counters, bt->counter_words, s); })))
     return rc;
-  if ((rc = run("lists", [&] {
-         hipError_t e = cm::launch_fwht_reg_lists_batch(bt->jobs[1], bt->list_blocks[1], bt->n_list_blocks[1], s);
-         if (e == hipSuccess)
-           e = cm::launch_fft_reg_lists_batch(bt->jobs[0], bt->list_blocks[0], bt->n_list_blocks[0], s);
-         return e;
-       })))
-    return rc;
-  if ((rc = run("compress_fwht", [&] {
-         return cm::launch_fwht_reg_batch(bt->jobs[1], bt->items[1], bt->qstart[1], bt->counters + 8, bt->grid[1], s);
-       })))
-    return rc;
-  if ((rc = run("compress_fft", [&] {
-         return cm::launch_fft_reg_batch(bt->jobs[0], bt->items[0], bt->qstart[0], bt->counters, bt->grid[0], s);
-       })))
-    return rc;
+  for (int k = 0; k < kKinds; ++k) {
+    if (!bt->n_items[k] && !bt->n_list_blocks[k]) continue;
+    const int logg = k % 3;
+    const bool fw = k >= 3;
+    if ((rc = run("lists", [&] {
+           if (logg == 0)
+             return fw ? cm::launch_fwht_reg_lists_batch(bt->jobs[k], bt->list_blocks[k], bt->n_list_blocks[k], s)
+                       : cm::launch_fft_reg_lists_batch(bt->jobs[k], bt->list_blocks[k], bt->n_list_blocks[k], s);
+           return fw ? cm::launch_fwht_grp_lists_batch(logg, bt->jobs[k], bt->list_blocks[k], bt->n_list_blocks[k], s)
+                     : cm::launch_fft_grp_lists_batch(logg, bt->jobs[k], bt->list_blocks[k], bt->n_list_blocks[k], s);
+         })))
+      return rc;
+  }
+  for (int k = 0; k < kKinds; ++k) {
+    if (!bt->n_items[k]) continue;
+    const int logg = k % 3;
+    const bool fw = k >= 3;
+    unsigned* ctr = bt->counters + 8 * k;
+    if ((rc = run(fw ? "compress_fwht" : "compress_fft", [&] {
+           if (logg == 0)
+             return fw ? cm::launch_fwht_reg_batch(bt->jobs[k], bt->items[k], bt->qstart[k], ctr, bt->grid[k], s)
+                       : cm::launch_fft_reg_batch(bt->jobs[k], bt->items[k], bt->qstart[k], ctr, bt->grid[k], s);
+           return fw ? cm::launch_fwht_grp_batch(logg, bt->jobs[k], bt->items[k], bt->qstart[k], ctr, bt->grid[k], s)
+                     : cm::launch_fft_grp_batch(logg, bt->jobs[k], bt->items[k], bt->qstart[k], ctr, bt->grid[k], s);
+         })))
+      return rc;
+  }
   if ((rc = run("finish", [&] {
          return cm::launch_finish_batch(bt->finish, bt->sum_blocks, bt->n_sum, bt->red_blocks, bt->n_red,
                                         bt->red_lds, bt->comb_blocks, bt->n_comb, s);
@@ -597,7 +641,7 @@ int run_locked(cm_batch* bt) {
     hipGraph_t graph = nullptr;
     e = hipStreamEndCapture(s, &graph);
     if (rc) {
-      if (graph) hipGraphDestroy(graph);
+      if (graph) (void)hipGraphDestroy(graph);
       return rc;
     }
     if (e != hipSuccess) return error(CM_EDEVICE, std::string("end capture: ") + hipGetErrorString(e));
@@ -611,10 +655,10 @@ int run_locked(cm_batch* bt) {
 
 void free_locked(cm_batch* bt) {
   if (!bt) return;
-  hipStreamSynchronize(bt->stream);
-  if (bt->exec) hipGraphExecDestroy(bt->exec);
-  if (bt->graph) hipGraphDestroy(bt->graph);
-  if (bt->arena && bt->owns_arena) hipFree(bt->arena);
+  (void)hipStreamSynchronize(bt->stream);
+  if (bt->exec) (void)hipGraphExecDestroy(bt->exec);
+  if (bt->graph) (void)hipGraphDestroy(bt->graph);
+  if (bt->arena && bt->owns_arena) (void)hipFree(bt->arena);
   delete bt;
 }
 
@@ -633,12 +677,12 @@ int tune(int count, const cm_product* ps, unsigned flags, int* best) {
     if ((rc = build_plan(count, ps, flags & ~CM_BATCH_GRAPH, grain, cm::PlanEnv{}, &bt))) break;
     double ms_min = 1e300;
     for (int r = 0; r < 3 && rc == CM_OK; ++r) {
-      hipEventRecord(e0, s);
+      (void)hipEventRecord(e0, s);
       rc = enqueue(bt, true);
-      hipEventRecord(e1, s);
+      (void)hipEventRecord(e1, s);
       if (hipEventSynchronize(e1) != hipSuccess) rc = error(CM_EDEVICE, "autotune run failed");
       float ms = 0.f;
-      hipEventElapsedTime(&ms, e0, e1);
+      (void)hipEventElapsedTime(&ms, e0, e1);
       if (r > 0) ms_min = std::min(ms_min, double(ms));
     }
     free_locked(bt);
@@ -648,8 +692,8 @@ int tune(int count, const cm_product* ps, unsigned flags, int* best) {
       *best = grain;
     }
   }
-  hipEventDestroy(e0);
-  hipEventDestroy(e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
   return rc;
 }
 
@@ -672,7 +716,7 @@ ProductView batch_product(const cm_batch* bt, int i) {
 
 void batch_destroy(cm_batch* bt) {
   if (!bt) return;
-  if (bt->arena && bt->owns_arena) hipFree(bt->arena);
+  if (bt->arena && bt->owns_arena) (void)hipFree(bt->arena);
   delete bt;
 }
 

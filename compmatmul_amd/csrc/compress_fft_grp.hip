@@ -63,13 +63,12 @@ __host__ __device__ __forceinline__ Layout layout(const RegJob& j) {
 // coefficient s W_b^(h s0) (times i for B).  Entry: l << 3 (byte offset of
 // a complex, or a dummy slot 2048 + lane) | key offset in the segment << 18.
 template <int LOG_G>
-__global__ void __launch_bounds__(64) k_fft_grp_lists(RegJob j) {
+__device__ __forceinline__ void fft_grp_lists_body(const RegJob& j, int g) {
   constexpr int G = 1 << LOG_G;
   __shared__ uint32_t cnt[B2];
   __shared__ uint8_t rnd[SEG];
   __shared__ int nr[RCAP];
   const Layout lay = layout(j);
-  const int g = blockIdx.x;
   const int b1 = 1 << j.lbs;
   const int t = g >> j.lbs;
   const int rest = g & (b1 - 1);
@@ -153,6 +152,16 @@ __global__ void __launch_bounds__(64) k_fft_grp_lists(RegJob j) {
     out = acc;
     __syncthreads();
   }
+}
+
+template <int LOG_G>
+__global__ void __launch_bounds__(64) k_fft_grp_lists(RegJob j) { fft_grp_lists_body<LOG_G>(j, blockIdx.x); }
+
+template <int LOG_G>
+__global__ void __launch_bounds__(64) k_fft_grp_lists_batch(const RegJob* __restrict__ jobs,
+                                                          const int2* __restrict__ blocks) {
+  const int2 e = blocks[blockIdx.x];
+  fft_grp_lists_body<LOG_G>(jobs[e.x], e.y);
 }
 
 // KIND 0 plain writes, 1 read-add-writes, 2 LDS atomics (as in the
@@ -337,6 +346,34 @@ k_compress_fft_grp(RegJob j, int units) {
   fft_grp_unit<LOG_G>(j, u, S, Rb);
 }
 
+// Persistent form for many products: the XCD work queues of the single-wave
+// batch kernels; thread 0 claims each item and the workgroup reads it from
+// LDS.
+template <int LOG_G>
+__global__ void __launch_bounds__(64 << LOG_G) __attribute__((amdgpu_waves_per_eu(2, 2)))
+k_compress_fft_grp_batch(const RegJob* __restrict__ jobs, const int2* __restrict__ items,
+                  const int* __restrict__ qstart, unsigned* __restrict__ counters) {
+  __shared__ __align__(16) float2 S[SLEN << LOG_G];
+  __shared__ __align__(16) float Rb[SEG];
+  __shared__ int item;
+  const int home = blockIdx.x & 7;
+  for (int qi = 0; qi < 8; ++qi) {
+    const int q = (home + qi) & 7;
+    const int base = qstart[q];
+    const int len = qstart[q + 1] - base;
+    for (;;) {
+      if (threadIdx.x == 0) item = int(atomicAdd(&counters[q], 1u));
+      __syncthreads();
+      const int it = item;
+      __syncthreads();  // every thread has read `item` before it is rewritten
+      if (it >= len) break;
+      const int2 e = items[base + it];
+      fft_grp_unit<LOG_G>(jobs[e.x], e.y, S, Rb);
+      __syncthreads();
+    }
+  }
+}
+
 template <int LOG_G>
 int slots_t() {
   int dev = 0, cus = 0, per_cu = 0;
@@ -377,6 +414,26 @@ size_t fft_grp_lists_bytes(int d, int lb, int m, int p) {
 hipError_t launch_fft_grp(const RegJob& j, hipStream_t s) {
   if (j.nchunks == 0) return hipSuccess;
   return fft_grp_log_g(j.lb) == 2 ? launch_t<2>(j, s) : launch_t<1>(j, s);
+}
+
+hipError_t launch_fft_grp_lists_batch(int log_g, const RegJob* jobs, const int2* blocks, int nblocks,
+                                          hipStream_t s) {
+  if (nblocks <= 0) return hipSuccess;
+  if (log_g == 2)
+    k_fft_grp_lists_batch<2><<<dim3(unsigned(nblocks)), dim3(64), 0, s>>>(jobs, blocks);
+  else
+    k_fft_grp_lists_batch<1><<<dim3(unsigned(nblocks)), dim3(64), 0, s>>>(jobs, blocks);
+  return hipGetLastError();
+}
+
+hipError_t launch_fft_grp_batch(int log_g, const RegJob* jobs, const int2* items, const int* qstart,
+                                   unsigned* counters, int grid, hipStream_t s) {
+  if (grid <= 0) return hipSuccess;
+  if (log_g == 2)
+    k_compress_fft_grp_batch<2><<<dim3(unsigned(grid)), dim3(256), 0, s>>>(jobs, items, qstart, counters);
+  else
+    k_compress_fft_grp_batch<1><<<dim3(unsigned(grid)), dim3(128), 0, s>>>(jobs, items, qstart, counters);
+  return hipGetLastError();
 }
 
 }  // namespace cm

@@ -70,13 +70,12 @@ __host__ __device__ __forceinline__ Layout layout(const RegJob& j) {
 // by the group bits.  Entry: l << 2 (or a dummy slot 2048 + lane) | key
 // offset in the 2048-key segment << 18 | sign << 31.
 template <int LOG_G>
-__global__ void __launch_bounds__(64) k_grp_lists(RegJob j) {
+__device__ __forceinline__ void grp_lists_body(const RegJob& j, int g) {
   constexpr int G = 1 << LOG_G;
   __shared__ uint32_t cnt[B2];
   __shared__ uint8_t rnd[SEG];
   __shared__ int nr[RCAP];
   const Layout lay = layout(j);
-  const int g = blockIdx.x;
   const bool side_b = g >= lay.lists;
   const int ts = side_b ? g - lay.lists : g;
   const int t = ts >> j.lbs;
@@ -146,6 +145,16 @@ __global__ void __launch_bounds__(64) k_grp_lists(RegJob j) {
     out = acc;
     __syncthreads();
   }
+}
+
+template <int LOG_G>
+__global__ void __launch_bounds__(64) k_grp_lists(RegJob j) { grp_lists_body<LOG_G>(j, blockIdx.x); }
+
+template <int LOG_G>
+__global__ void __launch_bounds__(64) k_grp_lists_batch(const RegJob* __restrict__ jobs,
+                                                          const int2* __restrict__ blocks) {
+  const int2 e = blocks[blockIdx.x];
+  grp_lists_body<LOG_G>(jobs[e.x], e.y);
 }
 
 __device__ __forceinline__ void wht_reg(float (&x)[R]) {
@@ -339,6 +348,34 @@ __global__ void __launch_bounds__(64 << LOG_G) k_compress_fwht_grp(RegJob j, int
   grp_unit<LOG_G>(j, u, S, Rb);
 }
 
+// Persistent form for many products: the XCD work queues of the single-wave
+// batch kernels; thread 0 claims each item and the workgroup reads it from
+// LDS.
+template <int LOG_G>
+__global__ void __launch_bounds__(64 << LOG_G)
+k_compress_fwht_grp_batch(const RegJob* __restrict__ jobs, const int2* __restrict__ items,
+                  const int* __restrict__ qstart, unsigned* __restrict__ counters) {
+  __shared__ __align__(16) float S[SLEN << LOG_G];
+  __shared__ __align__(16) float Rb[SEG];
+  __shared__ int item;
+  const int home = blockIdx.x & 7;
+  for (int qi = 0; qi < 8; ++qi) {
+    const int q = (home + qi) & 7;
+    const int base = qstart[q];
+    const int len = qstart[q + 1] - base;
+    for (;;) {
+      if (threadIdx.x == 0) item = int(atomicAdd(&counters[q], 1u));
+      __syncthreads();
+      const int it = item;
+      __syncthreads();  // every thread has read `item` before it is rewritten
+      if (it >= len) break;
+      const int2 e = items[base + it];
+      grp_unit<LOG_G>(jobs[e.x], e.y, S, Rb);
+      __syncthreads();
+    }
+  }
+}
+
 template <int LOG_G>
 int slots_t() {
   int dev = 0, cus = 0, per_cu = 0;
@@ -379,6 +416,26 @@ size_t fwht_grp_lists_bytes(int d, int lb, int m, int p) {
 hipError_t launch_fwht_grp(const RegJob& j, hipStream_t s) {
   if (j.nchunks == 0) return hipSuccess;
   return fwht_grp_log_g(j.lb) == 2 ? launch_t<2>(j, s) : launch_t<1>(j, s);
+}
+
+hipError_t launch_fwht_grp_lists_batch(int log_g, const RegJob* jobs, const int2* blocks, int nblocks,
+                                          hipStream_t s) {
+  if (nblocks <= 0) return hipSuccess;
+  if (log_g == 2)
+    k_grp_lists_batch<2><<<dim3(unsigned(nblocks)), dim3(64), 0, s>>>(jobs, blocks);
+  else
+    k_grp_lists_batch<1><<<dim3(unsigned(nblocks)), dim3(64), 0, s>>>(jobs, blocks);
+  return hipGetLastError();
+}
+
+hipError_t launch_fwht_grp_batch(int log_g, const RegJob* jobs, const int2* items, const int* qstart,
+                                   unsigned* counters, int grid, hipStream_t s) {
+  if (grid <= 0) return hipSuccess;
+  if (log_g == 2)
+    k_compress_fwht_grp_batch<2><<<dim3(unsigned(grid)), dim3(256), 0, s>>>(jobs, items, qstart, counters);
+  else
+    k_compress_fwht_grp_batch<1><<<dim3(unsigned(grid)), dim3(128), 0, s>>>(jobs, items, qstart, counters);
+  return hipGetLastError();
 }
 
 }  // namespace cm

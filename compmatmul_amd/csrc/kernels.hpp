@@ -95,6 +95,16 @@ int fft_grp_log_g(int lb);
 int fft_grp_slots(int lb);
 size_t fft_grp_lists_bytes(int d, int lb, int m, int p);
 hipError_t launch_fft_grp(const RegJob& j, hipStream_t s);
+// Many products on the slice-group kernels (workgroups of 2^log_g waves):
+// schedules and the persistent compress launch, as for the single-wave ones.
+hipError_t launch_fwht_grp_lists_batch(int log_g, const RegJob* jobs, const int2* blocks, int nblocks,
+                                       hipStream_t s);
+hipError_t launch_fwht_grp_batch(int log_g, const RegJob* jobs, const int2* items, const int* qstart,
+                                 unsigned* counters, int grid, hipStream_t s);
+hipError_t launch_fft_grp_lists_batch(int log_g, const RegJob* jobs, const int2* blocks, int nblocks,
+                                      hipStream_t s);
+hipError_t launch_fft_grp_batch(int log_g, const RegJob* jobs, const int2* items, const int* qstart,
+                                unsigned* counters, int grid, hipStream_t s);
 
 // Many products.  Schedules: `blocks[g]` = (job, list) of workgroup g.
 // Compress: one persistent launch drains eight work queues, queue x holding
