@@ -203,11 +203,13 @@ __device__ __forceinline__ void scatter_section(const uint32_t* __restrict__ lp,
 // Scatter one row for the G slices of the group, cross-partition WHT, then
 // the register WHT of this wave's slice.  On return lane L, register r holds
 // frequency f_of(L, r) of slice group*G + w.
+// With `staged`, the row (one segment) is already in Rb.
 template <int LOG_G>
 __device__ __forceinline__ void sketch_transform_grp(const float* __restrict__ row, int keys,
                                                      const uint32_t* __restrict__ list,
                                                      const int* __restrict__ segcnt, float* S,
-                                                     float* Rb, int w, int lane, float (&x)[R]) {
+                                                     float* Rb, bool staged, int w, int lane,
+                                                     float (&x)[R]) {
   constexpr int G = 1 << LOG_G;
   float* Sw = S + w * SLEN;
   const int tid = w * 64 + lane;
@@ -219,15 +221,17 @@ __device__ __forceinline__ void sketch_transform_grp(const float* __restrict__ r
   const int nseg = (keys + SEG - 1) / SEG;
   const uint32_t* __restrict__ lp = list;
   for (int seg = 0; seg < nseg; ++seg) {
-    const int s0 = seg * SEG;
-    const int sl = min(SEG, keys - s0);
-    float v[SEG / (64 * G)];
+    if (!staged) {
+      const int s0 = seg * SEG;
+      const int sl = min(SEG, keys - s0);
+      float v[SEG / (64 * G)];
 #pragma unroll
-    for (int q = 0; q < SEG / (64 * G); ++q) v[q] = row[s0 + min(q * 64 * G + tid, sl - 1)];
-    __syncthreads();  // every wave is done with the previous segment
+      for (int q = 0; q < SEG / (64 * G); ++q) v[q] = row[s0 + min(q * 64 * G + tid, sl - 1)];
+      __syncthreads();  // every wave is done with the previous segment
 #pragma unroll
-    for (int q = 0; q < SEG / (64 * G); ++q) Rb[q * 64 * G + tid] = v[q];
-    __syncthreads();
+      for (int q = 0; q < SEG / (64 * G); ++q) Rb[q * 64 * G + tid] = v[q];
+      __syncthreads();
+    }
     const int* __restrict__ nr = segcnt + seg * RCAP;
     int nw = 0;
     for (int r = 1; r < RCAP - 1; ++r) nw += nr[r];
@@ -301,8 +305,9 @@ __device__ __forceinline__ int f_of(int L, int r) {
   return ((L & 31) << 6) | ((r & 1) << 5) | ((r & ~1) + (L >> 5));
 }
 
-// unit u = (chunk, t, group); wave w owns slice group * G + w
-template <int LOG_G>
+// unit u = (chunk, t, group); wave w owns slice group * G + w.  PAIRED: both
+// rows fit one segment (m, p <= SEG) and are staged together.
+template <int LOG_G, bool PAIRED>
 __device__ __forceinline__ void grp_unit(const RegJob& j, int u, float* S, float* Rb) {
   constexpr int G = 1 << LOG_G;
   const Layout lay = layout(j);
@@ -320,10 +325,31 @@ __device__ __forceinline__ void grp_unit(const RegJob& j, int u, float* S, float
   float acc[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) acc[r] = 0.f;
+  // rows of at most one segment each: stage A's and B's row together, so
+  // the load latency is exposed once per k
+  constexpr bool paired = PAIRED;
   for (int k = k0; k < k1; ++k) {
     float xa[R], xb[R];
-    sketch_transform_grp<LOG_G>(j.AT + int64_t(k) * j.ld_at, j.m, la, sca, S, Rb, w, lane, xa);
-    sketch_transform_grp<LOG_G>(j.B + int64_t(k) * j.ld_b, j.p, lb, scb, S, Rb, w, lane, xb);
+    const float* arow = j.AT + int64_t(k) * j.ld_at;
+    const float* brow = j.B + int64_t(k) * j.ld_b;
+    if constexpr (paired) {
+      const int tid = threadIdx.x;
+      float va[SEG / (64 * G)], vb[SEG / (64 * G)];
+#pragma unroll
+      for (int q = 0; q < SEG / (64 * G); ++q) {
+        va[q] = j.m > 0 ? arow[min(q * 64 * G + tid, j.m - 1)] : 0.f;
+        vb[q] = j.p > 0 ? brow[min(q * 64 * G + tid, j.p - 1)] : 0.f;
+      }
+      __syncthreads();  // every wave is done with the previous rows
+#pragma unroll
+      for (int q = 0; q < SEG / (64 * G); ++q) {
+        Rb[q * 64 * G + tid] = va[q];
+        Rb[SEG + q * 64 * G + tid] = vb[q];
+      }
+      __syncthreads();
+    }
+    sketch_transform_grp<LOG_G>(arow, j.m, la, sca, S, Rb, paired, w, lane, xa);
+    sketch_transform_grp<LOG_G>(brow, j.p, lb, scb, S, paired ? Rb + SEG : Rb, paired, w, lane, xb);
 #pragma unroll
     for (int r = 0; r < R; ++r) acc[r] = fmaf(xa[r], xb[r], acc[r]);
   }
@@ -337,15 +363,18 @@ __device__ __forceinline__ void grp_unit(const RegJob& j, int u, float* S, float
   for (int r = 0; r < R; ++r) out[r * 64 + lane] = Sw[r * 64 + lane];
 }
 
-template <int LOG_G>
-__global__ void __launch_bounds__(64 << LOG_G) k_compress_fwht_grp(RegJob j, int units) {
+// Three waves per SIMD: the paired form then spills a few registers, and
+// is still faster than at two waves (measured on fwht 2048^3, b = 2^14).
+template <int LOG_G, bool PAIRED>
+__global__ void __launch_bounds__(64 << LOG_G) __attribute__((amdgpu_waves_per_eu(3, 3)))
+k_compress_fwht_grp(RegJob j, int units) {
   __shared__ __align__(16) float S[SLEN << LOG_G];
-  __shared__ __align__(16) float Rb[SEG];
+  __shared__ __align__(16) float Rb[2 * SEG];
   const int G = gridDim.x;
   const int g = blockIdx.x;
   const int u = (g & 7) * (G >> 3) + (g >> 3);  // XCD-aware numbering
   if (u >= units) return;
-  grp_unit<LOG_G>(j, u, S, Rb);
+  grp_unit<LOG_G, PAIRED>(j, u, S, Rb);
 }
 
 // Persistent form for many products: the XCD work queues of the single-wave
@@ -356,7 +385,7 @@ __global__ void __launch_bounds__(64 << LOG_G)
 k_compress_fwht_grp_batch(const RegJob* __restrict__ jobs, const int2* __restrict__ items,
                   const int* __restrict__ qstart, unsigned* __restrict__ counters) {
   __shared__ __align__(16) float S[SLEN << LOG_G];
-  __shared__ __align__(16) float Rb[SEG];
+  __shared__ __align__(16) float Rb[2 * SEG];
   __shared__ int item;
   const int home = blockIdx.x & 7;
   for (int qi = 0; qi < 8; ++qi) {
@@ -370,7 +399,7 @@ k_compress_fwht_grp_batch(const RegJob* __restrict__ jobs, const int2* __restric
       __syncthreads();  // every thread has read `item` before it is rewritten
       if (it >= len) break;
       const int2 e = items[base + it];
-      grp_unit<LOG_G>(jobs[e.x], e.y, S, Rb);
+      grp_unit<LOG_G, false>(jobs[e.x], e.y, S, Rb);
       __syncthreads();
     }
   }
@@ -381,7 +410,7 @@ int slots_t() {
   int dev = 0, cus = 0, per_cu = 0;
   if (hipGetDevice(&dev) != hipSuccess ||
       hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_compress_fwht_grp<LOG_G>, 64 << LOG_G, 0) !=
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_compress_fwht_grp<LOG_G, true>, 64 << LOG_G, 0) !=
           hipSuccess ||
       cus <= 0 || per_cu <= 0)
     return 512;
@@ -394,7 +423,10 @@ hipError_t launch_t(const RegJob& j, hipStream_t s) {
   k_grp_lists<LOG_G><<<dim3(unsigned(2 * lists)), dim3(64), 0, s>>>(j);
   const int units = j.nchunks * (lists >> LOG_G);
   const int grid = (units + 7) / 8 * 8;
-  k_compress_fwht_grp<LOG_G><<<dim3(unsigned(grid)), dim3(64 << LOG_G), 0, s>>>(j, units);
+  if (j.m <= SEG && j.p <= SEG)
+    k_compress_fwht_grp<LOG_G, true><<<dim3(unsigned(grid)), dim3(64 << LOG_G), 0, s>>>(j, units);
+  else
+    k_compress_fwht_grp<LOG_G, false><<<dim3(unsigned(grid)), dim3(64 << LOG_G), 0, s>>>(j, units);
   return hipGetLastError();
 }
 

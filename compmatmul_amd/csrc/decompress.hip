@@ -91,14 +91,31 @@ __global__ void __launch_bounds__(NT) k_decompress(DecompressArgs a) {
 }
 
 // LDS-staged variant for sketch rows of at most 64 KB: the workgroup (512
-// threads, one column each, up to 12 rows per thread) streams the d rows of the
-// sketch through two LDS buffers (row t+1 is fetched while row t is
-// gathered), so the random gathers hit LDS instead of 128-byte L2 lines.
+// threads, one column each, up to 12 rows per thread) streams the d rows of
+// the sketch through two LDS buffers, so the random gathers hit LDS instead
+// of 128-byte L2 lines.  Rows are copied with global_load_lds (straight into
+// LDS, 1 KB per wave instruction, no registers): row t+1 is in flight while
+// row t is gathered, and the registers go to rows of C instead.
 constexpr int LNT = 512;
-// rows per thread: as many as fit ~112 VGPRs of gathered values
+// rows per thread: about 120 values per thread (more spills for the larger median networks) of gathered values
 template <typename T, int D>
 constexpr int rows_per_thread() {
-  return (sizeof(T) == 8 ? 56 : 112) / D < 1 ? 1 : ((sizeof(T) == 8 ? 56 : 112) / D > 12 ? 12 : (sizeof(T) == 8 ? 56 : 112) / D);
+  return (sizeof(T) == 8 ? 60 : 120) / D < 1 ? 1 : ((sizeof(T) == 8 ? 60 : 120) / D > 12 ? 12 : (sizeof(T) == 8 ? 60 : 120) / D);
+}
+
+// start copying sketch row t into LDS buffer t & 1
+template <typename T>
+__device__ __forceinline__ void stage_row_async(const T* __restrict__ P, T* buf0, int b, int t,
+                                                int tid) {
+  const int vec = b * int(sizeof(T)) / 16;  // float4 per row
+  const int wave = tid >> 6, lane = tid & 63;
+  const float4* src = reinterpret_cast<const float4*>(P + int64_t(t) * b);
+  float4* dst = reinterpret_cast<float4*>(buf0 + (t & 1) * b);
+  for (int q0 = wave * 64; q0 < vec; q0 += LNT) {
+    if (q0 + lane < vec)  // lanes past the row write nothing
+      __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)(src + q0 + lane),
+                                       (__attribute__((address_space(3))) void*)(dst + q0), 16, 0, 0);
+  }
 }
 
 template <typename T, bool FWHT, int D>
@@ -114,26 +131,16 @@ __device__ __forceinline__ void decompress_lds_body(const DecompressArgs& a, int
   const bool jok = j < a.p;
   const uint32_t mask = uint32_t(b - 1);
   const T* __restrict__ P = static_cast<const T*>(a.P);
-  const int vec = b * int(sizeof(T)) / 16;  // float4 per row
-  constexpr int MAXV = 65536 / 16 / LNT;    // float4 per thread per row (<= 8)
   uint32_t eb[D];
 #pragma unroll
   for (int t = 0; t < D; ++t) eb[t] = jok ? a.hB[int64_t(t) * a.p + j] : 0u;
-  {
-    const float4* src = reinterpret_cast<const float4*>(P);
-    float4* dst = reinterpret_cast<float4*>(buf0);
-    for (int q = tid; q < vec; q += LNT) dst[q] = src[q];
-  }
+  stage_row_async<T>(P, buf0, b, 0, tid);
   T v[ER][D];
 #pragma unroll
   for (int t = 0; t < D; ++t) {
-    __syncthreads();
-    float4 nxt[MAXV];
-    if (t + 1 < D) {
-      const float4* src = reinterpret_cast<const float4*>(P + int64_t(t + 1) * b);
-#pragma unroll
-      for (int q = 0; q < MAXV; ++q) nxt[q] = src[min(q * LNT + tid, vec - 1)];
-    }
+    __builtin_amdgcn_s_waitcnt(0);  // this thread's copies of row t are done
+    __syncthreads();                // everyone's, and row t-1 is no longer read
+    if (t + 1 < D) stage_row_async<T>(P, buf0, b, t + 1, tid);
     const T* cur = buf0 + (t & 1) * b;
 #pragma unroll
     for (int e = 0; e < ER; ++e) {
@@ -142,12 +149,6 @@ __device__ __forceinline__ void decompress_lds_body(const DecompressArgs& a, int
       const uint32_t h = FWHT ? (hbucket(ea) ^ hbucket(eb[t])) : ((hbucket(ea) + hbucket(eb[t])) & mask);
       const T val = cur[h];
       v[e][t] = (hneg(ea) ^ hneg(eb[t])) ? -val : val;
-    }
-    if (t + 1 < D) {
-      float4* dst = reinterpret_cast<float4*>(buf0 + ((t + 1) & 1) * b);
-#pragma unroll
-      for (int q = 0; q < MAXV; ++q)
-        if (q * LNT + tid < vec) dst[q * LNT + tid] = nxt[q];
     }
   }
   if (!jok) return;
