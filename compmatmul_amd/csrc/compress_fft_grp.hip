@@ -65,8 +65,8 @@ __host__ __device__ __forceinline__ Layout layout(const RegJob& j) {
 template <int LOG_G>
 __device__ __forceinline__ void fft_grp_lists_body(const RegJob& j, int g) {
   constexpr int G = 1 << LOG_G;
+  constexpr int Q = SEG / 64;  // keys per lane and segment
   __shared__ uint32_t cnt[B2];
-  __shared__ uint8_t rnd[SEG];
   __shared__ int nr[RCAP];
   const Layout lay = layout(j);
   const int b1 = 1 << j.lbs;
@@ -86,21 +86,28 @@ __device__ __forceinline__ void fft_grp_lists_body(const RegJob& j, int g) {
   int out = 0;
   for (int base0 = 0; base0 < keys; base0 += SEG) {
     const int sl = min(SEG, keys - base0);
+    // the whole segment's hash entries in flight at once
+    uint32_t ek[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int key = base0 + q * 64 + lane;
+      ek[q] = q * 64 + lane < sl ? (key < m ? hA[key] : hB[key - m]) : 0u;
+    }
     if (lane < RCAP) nr[lane] = 0;
     __syncthreads();
-    for (int c = 0; c < sl; c += 64) {
-      if (c + lane < sl) {
-        const int key = base0 + c + lane;
-        const uint32_t h = hbucket(key < m ? hA[key] : hB[key - m]);
-        int r = -1;
-        if (((h >> LB2) & (G - 1)) == w) {
-          r = min(int(atomicAdd(&cnt[h & (B2 - 1)], 1u)), RCAP - 1);
-          atomicAdd(&nr[r], 1);
-        }
-        rnd[c + lane] = uint8_t(r < 0 ? 255 : r);
+    // round = occurrence of the bucket so far, in key order (the wave's LDS
+    // atomics complete in program order), -1 outside this partition
+    int rk[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const uint32_t h = hbucket(ek[q]);
+      rk[q] = -1;
+      if (q * 64 + lane < sl && ((h >> LB2) & (G - 1)) == w) {
+        rk[q] = min(int(atomicAdd(&cnt[h & (B2 - 1)], 1u)), RCAP - 1);
+        atomicAdd(&nr[rk[q]], 1);
       }
-      __syncthreads();
     }
+    __syncthreads();
     int base[RCAP], run[RCAP];
     int acc = out;
 #pragma unroll
@@ -109,21 +116,18 @@ __device__ __forceinline__ void fft_grp_lists_body(const RegJob& j, int g) {
       acc += (nr[r] + 63) / 64;
       run[r] = 0;
     }
-    for (int c = 0; c < sl; c += 64) {
-      const bool ok = c + lane < sl;
-      const int rv = ok ? int(rnd[c + lane]) : 255;
-      const int r = rv == 255 ? -1 : rv;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int r = rk[q];
       uint32_t ent = 0;
       float2 coef = make_float2(0.f, 0.f);
       if (r >= 0) {
-        const int key = base0 + c + lane;
-        const bool isb = key >= m;
-        const uint32_t e = isb ? hB[key - m] : hA[key];
-        const uint32_t h = hbucket(e);
+        const bool isb = base0 + q * 64 + lane >= m;
+        const uint32_t h = hbucket(ek[q]);
         const float2 wv = tw[(uint64_t(h) * s0) & bmask];
-        const float sg = hneg(e) ? -1.f : 1.f;
+        const float sg = hneg(ek[q]) ? -1.f : 1.f;
         coef = isb ? make_float2(-sg * wv.y, sg * wv.x) : make_float2(sg * wv.x, sg * wv.y);
-        ent = ((h & (B2 - 1)) << 3) | (uint32_t(c + lane) << 18);
+        ent = ((h & (B2 - 1)) << 3) | (uint32_t(q * 64 + lane) << 18);
       }
 #pragma unroll
       for (int rr = 0; rr < RCAP; ++rr) {

@@ -66,7 +66,13 @@ __device__ __forceinline__ void hash_body(const HashArgs& a, int blk) {
   }
 }
 
-__global__ void __launch_bounds__(256) k_hash(HashArgs a) { hash_body(a, blockIdx.x); }
+// The parameter block is read in place from the kernel-argument segment (it
+// is the only argument, at offset 0): indexing the by-value argument with a
+// run-time index would copy all of it (2 KB) into every lane's scratch.
+__global__ void __launch_bounds__(256) k_hash(HashArgs) {
+  const void* seg = (const void*)__builtin_amdgcn_kernarg_segment_ptr();
+  hash_body(*static_cast<const HashArgs*>(seg), blockIdx.x);
+}
 
 __global__ void __launch_bounds__(256) k_hash_batch(const HashArgs* __restrict__ jobs,
                                                     const int2* __restrict__ blocks) {
