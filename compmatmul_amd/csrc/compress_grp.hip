@@ -206,13 +206,14 @@ __device__ __forceinline__ void scatter_section(const uint32_t* __restrict__ lp,
 // Scatter one row for the G slices of the group, cross-partition WHT, then
 // the register WHT of this wave's slice.  On return lane L, register r holds
 // frequency f_of(L, r) of slice group*G + w.
-// With `staged`, the row (one segment) is already in Rb.
-template <int LOG_G>
+// With `staged`, the row (one segment) is already in Rb; `after_scatter`
+// runs once every wave has finished reading it.
+template <int LOG_G, typename AfterScatter>
 __device__ __forceinline__ void sketch_transform_grp(const float* __restrict__ row, int keys,
                                                      const uint32_t* __restrict__ list,
                                                      const int* __restrict__ segcnt, float* S,
                                                      float* Rb, bool staged, int w, int lane,
-                                                     float (&x)[R]) {
+                                                     float (&x)[R], AfterScatter after_scatter) {
   constexpr int G = 1 << LOG_G;
   float* Sw = S + w * SLEN;
   const int tid = w * 64 + lane;
@@ -246,6 +247,7 @@ __device__ __forceinline__ void sketch_transform_grp(const float* __restrict__ r
     lp += int64_t(nr[RCAP - 1]) * 64;
   }
   __syncthreads();  // all partitions complete
+  after_scatter();   // the staged row is free again
   if constexpr (G > 1) {
     // wave w transforms positions l in [w 2048/G, (w+1) 2048/G) across the
     // G partitions (natural-order WHT over hw)
@@ -308,8 +310,32 @@ __device__ __forceinline__ int f_of(int L, int r) {
   return ((L & 31) << 6) | ((r & 1) << 5) | ((r & ~1) + (L >> 5));
 }
 
+// Start copying row[0, len) (len <= SEG) into LDS dst[0, SEG) with
+// global_load_lds, 4 bytes per lane and no registers: wave w of the
+// workgroup copies elements q 64G + 64 w + lane; lanes past the row copy its
+// last element into slots the schedule never reads.
+template <int G>
+__device__ __forceinline__ void copy_row_async(const float* __restrict__ row, int len, float* dst,
+                                               int tid) {
+  if (len <= 0) return;
+  const int wbase = tid & ~63;
+  // opaque per call, so the per-lane offsets are recomputed here (two VALU
+  // each) instead of being hoisted out of the k loop and kept live
+  int lane_id = tid;
+  asm volatile("" : "+v"(lane_id));
+#pragma unroll
+  for (int q = 0; q < SEG / (64 * G); ++q) {
+    const int e = q * 64 * G + lane_id;
+    __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)(row + min(e, len - 1)),
+                                     (__attribute__((address_space(3))) void*)(dst + q * 64 * G + wbase), 4,
+                                     0, 0);
+  }
+}
+
 // unit u = (chunk, t, group); wave w owns slice group * G + w.  PAIRED: both
-// rows fit one segment (m, p <= SEG) and are staged together.
+// rows fit one segment (m, p <= SEG); they are copied into LDS ahead of use
+// (row k + 1 of A while A's row k is transformed, likewise for B), so the
+// load latency is hidden behind the transforms.
 template <int LOG_G, bool PAIRED>
 __device__ __forceinline__ void grp_unit(const RegJob& j, int u, float* S, float* Rb) {
   constexpr int G = 1 << LOG_G;
@@ -328,31 +354,27 @@ __device__ __forceinline__ void grp_unit(const RegJob& j, int u, float* S, float
   float acc[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) acc[r] = 0.f;
-  // rows of at most one segment each: stage A's and B's row together, so
-  // the load latency is exposed once per k
   constexpr bool paired = PAIRED;
+  const int tid = threadIdx.x;
+  if (paired && k0 < k1) {
+    copy_row_async<G>(j.AT + int64_t(k0) * j.ld_at, j.m, Rb, tid);
+    copy_row_async<G>(j.B + int64_t(k0) * j.ld_b, j.p, Rb + SEG, tid);
+  }
   for (int k = k0; k < k1; ++k) {
     float xa[R], xb[R];
     const float* arow = j.AT + int64_t(k) * j.ld_at;
     const float* brow = j.B + int64_t(k) * j.ld_b;
     if constexpr (paired) {
-      const int tid = threadIdx.x;
-      float va[SEG / (64 * G)], vb[SEG / (64 * G)];
-#pragma unroll
-      for (int q = 0; q < SEG / (64 * G); ++q) {
-        va[q] = j.m > 0 ? arow[min(q * 64 * G + tid, j.m - 1)] : 0.f;
-        vb[q] = j.p > 0 ? brow[min(q * 64 * G + tid, j.p - 1)] : 0.f;
-      }
-      __syncthreads();  // every wave is done with the previous rows
-#pragma unroll
-      for (int q = 0; q < SEG / (64 * G); ++q) {
-        Rb[q * 64 * G + tid] = va[q];
-        Rb[SEG + q * 64 * G + tid] = vb[q];
-      }
-      __syncthreads();
+      __builtin_amdgcn_s_waitcnt(0);  // this wave's copies of rows k have landed
+      __syncthreads();                // and every wave's
     }
-    sketch_transform_grp<LOG_G>(arow, j.m, la, sca, S, Rb, paired, w, lane, xa);
-    sketch_transform_grp<LOG_G>(brow, j.p, lb, scb, S, paired ? Rb + SEG : Rb, paired, w, lane, xb);
+    const bool more = paired && k + 1 < k1;
+    sketch_transform_grp<LOG_G>(arow, j.m, la, sca, S, Rb, paired, w, lane, xa, [&] {
+      if (more) copy_row_async<G>(arow + j.ld_at, j.m, Rb, tid);
+    });
+    sketch_transform_grp<LOG_G>(brow, j.p, lb, scb, S, paired ? Rb + SEG : Rb, paired, w, lane, xb, [&] {
+      if (more) copy_row_async<G>(brow + j.ld_b, j.p, Rb + SEG, tid);
+    });
 #pragma unroll
     for (int r = 0; r < R; ++r) acc[r] = fmaf(xa[r], xb[r], acc[r]);
   }
