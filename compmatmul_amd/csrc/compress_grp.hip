@@ -203,17 +203,85 @@ __device__ __forceinline__ void scatter_section(const uint32_t* __restrict__ lp,
   }
 }
 
+// Schedule groups kept in registers for a whole unit: the schedule of a
+// (t, slice) list does not depend on k.  Slots past the list hold dummy
+// entries (plain writes of Rb[0] into the pad slot 2048 + lane).
+constexpr int CA = 12;
+
+// An empty operand (keys == 0) has no schedule: its counts are never
+// written, so they are not read.
+__device__ __forceinline__ void load_cache(const uint32_t* __restrict__ lp,
+                                           const int* __restrict__ nr, int keys, uint32_t (&c)[CA],
+                                           int lane) {
+  int ng = 0;
+  if (keys > 0)
+    for (int r = 0; r < RCAP; ++r) ng += nr[r];
+#pragma unroll
+  for (int g = 0; g < CA; ++g) c[g] = g < ng ? lp[g * 64 + lane] : uint32_t(B2 + lane) << 2;
+}
+
+// The scatter of one single-segment list from the cached groups, then any
+// groups past the cache from memory.  Group g is a plain write below nr[0],
+// a read-add-write below nr[0] + (rounds 1 .. RCAP-2), else an LDS atomic;
+// one wave's LDS operations complete in order, as the rounds require.
+__device__ __forceinline__ void scatter_cached(const uint32_t (&c)[CA],
+                                               const uint32_t* __restrict__ lp,
+                                               const int* __restrict__ nr, int keys, float* S,
+                                               const float* Rb, int lane) {
+  const bool any = keys > 0;
+  const int n0 = any ? nr[0] : 0;
+  int e1 = n0;
+  if (any)
+    for (int r = 1; r < RCAP - 1; ++r) e1 += nr[r];
+  const int ng = any ? e1 + nr[RCAP - 1] : 0;
+  const char* Rbb = reinterpret_cast<const char*>(Rb);
+  char* Sb = reinterpret_cast<char*>(S);
+  constexpr int BATCH = 4;  // LDS reads in flight per batch
+#pragma unroll
+  for (int g0 = 0; g0 < CA; g0 += BATCH) {
+    // opaque copies: the fields are extracted here, not hoisted out of the
+    // k loop as three live registers per entry
+    uint32_t e[BATCH];
+    float v[BATCH];
+#pragma unroll
+    for (int q = 0; q < BATCH; ++q) {
+      e[q] = c[g0 + q];
+      asm volatile("" : "+v"(e[q]));
+      v[q] = flip(*reinterpret_cast<const float*>(Rbb + ((e[q] >> 16) & 0x1ffcu)), e[q] & F_SIGN);
+    }
+#pragma unroll
+    for (int q = 0; q < BATCH; ++q) {
+      const int g = g0 + q;
+      float* d = reinterpret_cast<float*>(Sb + (e[q] & 0xfffcu));
+      if (g < n0 || g >= ng)
+        *d = v[q];
+      else if (g < e1)
+        *d += v[q];
+      else
+        __hip_atomic_fetch_add(d, v[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+  }
+  if (ng > CA) {  // a deep schedule: the rest from memory, section by section
+    if (n0 > CA) scatter_section<0>(lp + CA * 64, n0 - CA, S, Rb, lane);
+    const int s1 = max(n0, CA);
+    if (e1 > s1) scatter_section<1>(lp + int64_t(s1) * 64, e1 - s1, S, Rb, lane);
+    const int s2 = max(e1, CA);
+    if (ng > s2) scatter_section<2>(lp + int64_t(s2) * 64, ng - s2, S, Rb, lane);
+  }
+}
+
 // Scatter one row for the G slices of the group, cross-partition WHT, then
 // the register WHT of this wave's slice.  On return lane L, register r holds
 // frequency f_of(L, r) of slice group*G + w.
 // With `staged`, the row (one segment) is already in Rb; `after_scatter`
 // runs once every wave has finished reading it.
-template <int LOG_G, typename AfterScatter>
+template <int LOG_G, bool CACHED, typename AfterScatter>
 __device__ __forceinline__ void sketch_transform_grp(const float* __restrict__ row, int keys,
                                                      const uint32_t* __restrict__ list,
                                                      const int* __restrict__ segcnt, float* S,
                                                      float* Rb, bool staged, int w, int lane,
-                                                     float (&x)[R], AfterScatter after_scatter) {
+                                                     float (&x)[R], const uint32_t (&cache)[CA],
+                                                     AfterScatter after_scatter) {
   constexpr int G = 1 << LOG_G;
   float* Sw = S + w * SLEN;
   const int tid = w * 64 + lane;
@@ -224,7 +292,8 @@ __device__ __forceinline__ void sketch_transform_grp(const float* __restrict__ r
   }
   const int nseg = (keys + SEG - 1) / SEG;
   const uint32_t* __restrict__ lp = list;
-  for (int seg = 0; seg < nseg; ++seg) {
+  if constexpr (CACHED) scatter_cached(cache, list, segcnt, keys, Sw, Rb, lane);  // staged, one segment
+  for (int seg = 0; seg < (CACHED ? 0 : nseg); ++seg) {
     if (!staged) {
       const int s0 = seg * SEG;
       const int sl = min(SEG, keys - s0);
@@ -356,6 +425,12 @@ __device__ __forceinline__ void grp_unit(const RegJob& j, int u, float* S, float
   for (int r = 0; r < R; ++r) acc[r] = 0.f;
   constexpr bool paired = PAIRED;
   const int tid = threadIdx.x;
+  // paired lists are single-segment: keep their first groups in registers
+  uint32_t ca[CA], cb[CA];
+  if constexpr (paired) {
+    load_cache(la, sca, j.m, ca, lane);
+    load_cache(lb, scb, j.p, cb, lane);
+  }
   if (paired && k0 < k1) {
     copy_row_async<G>(j.AT + int64_t(k0) * j.ld_at, j.m, Rb, tid);
     copy_row_async<G>(j.B + int64_t(k0) * j.ld_b, j.p, Rb + SEG, tid);
@@ -369,12 +444,13 @@ __device__ __forceinline__ void grp_unit(const RegJob& j, int u, float* S, float
       __syncthreads();                // and every wave's
     }
     const bool more = paired && k + 1 < k1;
-    sketch_transform_grp<LOG_G>(arow, j.m, la, sca, S, Rb, paired, w, lane, xa, [&] {
+    sketch_transform_grp<LOG_G, paired>(arow, j.m, la, sca, S, Rb, paired, w, lane, xa, ca, [&] {
       if (more) copy_row_async<G>(arow + j.ld_at, j.m, Rb, tid);
     });
-    sketch_transform_grp<LOG_G>(brow, j.p, lb, scb, S, paired ? Rb + SEG : Rb, paired, w, lane, xb, [&] {
-      if (more) copy_row_async<G>(brow + j.ld_b, j.p, Rb + SEG, tid);
-    });
+    sketch_transform_grp<LOG_G, paired>(brow, j.p, lb, scb, S, paired ? Rb + SEG : Rb, paired, w, lane, xb,
+                                        cb, [&] {
+                                          if (more) copy_row_async<G>(brow + j.ld_b, j.p, Rb + SEG, tid);
+                                        });
 #pragma unroll
     for (int r = 0; r < R; ++r) acc[r] = fmaf(xa[r], xb[r], acc[r]);
   }
