@@ -227,8 +227,33 @@ __device__ __forceinline__ void dft_g(float2 (&y)[G]) {
   }
 }
 
+// Start copying keys [0, sl) of the concatenated row (A's entries, then
+// B's) into LDS dst[0, SEG) with global_load_lds, 4 bytes per lane and no
+// registers; lanes past the segment copy its last key.
+template <int G>
+__device__ __forceinline__ void copy_seg0_async(const float* __restrict__ ar, int m,
+                                                const float* __restrict__ br, int sl, float* dst,
+                                                int tid) {
+  const int wbase = tid & ~63;
+  // opaque per call, so the per-lane addresses are recomputed here instead
+  // of being hoisted out of the k loop and kept live
+  int lane_id = tid;
+  asm volatile("" : "+v"(lane_id));
+#pragma unroll
+  for (int q = 0; q < SEG / (64 * G); ++q) {
+    const int key = min(q * 64 * G + lane_id, sl - 1);
+    const float* src = key < m ? ar + key : br + (key - m);
+    __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)src,
+                                     (__attribute__((address_space(3))) void*)(dst + q * 64 * G + wbase), 4,
+                                     0, 0);
+  }
+}
+
 // unit u = (chunk, t, s0): wave j owns slice s0 + j b1/G
-template <int LOG_G>
+// ASYNC: copy the first segment of row k + 1 during the transforms of k
+// (single-product launches; measured slower in the persistent batch kernel,
+// whose units are mostly short).
+template <int LOG_G, bool ASYNC>
 __device__ __forceinline__ void fft_grp_unit(const RegJob& j, int u, float2* S, float* Rb) {
   constexpr int G = 1 << LOG_G;
   const Layout lay = layout(j);
@@ -258,6 +283,12 @@ __device__ __forceinline__ void fft_grp_unit(const RegJob& j, int u, float2* S, 
   float qr[R], qi[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) qr[r] = qi[r] = 0.f;
+  // the first segment of row k + 1 is copied straight into LDS while the
+  // transforms of k run; later segments are staged through registers, whose
+  // loads are issued before the barrier and overlap the other waves'
+  // scatter of the previous segment (an LDS copy could only start after it)
+  if (ASYNC && k0 < k1 && nseg > 0)
+    copy_seg0_async<G>(j.AT + int64_t(k0) * j.ld_at, j.m, j.B + int64_t(k0) * j.ld_b, min(SEG, keys), Rb, tid);
   for (int k = k0; k < k1; ++k) {
     const float* ar = j.AT + int64_t(k) * j.ld_at;
     const float* br = j.B + int64_t(k) * j.ld_b;
@@ -268,18 +299,23 @@ __device__ __forceinline__ void fft_grp_unit(const RegJob& j, int u, float2* S, 
     }
     int64_t off = 0;
     for (int seg = 0; seg < nseg; ++seg) {
-      const int base0 = seg * SEG;
-      const int sl = min(SEG, keys - base0);
-      float v[SEG / (64 * G)];
+      if (ASYNC && seg == 0) {
+        __builtin_amdgcn_s_waitcnt(0);  // this wave's copies of the segment have landed
+        __syncthreads();                // and every wave's
+      } else {
+        const int base0 = seg * SEG;
+        const int sl = min(SEG, keys - base0);
+        float v[SEG / (64 * G)];
 #pragma unroll
-      for (int q = 0; q < SEG / (64 * G); ++q) {
-        const int key = base0 + min(q * 64 * G + tid, sl - 1);
-        v[q] = key < j.m ? ar[key] : br[key - j.m];
+        for (int q = 0; q < SEG / (64 * G); ++q) {
+          const int key = base0 + min(q * 64 * G + tid, sl - 1);
+          v[q] = key < j.m ? ar[key] : br[key - j.m];
+        }
+        __syncthreads();  // every wave is done with the previous segment
+#pragma unroll
+        for (int q = 0; q < SEG / (64 * G); ++q) Rb[q * 64 * G + tid] = v[q];
+        __syncthreads();
       }
-      __syncthreads();  // every wave is done with the previous segment
-#pragma unroll
-      for (int q = 0; q < SEG / (64 * G); ++q) Rb[q * 64 * G + tid] = v[q];
-      __syncthreads();
       const int* __restrict__ nr = sc + seg * RCAP;
       int nw = 0;
       for (int r = 1; r < RCAP - 1; ++r) nw += nr[r];
@@ -290,7 +326,9 @@ __device__ __forceinline__ void fft_grp_unit(const RegJob& j, int u, float2* S, 
       scatter_section<2>(ls + off, cs + off, nr[RCAP - 1], Sw, Rb, lane);
       off += int64_t(nr[RCAP - 1]) * 64;
     }
-    __syncthreads();  // all partitions complete
+    __syncthreads();  // all partitions complete, and every wave is done reading Rb
+    if (ASYNC && nseg > 0 && k + 1 < k1)
+      copy_seg0_async<G>(ar + j.ld_at, j.m, br + j.ld_b, min(SEG, keys), Rb, tid);
     {
       // positions l of this wave: [w 2048/G, (w+1) 2048/G); output
       // partition jj gets the DFT over hw times W_{2048 G}^(l jj)
@@ -347,7 +385,7 @@ k_compress_fft_grp(RegJob j, int units) {
   const int g = blockIdx.x;
   const int u = (g & 7) * (G >> 3) + (g >> 3);  // XCD-aware numbering
   if (u >= units) return;
-  fft_grp_unit<LOG_G>(j, u, S, Rb);
+  fft_grp_unit<LOG_G, true>(j, u, S, Rb);
 }
 
 // Persistent form for many products: the XCD work queues of the single-wave
@@ -372,7 +410,7 @@ k_compress_fft_grp_batch(const RegJob* __restrict__ jobs, const int2* __restrict
       __syncthreads();  // every thread has read `item` before it is rewritten
       if (it >= len) break;
       const int2 e = items[base + it];
-      fft_grp_unit<LOG_G>(jobs[e.x], e.y, S, Rb);
+      fft_grp_unit<LOG_G, false>(jobs[e.x], e.y, S, Rb);
       __syncthreads();
     }
   }
