@@ -41,6 +41,20 @@ __device__ __forceinline__ float flip(float v, uint32_t sgn) {
   return __int_as_float(__float_as_int(v) ^ sgn);
 }
 
+// Partition layout.  G = 4 stores position l of a partition at
+// (l >> 5) * TROW + (l & 31): 64 rows of 32 positions padded to 33 floats,
+// so both the in-place write of the cross pass and the transposed read are
+// conflict-free; slot 32 of row i is the dummy slot of lane i.  G <= 2 keeps
+// positions in order, dummies at 2048 + lane.
+template <int LOG_G>
+__host__ __device__ __forceinline__ uint32_t slot_of(uint32_t l) {
+  return LOG_G == 2 ? (l >> 5) * TROW + (l & 31) : l;
+}
+template <int LOG_G>
+__host__ __device__ __forceinline__ uint32_t dummy_slot(int lane) {
+  return LOG_G == 2 ? uint32_t(lane) * TROW + 32 : uint32_t(B2 + lane);
+}
+
 __host__ __device__ __forceinline__ int64_t list_cap(int keys) {
   const int64_t segs = (int64_t(keys) + SEG - 1) / SEG;
   return int64_t(keys) + segs * 64 * RCAP;
@@ -123,7 +137,7 @@ __device__ __forceinline__ void grp_lists_body(const RegJob& j, int g) {
       if (r >= 0) {
         const uint32_t h = hbucket(ek[q]);
         const uint32_t neg = hneg(ek[q]) ^ (__popc((h >> (LB2 + LOG_G)) & group) & 1u);
-        ent = ((h & (B2 - 1)) << 2) | (uint32_t(q * 64 + lane) << 18) | (neg ? F_SIGN : 0u);
+        ent = (slot_of<LOG_G>(h & (B2 - 1)) << 2) | (uint32_t(q * 64 + lane) << 18) | (neg ? F_SIGN : 0u);
       }
 #pragma unroll
       for (int rr = 0; rr < RCAP; ++rr) {
@@ -139,7 +153,7 @@ __device__ __forceinline__ void grp_lists_body(const RegJob& j, int g) {
     for (int rr = 0; rr < RCAP; ++rr) {  // pad rounds to whole groups (dummy slots)
       const int full = (nr[rr] + 63) / 64 * 64;
       for (int slot = nr[rr] + lane; slot < full; slot += 64)
-        L[int64_t(base[rr]) * 64 + slot] = uint32_t(B2 + (slot & 63)) << 2;
+        L[int64_t(base[rr]) * 64 + slot] = dummy_slot<LOG_G>(slot & 63) << 2;
     }
     if (lane < RCAP) {
       const int64_t row = int64_t(g) * MAXSEG + s0 / SEG;
@@ -158,6 +172,25 @@ __global__ void __launch_bounds__(64) k_grp_lists_batch(const RegJob* __restrict
                                                           const int2* __restrict__ blocks) {
   const int2 e = blocks[blockIdx.x];
   grp_lists_body<LOG_G>(jobs[e.x], e.y);
+}
+
+// x[i] = x[i] of lane L ^ 1 times s, plus x[i], for i < 8: one
+// v_fmac_f32 with a DPP quad_perm source each (the compiler does not fold a
+// DPP move into an fma).  The s_nop gives the two wait states a DPP read of
+// a VGPR needs after a VALU write; each x[i] is read once, before its write.
+__device__ __forceinline__ void dpp_butterfly8(float* x, float s) {
+  asm volatile(
+      "s_nop 1\n\t"
+      "v_fmac_f32_dpp %0, %0, %8 quad_perm:[1,0,3,2] row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f32_dpp %1, %1, %8 quad_perm:[1,0,3,2] row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f32_dpp %2, %2, %8 quad_perm:[1,0,3,2] row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f32_dpp %3, %3, %8 quad_perm:[1,0,3,2] row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f32_dpp %4, %4, %8 quad_perm:[1,0,3,2] row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f32_dpp %5, %5, %8 quad_perm:[1,0,3,2] row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f32_dpp %6, %6, %8 quad_perm:[1,0,3,2] row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f32_dpp %7, %7, %8 quad_perm:[1,0,3,2] row_mask:0xf bank_mask:0xf"
+      : "+v"(x[0]), "+v"(x[1]), "+v"(x[2]), "+v"(x[3]), "+v"(x[4]), "+v"(x[5]), "+v"(x[6]), "+v"(x[7])
+      : "v"(s));
 }
 
 __device__ __forceinline__ void wht_reg(float (&x)[R]) {
@@ -210,6 +243,7 @@ constexpr int CA = 12;
 
 // An empty operand (keys == 0) has no schedule: its counts are never
 // written, so they are not read.
+template <int LOG_G>
 __device__ __forceinline__ void load_cache(const uint32_t* __restrict__ lp,
                                            const int* __restrict__ nr, int keys, uint32_t (&c)[CA],
                                            int lane) {
@@ -217,7 +251,7 @@ __device__ __forceinline__ void load_cache(const uint32_t* __restrict__ lp,
   if (keys > 0)
     for (int r = 0; r < RCAP; ++r) ng += nr[r];
 #pragma unroll
-  for (int g = 0; g < CA; ++g) c[g] = g < ng ? lp[g * 64 + lane] : uint32_t(B2 + lane) << 2;
+  for (int g = 0; g < CA; ++g) c[g] = g < ng ? lp[g * 64 + lane] : dummy_slot<LOG_G>(lane) << 2;
 }
 
 // The scatter of one single-segment list from the cached groups, then any
@@ -287,8 +321,10 @@ __device__ __forceinline__ void sketch_transform_grp(const float* __restrict__ r
   const int tid = w * 64 + lane;
   {
     float4* S4 = reinterpret_cast<float4*>(Sw);
+    constexpr int Z4 = (LOG_G == 2 ? SLEN : B2) / 4;  // float4 to clear
 #pragma unroll
-    for (int q = 0; q < B2 / 4 / 64; ++q) S4[q * 64 + lane] = make_float4(0.f, 0.f, 0.f, 0.f);
+    for (int q = 0; q < (Z4 + 63) / 64; ++q)
+      if (q * 64 + lane < Z4) S4[q * 64 + lane] = make_float4(0.f, 0.f, 0.f, 0.f);
   }
   const int nseg = (keys + SEG - 1) / SEG;
   const uint32_t* __restrict__ lp = list;
@@ -317,6 +353,59 @@ __device__ __forceinline__ void sketch_transform_grp(const float* __restrict__ r
   }
   __syncthreads();  // all partitions complete
   after_scatter();   // the staged row is free again
+  if constexpr (LOG_G == 2) {
+    // One register pass does the cross-partition WHT together with
+    // position bits 6..8: lane L, register (q << 2) | hw holds position
+    // w 512 + q 64 + L of partition hw.  The results go back to the same
+    // slots (this wave's region, so no other wave is reading them), now as
+    // slice (q << 2 | j) -> row j of the padded layout.
+    const int p0 = w * 512 + lane;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int a = int(slot_of<2>(uint32_t(p0 + q * 64)));
+#pragma unroll
+      for (int hw = 0; hw < 4; ++hw) x[(q << 2) | hw] = S[hw * SLEN + a];
+    }
+    wht_reg(x);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int a = int(slot_of<2>(uint32_t(p0 + q * 64)));
+#pragma unroll
+      for (int jj = 0; jj < 4; ++jj) S[jj * SLEN + a] = x[(q << 2) | jj];
+    }
+    __syncthreads();  // every wave's region is written
+    // wave w now reads its slice: row `lane` = positions 32 lane + r
+    {
+      const float* rbase = Sw + lane * TROW;
+#pragma unroll
+      for (int r = 0; r < R; ++r) x[r] = rbase[r];
+    }
+    wht_reg(x);  // position bits 0..4
+#pragma unroll
+    for (int jj = 0; jj < R / 2; ++jj) {  // position bit 10 (lane bit 5)
+      const auto pq = __builtin_amdgcn_permlane32_swap(__float_as_uint(x[2 * jj]),
+                                                       __float_as_uint(x[2 * jj + 1]), false, false);
+      const float p = __uint_as_float(pq[0]), q = __uint_as_float(pq[1]);
+      x[2 * jj] = p + q;
+      x[2 * jj + 1] = p - q;
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) {  // position bit 9 (lane bit 4)
+      if (r & 2) continue;
+      const auto pq = __builtin_amdgcn_permlane16_swap(__float_as_uint(x[r]), __float_as_uint(x[r | 2]),
+                                                       false, false);
+      const float p = __uint_as_float(pq[0]), q = __uint_as_float(pq[1]);
+      x[r] = p + q;
+      x[r | 2] = p - q;
+    }
+    // position bit 5 (lane bit 0): x[L ^ 1] s + x[L] with s = -1 on odd
+    // lanes, which therefore hold the negated value; A and B carry the same
+    // signs, so their product is unchanged
+    const float sgn = (lane & 1) ? -1.f : 1.f;
+#pragma unroll
+    for (int r = 0; r < R; r += 8) dpp_butterfly8(x + r, sgn);
+    return;
+  }
   if constexpr (G > 1) {
     // wave w transforms positions l in [w 2048/G, (w+1) 2048/G) across the
     // G partitions (natural-order WHT over hw)
@@ -379,6 +468,14 @@ __device__ __forceinline__ int f_of(int L, int r) {
   return ((L & 31) << 6) | ((r & 1) << 5) | ((r & ~1) + (L >> 5));
 }
 
+// G = 4: frequency of lane L, register r after the merged cross pass
+// (register bits 0, 1 -> f10, f9; 2..4 -> f2..f4; lane bit 0 -> f5,
+// 1..3 -> f6..f8, 4 -> f1, 5 -> f0)
+__device__ __forceinline__ int f_of4(int L, int r) {
+  return ((r & 1) << 10) | (((r >> 1) & 1) << 9) | (r & 0x1c) | ((L & 15) << 5) | (((L >> 4) & 1) << 1) |
+         ((L >> 5) & 1);
+}
+
 // Start copying row[0, len) (len <= SEG) into LDS dst[0, SEG) with
 // global_load_lds, 4 bytes per lane and no registers: wave w of the
 // workgroup copies elements q 64G + 64 w + lane; lanes past the row copy its
@@ -428,8 +525,8 @@ __device__ __forceinline__ void grp_unit(const RegJob& j, int u, float* S, float
   // paired lists are single-segment: keep their first groups in registers
   uint32_t ca[CA], cb[CA];
   if constexpr (paired) {
-    load_cache(la, sca, j.m, ca, lane);
-    load_cache(lb, scb, j.p, cb, lane);
+    load_cache<LOG_G>(la, sca, j.m, ca, lane);
+    load_cache<LOG_G>(lb, scb, j.p, cb, lane);
   }
   if (paired && k0 < k1) {
     copy_row_async<G>(j.AT + int64_t(k0) * j.ld_at, j.m, Rb, tid);
@@ -457,7 +554,7 @@ __device__ __forceinline__ void grp_unit(const RegJob& j, int u, float* S, float
   float* Sw = S + w * SLEN;
   __syncthreads();
 #pragma unroll
-  for (int r = 0; r < R; ++r) Sw[f_of(lane, r)] = acc[r];
+  for (int r = 0; r < R; ++r) Sw[LOG_G == 2 ? f_of4(lane, r) : f_of(lane, r)] = acc[r];
   __syncthreads();
   float* out = j.partial + (int64_t(chunk) * lay.lists + ts) * B2;
 #pragma unroll
@@ -482,7 +579,7 @@ k_compress_fwht_grp(RegJob j, int units) {
 // batch kernels; thread 0 claims each item and the workgroup reads it from
 // LDS.
 template <int LOG_G>
-__global__ void __launch_bounds__(64 << LOG_G)
+__global__ void __launch_bounds__(64 << LOG_G) __attribute__((amdgpu_waves_per_eu(3, 3)))
 k_compress_fwht_grp_batch(const RegJob* __restrict__ jobs, const int2* __restrict__ items,
                   const int* __restrict__ qstart, unsigned* __restrict__ counters) {
   __shared__ __align__(16) float S[SLEN << LOG_G];
