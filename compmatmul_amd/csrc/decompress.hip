@@ -9,7 +9,7 @@
 // One thread owns a column j for a group of rows: the d packed hash entries
 // of column j stay in registers, the row entries are wave-uniform loads,
 // the sketch (d x b) is gathered from L2, and the median comes from a
-// register sorting network (d <= 16) or a rank count (larger d).
+// register selection network (d <= 16) or a rank count (larger d).
 // Recovery appends (i, j, value) for every estimate > tau through a global
 // counter; the host sorts the list by (i, j).
 #include "device_common.hpp"
@@ -27,18 +27,60 @@ __device__ __forceinline__ float vmax(float a, float b) { return fmaxf(a, b); }
 __device__ __forceinline__ double vmin(double a, double b) { return fmin(a, b); }
 __device__ __forceinline__ double vmax(double a, double b) { return fmax(a, b); }
 
-// Odd-even transposition sort: D rounds sort any D values.
+// Median selection network for D <= 16 values, built at compile time:
+// Batcher's odd-even merge sort of the next power of two N (the padding
+// positions D..N-1 act as +inf, so comparators touching them are dropped),
+// then pruned backwards to the comparators that can move a value into
+// output D/2.  Compare-exchanges only move values, so v[D/2] afterwards is
+// exactly element D/2 of the sorted values.  D = 15: 49 comparators (a full
+// odd-even transposition sort takes 105).
+struct CmpNet {
+  int n;
+  unsigned char a[64], b[64];
+};
+
+constexpr CmpNet median_net(int D) {
+  CmpNet full{0, {}, {}};
+  int N = 1;
+  while (N < D) N *= 2;
+  for (int p = 1; p < N; p *= 2)
+    for (int k = p; k >= 1; k /= 2)
+      for (int j = k % p; j + k < N; j += 2 * k)
+        for (int i = 0; i < (k < N - j - k ? k : N - j - k); ++i)
+          if ((i + j) / (2 * p) == (i + j + k) / (2 * p) && i + j + k < D) {
+            full.a[full.n] = static_cast<unsigned char>(i + j);
+            full.b[full.n] = static_cast<unsigned char>(i + j + k);
+            ++full.n;
+          }
+  bool need[16] = {};
+  bool keep[64] = {};
+  need[D / 2] = true;
+  for (int c = full.n - 1; c >= 0; --c)
+    if (need[full.a[c]] || need[full.b[c]]) {
+      keep[c] = true;
+      need[full.a[c]] = need[full.b[c]] = true;
+    }
+  CmpNet out{0, {}, {}};
+  for (int c = 0; c < full.n; ++c)
+    if (keep[c]) {
+      out.a[out.n] = full.a[c];
+      out.b[out.n] = full.b[c];
+      ++out.n;
+    }
+  return out;
+}
+
 template <typename T, int D>
 __device__ __forceinline__ T median_network(T (&v)[D]) {
+  static_assert(D >= 1 && D <= 16, "selection network for d <= 16");
+  constexpr CmpNet net = median_net(D);
 #pragma unroll
-  for (int r = 0; r < D; ++r) {
-#pragma unroll
-    for (int j = r & 1; j + 1 < D; j += 2) {
-      const T lo = vmin(v[j], v[j + 1]);
-      const T hi = vmax(v[j], v[j + 1]);
-      v[j] = lo;
-      v[j + 1] = hi;
-    }
+  for (int c = 0; c < net.n; ++c) {
+    const int i = net.a[c], k = net.b[c];
+    const T lo = vmin(v[i], v[k]);
+    const T hi = vmax(v[i], v[k]);
+    v[i] = lo;
+    v[k] = hi;
   }
   return v[D / 2];
 }
@@ -97,7 +139,7 @@ __global__ void __launch_bounds__(NT) k_decompress(DecompressArgs a) {
 // LDS, 1 KB per wave instruction, no registers): row t+1 is in flight while
 // row t is gathered, and the registers go to rows of C instead.
 constexpr int LNT = 512;
-// rows per thread: about 120 values per thread (more spills for the larger median networks) of gathered values
+// rows per thread: about 120 gathered values per thread
 template <typename T, int D>
 constexpr int rows_per_thread() {
   return (sizeof(T) == 8 ? 60 : 120) / D < 1 ? 1 : ((sizeof(T) == 8 ? 60 : 120) / D > 12 ? 12 : (sizeof(T) == 8 ? 60 : 120) / D);

@@ -15,7 +15,8 @@ HEADER = os.path.join(ROOT, "include", "compmatmul_amd.h")
 
 
 def declared_symbols():
-    text = open(HEADER).read()
+    with open(HEADER) as f:
+        text = f.read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     return sorted(set(re.findall(r"\b(cm_[a-z0-9_]+)\s*\(", text)))
 
