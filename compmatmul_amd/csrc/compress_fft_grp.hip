@@ -209,6 +209,26 @@ __device__ __forceinline__ void scatter_section(const uint32_t* __restrict__ lp,
   }
 }
 
+// W_128^k = exp(-2 pi i k / 128) for k < 32, rounded from double
+__constant__ float2 kW128[32] = {
+    {1.0f, -0.0f}, {0.99879545620517241f, -0.049067674327418015f},
+    {0.99518472667219693f, -0.098017140329560604f}, {0.98917650996478101f, -0.14673047445536175f},
+    {0.98078528040323043f, -0.19509032201612825f}, {0.97003125319454397f, -0.24298017990326387f},
+    {0.95694033573220882f, -0.29028467725446233f}, {0.94154406518302081f, -0.33688985339222005f},
+    {0.92387953251128674f, -0.38268343236508978f}, {0.90398929312344334f, -0.42755509343028208f},
+    {0.88192126434835505f, -0.47139673682599764f}, {0.85772861000027212f, -0.51410274419322166f},
+    {0.83146961230254524f, -0.55557023301960218f}, {0.80320753148064494f, -0.59569930449243336f},
+    {0.77301045336273699f, -0.63439328416364549f}, {0.74095112535495911f, -0.67155895484701833f},
+    {0.70710678118654757f, -0.70710678118654746f}, {0.67155895484701833f, -0.74095112535495911f},
+    {0.63439328416364549f, -0.77301045336273699f}, {0.59569930449243347f, -0.80320753148064483f},
+    {0.55557023301960229f, -0.83146961230254524f}, {0.51410274419322166f, -0.85772861000027212f},
+    {0.47139673682599781f, -0.88192126434835494f}, {0.4275550934302822f, -0.90398929312344334f},
+    {0.38268343236508984f, -0.92387953251128674f}, {0.33688985339222005f, -0.94154406518302081f},
+    {0.29028467725446233f, -0.95694033573220894f}, {0.24298017990326398f, -0.97003125319454397f},
+    {0.19509032201612833f, -0.98078528040323043f}, {0.14673047445536175f, -0.98917650996478101f},
+    {0.09801714032956077f, -0.99518472667219682f}, {0.049067674327418126f, -0.99879545620517241f},
+};
+
 // G-point DFT (natural order, W_G = exp(-2 pi i / G)), G in {2, 4}
 template <int G>
 __device__ __forceinline__ void dft_g(float2 (&y)[G]) {
@@ -334,11 +354,12 @@ __device__ __forceinline__ void fft_grp_unit(const RegJob& j, int u, float2* S, 
       // partition jj gets the DFT over hw times W_{2048 G}^(l jj)
       constexpr int PER = R / G;
       const int l0 = w * (B2 / G);
-      // the twiddles do not depend on k: the laundered table address makes
-      // the compiler reload them (L1-resident) instead of holding 48 VGPRs
-      // across the loop
-      const float2* twl = tw;
-      asm volatile("" : "+v"(twl));
+      // W_{2048 G}^(l jj) for l = l0 + lane + 64 q is the per-lane
+      // W_{2048 G}^((l0 + lane) jj), one table load per k, times the
+      // constant W_{32 G}^(q jj) = W_128^(q jj 4 / G): no load inside the loop
+      float2 twb[G];
+#pragma unroll
+      for (int jj = 1; jj < G; ++jj) twb[jj] = tw[int64_t((l0 + lane) * jj) << tw_shift];
 #pragma unroll 1  // unrolled, all positions' loads would be live at once
       for (int q = 0; q < PER; ++q) {
         const int l = l0 + q * 64 + lane;
@@ -348,7 +369,8 @@ __device__ __forceinline__ void fft_grp_unit(const RegJob& j, int u, float2* S, 
         dft_g<G>(y);
 #pragma unroll
         for (int jj = 1; jj < G; ++jj) {
-          const float2 wv = twl[int64_t(l * jj) << tw_shift];
+          const float2 c = kW128[(q * jj * (4 / G)) & 31];
+          const float2 wv = make_float2(twb[jj].x * c.x - twb[jj].y * c.y, twb[jj].x * c.y + twb[jj].y * c.x);
           y[jj] = make_float2(y[jj].x * wv.x - y[jj].y * wv.y, y[jj].x * wv.y + y[jj].y * wv.x);
         }
 #pragma unroll
