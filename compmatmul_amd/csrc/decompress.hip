@@ -201,6 +201,87 @@ __device__ __forceinline__ void decompress_lds_body(const DecompressArgs& a, int
   }
 }
 
+// Ring variant for 64 KB sketch rows (the two-buffer form above keeps one
+// row in flight): five 32 KB half-row slots fill the 160 KB of LDS, half k
+// of the d x 2 halves goes to slot k mod 5, and while row t is gathered the
+// halves 2t+2 .. 2t+4 are in flight (1.5 rows).  Row t's slots are refilled
+// after the barrier of row t + 1, so one barrier per row still suffices.
+constexpr int RING_SLOTS = 5;
+constexpr int RING_HALF = 32768;                  // bytes per half row
+constexpr int RING_LDS = RING_SLOTS * RING_HALF;  // 160 KB
+constexpr int RING_ISSUE = RING_HALF / 16 / LNT;  // dwordx4 copies per wave and half (4)
+
+template <typename T>
+__device__ __forceinline__ void stage_half_async(const T* __restrict__ P, unsigned char* ring, int b,
+                                                 int k, int tid) {
+  const unsigned char* src =
+      reinterpret_cast<const unsigned char*>(P + int64_t(k >> 1) * b) + (k & 1) * RING_HALF;
+  unsigned char* dst = ring + (k % RING_SLOTS) * RING_HALF;
+  const int wave = tid >> 6, lane = tid & 63;
+#pragma unroll
+  for (int q = 0; q < RING_ISSUE; ++q) {
+    const int c = q * LNT + wave * 64;  // first 16-byte chunk of this wave's copy
+    __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)(src + (c + lane) * 16),
+                                     (__attribute__((address_space(3))) void*)(dst + c * 16), 16, 0, 0);
+  }
+}
+
+template <typename T, bool FWHT, int D>
+__device__ __forceinline__ void decompress_ring_body(const DecompressArgs& a, int blk) {
+  constexpr int ER = rows_per_thread<T, D>();
+  constexpr int NH = 2 * D;  // halves to copy
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int b = a.b;  // b * sizeof(T) == 65536
+  const uint32_t half = uint32_t(b / 2);
+  const int tiles = (a.p + LNT - 1) / LNT;
+  const int tid = threadIdx.x;
+  const int j = int(blk % tiles) * LNT + tid;
+  const int i0 = int(blk / tiles) * ER;
+  const bool jok = j < a.p;
+  const uint32_t mask = uint32_t(b - 1);
+  const T* __restrict__ P = static_cast<const T*>(a.P);
+  uint32_t eb[D];
+#pragma unroll
+  for (int t = 0; t < D; ++t) eb[t] = jok ? a.hB[int64_t(t) * a.p + j] : 0u;
+#pragma unroll
+  for (int k = 0; k < (NH < 3 ? NH : 3); ++k) stage_half_async<T>(P, smem_raw, b, k, tid);
+  T v[ER][D];
+#pragma unroll
+  for (int t = 0; t < D; ++t) {
+    // halves 0 .. 2t+1 landed: at most one later half (RING_ISSUE copies of
+    // this wave) may still be in flight, none at the last row
+    if (t + 1 < D)
+      __builtin_amdgcn_s_waitcnt(0x0070 | RING_ISSUE);  // vmcnt(RING_ISSUE) lgkmcnt(0)
+    else
+      __builtin_amdgcn_s_waitcnt(0x0070);               // vmcnt(0) lgkmcnt(0)
+    __syncthreads();  // every wave's copies landed, and row t-1 is no longer read
+    if (2 * t + 3 < NH) stage_half_async<T>(P, smem_raw, b, 2 * t + 3, tid);
+    if (2 * t + 4 < NH) stage_half_async<T>(P, smem_raw, b, 2 * t + 4, tid);
+    const unsigned char* lo = smem_raw + ((2 * t) % RING_SLOTS) * RING_HALF;
+    const unsigned char* hi = smem_raw + ((2 * t + 1) % RING_SLOTS) * RING_HALF;
+#pragma unroll
+    for (int e = 0; e < ER; ++e) {
+      const int i = min(i0 + e, a.m - 1);
+      const uint32_t ea = a.hA[int64_t(t) * a.m + i];
+      const uint32_t h = FWHT ? (hbucket(ea) ^ hbucket(eb[t])) : ((hbucket(ea) + hbucket(eb[t])) & mask);
+      const unsigned char* base = h >= half ? hi : lo;
+      const T val = *reinterpret_cast<const T*>(base + (h & (half - 1)) * sizeof(T));
+      v[e][t] = (hneg(ea) ^ hneg(eb[t])) ? -val : val;
+    }
+  }
+  if (!jok) return;
+#pragma unroll
+  for (int e = 0; e < ER; ++e) {
+    if (i0 + e >= a.m) break;
+    emit<T>(a, i0 + e, j, median_network<T, D>(v[e]));
+  }
+}
+
+template <typename T, bool FWHT, int D>
+__global__ void __launch_bounds__(LNT) k_decompress_ring(DecompressArgs a) {
+  decompress_ring_body<T, FWHT, D>(a, blockIdx.x);
+}
+
 template <typename T, bool FWHT, int D>
 __global__ void __launch_bounds__(LNT) k_decompress_lds(DecompressArgs a) {
   decompress_lds_body<T, FWHT, D>(a, blockIdx.x);
@@ -251,6 +332,24 @@ __global__ void __launch_bounds__(NT) k_decompress_any(DecompressArgs a) {
 
 template <typename T, bool FWHT>
 hipError_t launch_t(const DecompressArgs& a, hipStream_t s) {
+  if (a.d <= 16 && size_t(a.b) * sizeof(T) == 65536) {  // 64 KB rows: the half-row ring
+    switch (a.d) {
+#define CM_D(N)                                                                               \
+  case N:                                                                                     \
+    if (hipFuncSetAttribute(reinterpret_cast<const void*>(&k_decompress_ring<T, FWHT, N>),    \
+                            hipFuncAttributeMaxDynamicSharedMemorySize, RING_LDS) != hipSuccess) \
+      break;                                                                                  \
+    k_decompress_ring<T, FWHT, N><<<dim3(unsigned(int64_t((a.p + LNT - 1) / LNT) *           \
+                                                 ((a.m + rows_per_thread<T, N>() - 1) /      \
+                                                  rows_per_thread<T, N>()))),                \
+                                    dim3(LNT), RING_LDS, s>>>(a);                             \
+    return hipGetLastError();
+      CM_D(1) CM_D(2) CM_D(3) CM_D(4) CM_D(5) CM_D(6) CM_D(7) CM_D(8)
+      CM_D(9) CM_D(10) CM_D(11) CM_D(12) CM_D(13) CM_D(14) CM_D(15) CM_D(16)
+#undef CM_D
+    }
+    (void)hipGetLastError();  // 160 KB refused: the two-buffer kernel below
+  }
   if (a.d <= 16 && size_t(a.b) * sizeof(T) <= 65536 && size_t(a.b) * sizeof(T) >= 16) {
     const size_t lds = 2 * size_t(a.b) * sizeof(T);
     switch (a.d) {
