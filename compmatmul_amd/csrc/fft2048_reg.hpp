@@ -60,6 +60,9 @@ __device__ __forceinline__ void dft32(float (&re)[R], float (&im)[R]) {
 }
 
 __device__ __forceinline__ int brev5(int q) { return int(__brev(uint32_t(q)) >> 27); }
+__host__ __device__ constexpr int cbrev5(int q) {
+  return ((q & 1) << 4) | ((q & 2) << 2) | (q & 4) | ((q & 8) >> 2) | ((q & 16) >> 4);
+}
 
 // Ordering of this wave's LDS accesses between the phases: a workgroup
 // barrier, or (when only this wave touches S) a compiler barrier, since the
@@ -85,12 +88,29 @@ __device__ __forceinline__ void dft2048_lds(float2* S, const float2* __restrict_
   }
   order<WAVE_LOCAL>();
   dft32(re, im);  // over n2; register q holds k2 = bitrev5(q)
-  // the table address is laundered so the compiler reloads the twiddles
-  // (L1-resident) instead of pinning 64 VGPRs across the k loop
+  // twiddle W_2048^(n1 k2), k2 = j + 8 i: the table (row bitrev5(k2) holds
+  // W_2048^(n1 k2)) gives A[j] = W^(n1 j) and B[i] = W^(n1 8 i), ten loads
+  // per call instead of 32; the other twiddles are products A[j] B[i].  The
+  // table address is laundered so the compiler reloads these (L1-resident)
+  // instead of pinning them across the k loop.
   asm volatile("" : "+v"(t2));
+  float2 A[8], B[4];
+  A[0] = make_float2(1.f, 0.f);
+  B[0] = A[0];
 #pragma unroll
-  for (int q = 0; q < R; ++q) {  // twiddle W_2048^(n1 k2)
-    const float2 w = t2[q * 64 + lane];
+  for (int jj = 1; jj < 8; ++jj) A[jj] = t2[cbrev5(jj) * 64 + lane];
+#pragma unroll
+  for (int ii = 1; ii < 4; ++ii) B[ii] = t2[cbrev5(8 * ii) * 64 + lane];
+#pragma unroll
+  for (int q = 1; q < R; ++q) {  // q = 0 is k2 = 0: W = 1
+    const int k2 = cbrev5(q), jj = k2 & 7, ii = k2 >> 3;
+    float2 w;
+    if (ii == 0)
+      w = A[jj];
+    else if (jj == 0)
+      w = B[ii];
+    else
+      w = make_float2(fmaf(A[jj].x, B[ii].x, -A[jj].y * B[ii].y), fmaf(A[jj].x, B[ii].y, A[jj].y * B[ii].x));
     const float xr = re[q], xi = im[q];
     re[q] = fmaf(xr, w.x, -xi * w.y);
     im[q] = fmaf(xr, w.y, xi * w.x);
