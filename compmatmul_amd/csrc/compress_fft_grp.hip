@@ -116,6 +116,11 @@ __device__ __forceinline__ void fft_grp_lists_body(const RegJob& j, int g) {
       acc += (nr[r] + 63) / 64;
       run[r] = 0;
     }
+    // every key's twiddle W_b^(h s0), all loads in flight at once (inside
+    // the per-key branch below they would be issued one after another)
+    float2 wq[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) wq[q] = tw[(uint64_t(hbucket(ek[q])) * s0) & bmask];
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
       const int r = rk[q];
@@ -124,7 +129,7 @@ __device__ __forceinline__ void fft_grp_lists_body(const RegJob& j, int g) {
       if (r >= 0) {
         const bool isb = base0 + q * 64 + lane >= m;
         const uint32_t h = hbucket(ek[q]);
-        const float2 wv = tw[(uint64_t(h) * s0) & bmask];
+        const float2 wv = wq[q];
         const float sg = hneg(ek[q]) ? -1.f : 1.f;
         coef = isb ? make_float2(-sg * wv.y, sg * wv.x) : make_float2(sg * wv.x, sg * wv.y);
         ent = ((h & (B2 - 1)) << 3) | (uint32_t(q * 64 + lane) << 18);
