@@ -187,11 +187,55 @@ __global__ void __launch_bounds__(NT) k_compress_lds(CompressArgs a) {
 // owns one element and reads every chunk in order: deterministic)
 template <typename T, typename ACC>
 __device__ __forceinline__ void chunk_sum_body(T* __restrict__ P, int64_t elems, int nchunks, int blk) {
+  // chunks are added in order 0, 1, 2, ... (bit-reproducible); the loads of
+  // eight chunks are issued before their additions
+  constexpr int U = 8;
+  if (sizeof(T) == 4 && (elems & 3) == 0) {  // four elements per thread
+    const int64_t idx = (int64_t(blk) * NT + threadIdx.x) * 4;
+    if (idx >= elems) return;
+    ACC s[4] = {ACC(0), ACC(0), ACC(0), ACC(0)};
+    int c = 0;
+    for (; c + U <= nchunks; c += U) {
+      float4 v[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) v[u] = *reinterpret_cast<const float4*>(P + (c + u) * elems + idx);
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        s[0] += ACC(v[u].x);
+        s[1] += ACC(v[u].y);
+        s[2] += ACC(v[u].z);
+        s[3] += ACC(v[u].w);
+      }
+    }
+    for (; c < nchunks; ++c) {
+      const float4 v = *reinterpret_cast<const float4*>(P + c * elems + idx);
+      s[0] += ACC(v.x);
+      s[1] += ACC(v.y);
+      s[2] += ACC(v.z);
+      s[3] += ACC(v.w);
+    }
+    *reinterpret_cast<float4*>(P + idx) = make_float4(float(s[0]), float(s[1]), float(s[2]), float(s[3]));
+    return;
+  }
   const int64_t idx = int64_t(blk) * NT + threadIdx.x;
   if (idx >= elems) return;
   ACC s = ACC(0);
-  for (int c = 0; c < nchunks; ++c) s += ACC(P[c * elems + idx]);
+  int c = 0;
+  for (; c + U <= nchunks; c += U) {
+    T v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) v[u] = P[(c + u) * elems + idx];
+#pragma unroll
+    for (int u = 0; u < U; ++u) s += ACC(v[u]);
+  }
+  for (; c < nchunks; ++c) s += ACC(P[c * elems + idx]);
   P[idx] = T(s);
+}
+
+// blocks of the chunk sum: four elements per thread where the layout allows
+template <typename T>
+__host__ __device__ __forceinline__ int64_t chunk_sum_threads(int64_t elems) {
+  return sizeof(T) == 4 && (elems & 3) == 0 ? elems / 4 : elems;
 }
 
 template <typename T, typename ACC>
@@ -348,7 +392,7 @@ hipError_t reduce_t(const ReduceArgs& a0, hipStream_t s) {
   const bool wide = a.f64_reduce || sizeof(T) == 8;
   if (a.nchunks > 1) {
     const int64_t elems = (int64_t(a.d) << a.lb) * (FWHT ? 1 : 2);
-    const unsigned blocks = unsigned((elems + NT - 1) / NT);
+    const unsigned blocks = unsigned((chunk_sum_threads<T>(elems) + NT - 1) / NT);
     T* P = static_cast<T*>(const_cast<void*>(a.partial));
     if (wide)
       k_chunk_sum<T, double><<<dim3(blocks), dim3(NT), 0, s>>>(P, elems, a.nchunks);
@@ -393,7 +437,7 @@ hipError_t launch_reduce_inverse(const ReduceArgs& a, hipStream_t s) {
 
 int finish_batch_blocks(const FinishJob& j, int* sum_blocks, int* red_blocks) {
   const int64_t elems = (int64_t(j.r.d) << j.r.lb) * (j.r.fwht ? 1 : 2);
-  *sum_blocks = j.r.nchunks > 1 ? int((elems + NT - 1) / NT) : 0;
+  *sum_blocks = j.r.nchunks > 1 ? int((chunk_sum_threads<float>(elems) + NT - 1) / NT) : 0;
   *red_blocks = j.r.d << (j.r.lb - j.r.lb2);
   return int(((int64_t(j.c.d) << j.c.lb_out) + NT - 1) / NT);
 }
