@@ -108,13 +108,15 @@ __device__ __forceinline__ void fft_grp_lists_body(const RegJob& j, int g) {
       }
     }
     __syncthreads();
-    int base[RCAP], run[RCAP];
+    static_assert(RCAP == 8, "round_slot places rounds 0..7");
+    int base[RCAP];
+    int64_t run[RCAP];  // next slot of each round
     int acc = out;
 #pragma unroll
     for (int r = 0; r < RCAP; ++r) {
       base[r] = acc;
+      run[r] = int64_t(acc) * 64;
       acc += (nr[r] + 63) / 64;
-      run[r] = 0;
     }
     // every key's twiddle W_b^(h s0), all loads in flight at once (inside
     // the per-key branch below they would be issued one after another)
@@ -134,15 +136,10 @@ __device__ __forceinline__ void fft_grp_lists_body(const RegJob& j, int g) {
         coef = isb ? make_float2(-sg * wv.y, sg * wv.x) : make_float2(sg * wv.x, sg * wv.y);
         ent = ((h & (B2 - 1)) << 3) | (uint32_t(q * 64 + lane) << 18);
       }
-#pragma unroll
-      for (int rr = 0; rr < RCAP; ++rr) {
-        const uint64_t mask = __ballot(r == rr);
-        if (r == rr) {
-          const int64_t pos = int64_t(base[rr]) * 64 + run[rr] + __popcll(mask & ((1ull << lane) - 1ull));
-          Ls[pos] = ent;
-          Cs[pos] = coef;
-        }
-        run[rr] += __popcll(mask);
+      const int64_t pos = round_slot(r, lane, run);
+      if (r >= 0) {
+        Ls[pos] = ent;
+        Cs[pos] = coef;
       }
     }
 #pragma unroll

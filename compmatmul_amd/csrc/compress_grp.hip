@@ -122,31 +122,24 @@ __device__ __forceinline__ void grp_lists_body(const RegJob& j, int g) {
       }
     }
     __syncthreads();
-    int base[RCAP], run[RCAP];
+    static_assert(RCAP == 8, "round_slot places rounds 0..7");
+    int base[RCAP];
+    int64_t run[RCAP];  // next slot of each round
     int acc = out;
 #pragma unroll
     for (int r = 0; r < RCAP; ++r) {
       base[r] = acc;
+      run[r] = int64_t(acc) * 64;
       acc += (nr[r] + 63) / 64;
-      run[r] = 0;
     }
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
       const int r = rk[q];
-      uint32_t ent = 0;
+      const int64_t pos = round_slot(r, lane, run);
       if (r >= 0) {
         const uint32_t h = hbucket(ek[q]);
         const uint32_t neg = hneg(ek[q]) ^ (__popc((h >> (LB2 + LOG_G)) & group) & 1u);
-        ent = (slot_of<LOG_G>(h & (B2 - 1)) << 2) | (uint32_t(q * 64 + lane) << 18) | (neg ? F_SIGN : 0u);
-      }
-#pragma unroll
-      for (int rr = 0; rr < RCAP; ++rr) {
-        const uint64_t mask = __ballot(r == rr);
-        if (r == rr) {
-          const int pos = run[rr] + __popcll(mask & ((1ull << lane) - 1ull));
-          L[int64_t(base[rr]) * 64 + pos] = ent;
-        }
-        run[rr] += __popcll(mask);
+        L[pos] = (slot_of<LOG_G>(h & (B2 - 1)) << 2) | (uint32_t(q * 64 + lane) << 18) | (neg ? F_SIGN : 0u);
       }
     }
 #pragma unroll
@@ -561,10 +554,11 @@ __device__ __forceinline__ void grp_unit(const RegJob& j, int u, float* S, float
   for (int r = 0; r < R; ++r) out[r * 64 + lane] = Sw[r * 64 + lane];
 }
 
-// Three waves per SIMD: the paired form then spills a few registers, and
-// is still faster than at two waves (measured on fwht 2048^3, b = 2^14).
+// Three waves per SIMD for G = 4 (160 VGPRs, no spill; LDS also allows
+// three workgroups per CU); G = 2 needs more registers per wave and targets
+// two.
 template <int LOG_G, bool PAIRED>
-__global__ void __launch_bounds__(64 << LOG_G) __attribute__((amdgpu_waves_per_eu(3, 3)))
+__global__ void __launch_bounds__(64 << LOG_G) __attribute__((amdgpu_waves_per_eu(LOG_G == 2 ? 3 : 2, LOG_G == 2 ? 3 : 2)))
 k_compress_fwht_grp(RegJob j, int units) {
   __shared__ __align__(16) float S[SLEN << LOG_G];
   __shared__ __align__(16) float Rb[2 * SEG];
@@ -579,7 +573,7 @@ k_compress_fwht_grp(RegJob j, int units) {
 // batch kernels; thread 0 claims each item and the workgroup reads it from
 // LDS.
 template <int LOG_G>
-__global__ void __launch_bounds__(64 << LOG_G) __attribute__((amdgpu_waves_per_eu(3, 3)))
+__global__ void __launch_bounds__(64 << LOG_G) __attribute__((amdgpu_waves_per_eu(LOG_G == 2 ? 3 : 2, LOG_G == 2 ? 3 : 2)))
 k_compress_fwht_grp_batch(const RegJob* __restrict__ jobs, const int2* __restrict__ items,
                   const int* __restrict__ qstart, unsigned* __restrict__ counters) {
   __shared__ __align__(16) float S[SLEN << LOG_G];

@@ -35,6 +35,29 @@ __device__ __forceinline__ uint32_t hneg(uint32_t e) { return e >> 31; }
 template <typename T>
 __device__ __forceinline__ T apply_sign(T v, uint32_t neg) { return neg ? -v : v; }
 
+// Schedule placement of one 64-key chunk (one key per lane) whose keys
+// fall into rounds 0..7 (r < 0: the key is not in this list): returns the
+// slot of this lane's key, run[r] plus the keys of round r in lower lanes,
+// and advances every run[rr] by its round's count in the chunk.  Three
+// ballots on the bits of r and one on validity give every lane the mask of
+// its round; the runs stay wave-uniform.
+__device__ __forceinline__ int64_t round_slot(int r, int lane, int64_t (&run)[8]) {
+  const bool ok = r >= 0;
+  const uint64_t bv = __ballot(ok);
+  const uint64_t b0 = __ballot(ok && (r & 1));
+  const uint64_t b1 = __ballot(ok && (r & 2));
+  const uint64_t b2 = __ballot(ok && (r & 4));
+  uint64_t same = bv & ((r & 1) ? b0 : ~b0) & ((r & 2) ? b1 : ~b1) & ((r & 4) ? b2 : ~b2);
+  const int below = __popcll(same & ((1ull << lane) - 1ull));
+  int64_t mine = run[0];
+#pragma unroll
+  for (int rr = 1; rr < 8; ++rr) mine = r == rr ? run[rr] : mine;
+#pragma unroll
+  for (int rr = 0; rr < 8; ++rr)
+    run[rr] += __popcll(bv & ((rr & 1) ? b0 : ~b0) & ((rr & 2) ? b1 : ~b1) & ((rr & 4) ? b2 : ~b2));
+  return mine + below;
+}
+
 // reverse the low `bits` bits of x (bits >= 1)
 __device__ __forceinline__ uint32_t bitrev(uint32_t x, int bits) {
   return __brev(x) >> (32 - bits);
