@@ -101,7 +101,7 @@ def test_plain_path_matches_fused(variant):
 
 
 # ---------------------------------------------------------------- decompress
-@pytest.mark.parametrize("d", [1, 2, 4, 7, 16, 17, 20])
+@pytest.mark.parametrize("d", list(range(1, 21)))
 def test_median_rank_any_d(d):
     # array.hpp:323-328: nth_element at d/2 (upper median for even d)
     rng = np.random.default_rng(d)
@@ -112,6 +112,23 @@ def test_median_rank_any_d(d):
         vals = sk.values()
         hA, hB = sk.hashes()
         assert np.array_equal(sk.decompress(), orc.decompress(variant, vals, hA, hB))
+
+
+@pytest.mark.parametrize("variant", ["fft", "fwht"])
+@pytest.mark.parametrize("d", [1, 2, 7, 15, 16])
+def test_decompress_f32_64k_rows(variant, d):
+    # FP32 sketch rows of 64 KB (b = 2^14) stream through the half-row LDS
+    # ring; the median is a selection, so the result is bit-exact
+    print(f"{variant} d={d} b=16384 fp32", flush=True)
+    rng = np.random.default_rng(100 + d)
+    AT = rng.standard_normal((24, 700)).astype(np.float32)
+    B = rng.standard_normal((24, 530)).astype(np.float32)
+    sk = cma.compress(variant, AT, B, d, 1 << 14, seed=31 + d, dtype=np.float32)
+    vals = sk.values()
+    hA, hB = sk.hashes()
+    want = orc.decompress(variant, vals, hA, hB)
+    assert np.array_equal(sk.decompress().astype(np.float64), want)
+    sk.free()
 
 
 def test_decompress_into_device_buffer():
