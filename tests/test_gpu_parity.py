@@ -198,7 +198,8 @@ def test_errors_raise():
                                      (1, 5, 3, 1 << 13), (300, 33, 0, 1 << 12),
                                      (60000, 3, 50, 1 << 12), (5000, 20, 4100, 1 << 15),
                                      (300, 40, 200, 1 << 10), (64, 9, 70, 1 << 10),
-                                     (700, 50, 65536, 1 << 16), (4096, 30, 4096, 1 << 13)])
+                                     (700, 50, 65536, 1 << 16), (4096, 30, 4096, 1 << 13),
+                                     (2048, 96, 0, 1 << 14), (0, 7, 40, 1 << 14)])
 def test_fwht_register_kernel_matches_lds_kernel(m, n, p, b):
     print(f"reg vs lds m={m} n={n} p={p} b={b}", flush=True)
     AT, B, _ = workloads.planted(m, n, max(p, 1), 4, 5.0, m + n, np.float32)
@@ -227,7 +228,8 @@ def test_fwht_register_kernel_matches_lds_kernel(m, n, p, b):
                                      (1, 5, 3, 1 << 13), (300, 33, 0, 1 << 12),
                                      (60000, 3, 50, 1 << 11), (5000, 20, 4100, 1 << 16),
                                      (300, 40, 200, 1 << 10), (64, 9, 70, 1 << 10),
-                                     (700, 50, 65000, 1 << 16), (4096, 30, 4096, 1 << 12)])
+                                     (700, 50, 65000, 1 << 16), (4096, 30, 4096, 1 << 12),
+                                     (2048, 96, 0, 1 << 14), (0, 7, 40, 1 << 14)])
 def test_fft_register_kernel_matches_lds_kernel(m, n, p, b):
     print(f"fft reg vs lds m={m} n={n} p={p} b={b}", flush=True)
     AT, B, _ = workloads.planted(m, n, max(p, 1), 4, 5.0, m + n + 1, np.float32)
@@ -245,9 +247,11 @@ def test_fft_register_kernel_matches_lds_kernel(m, n, p, b):
     want = orc.sketch_from_product("fft", AT.astype(np.float64).T @ B.astype(np.float64),
                                    hA, hB, b)
     if p == 0 or m == 0:
-        # the pair packing extracts A*B by cancellation: with B empty the
-        # residue is rounding of |A|^2 (~1e-7 relative), not exactly zero
-        scale = float((AT.astype(np.float64) ** 2).sum(axis=1).max())
+        # the pair packing extracts A*B by cancellation: with one operand
+        # empty the residue is rounding of the other's |.|^2 (~1e-7
+        # relative), not exactly zero
+        scale = max(float((AT.astype(np.float64) ** 2).sum(axis=1).max(initial=0.0)),
+                    float((B.astype(np.float64) ** 2).sum(axis=1).max(initial=0.0)), 1e-30)
         assert np.abs(v1).max() < 1e-6 * scale and np.abs(v2).max() < 1e-6 * scale
         assert np.abs(v3).max() < 1e-6 * scale
         return
