@@ -30,7 +30,16 @@ constexpr int LB2 = 11;          // slice length 2048
 constexpr int B2 = 1 << LB2;
 constexpr int R = B2 / 64;       // values per lane (32)
 constexpr int TROW = 33;         // padded transpose row
-constexpr int SLEN = 64 * TROW;  // LDS floats per partition
+// Floats per partition row and per partition: G = 4 pads rows of 32 to 36
+// floats (16-byte rows, read back with ds_read_b128), G <= 2 to 33.
+template <int LOG_G>
+__host__ __device__ constexpr int prow() {
+  return LOG_G == 2 ? 36 : TROW;
+}
+template <int LOG_G>
+__host__ __device__ constexpr int plen() {
+  return 64 * prow<LOG_G>();
+}
 constexpr int SEG = 2048;        // keys staged at a time
 constexpr int RCAP = 8;
 constexpr int MAXSEG = 32;       // keys <= MAXSEG * SEG = 65536
@@ -42,17 +51,18 @@ __device__ __forceinline__ float flip(float v, uint32_t sgn) {
 }
 
 // Partition layout.  G = 4 stores position l of a partition at
-// (l >> 5) * TROW + (l & 31): 64 rows of 32 positions padded to 33 floats,
-// so both the in-place write of the cross pass and the transposed read are
-// conflict-free; slot 32 of row i is the dummy slot of lane i.  G <= 2 keeps
-// positions in order, dummies at 2048 + lane.
+// (l >> 5) * 36 + (l & 31): 64 rows of 32 positions padded to 36 floats, so
+// the in-place write of the cross pass is conflict-free per half-wave and
+// the transposed read is eight 16-byte reads per lane, each pass of 8 lanes
+// on 32 distinct banks; slot 32 of row i is the dummy slot of lane i.  G <= 2
+// keeps positions in order, dummies at 2048 + lane.
 template <int LOG_G>
 __host__ __device__ __forceinline__ uint32_t slot_of(uint32_t l) {
-  return LOG_G == 2 ? (l >> 5) * TROW + (l & 31) : l;
+  return LOG_G == 2 ? (l >> 5) * prow<2>() + (l & 31) : l;
 }
 template <int LOG_G>
 __host__ __device__ __forceinline__ uint32_t dummy_slot(int lane) {
-  return LOG_G == 2 ? uint32_t(lane) * TROW + 32 : uint32_t(B2 + lane);
+  return LOG_G == 2 ? uint32_t(lane) * prow<2>() + 32 : uint32_t(B2 + lane);
 }
 
 __host__ __device__ __forceinline__ int64_t list_cap(int keys) {
@@ -310,11 +320,11 @@ __device__ __forceinline__ void sketch_transform_grp(const float* __restrict__ r
                                                      float (&x)[R], const uint32_t (&cache)[CA],
                                                      AfterScatter after_scatter) {
   constexpr int G = 1 << LOG_G;
-  float* Sw = S + w * SLEN;
+  float* Sw = S + w * plen<LOG_G>();
   const int tid = w * 64 + lane;
   {
     float4* S4 = reinterpret_cast<float4*>(Sw);
-    constexpr int Z4 = (LOG_G == 2 ? SLEN : B2) / 4;  // float4 to clear
+    constexpr int Z4 = (LOG_G == 2 ? plen<2>() : B2) / 4;  // float4 to clear
 #pragma unroll
     for (int q = 0; q < (Z4 + 63) / 64; ++q)
       if (q * 64 + lane < Z4) S4[q * 64 + lane] = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -357,21 +367,27 @@ __device__ __forceinline__ void sketch_transform_grp(const float* __restrict__ r
     for (int q = 0; q < 8; ++q) {
       const int a = int(slot_of<2>(uint32_t(p0 + q * 64)));
 #pragma unroll
-      for (int hw = 0; hw < 4; ++hw) x[(q << 2) | hw] = S[hw * SLEN + a];
+      for (int hw = 0; hw < 4; ++hw) x[(q << 2) | hw] = S[hw * plen<2>() + a];
     }
     wht_reg(x);
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
       const int a = int(slot_of<2>(uint32_t(p0 + q * 64)));
 #pragma unroll
-      for (int jj = 0; jj < 4; ++jj) S[jj * SLEN + a] = x[(q << 2) | jj];
+      for (int jj = 0; jj < 4; ++jj) S[jj * plen<2>() + a] = x[(q << 2) | jj];
     }
     __syncthreads();  // every wave's region is written
     // wave w now reads its slice: row `lane` = positions 32 lane + r
     {
-      const float* rbase = Sw + lane * TROW;
+      const float4* rbase = reinterpret_cast<const float4*>(Sw + lane * prow<2>());
 #pragma unroll
-      for (int r = 0; r < R; ++r) x[r] = rbase[r];
+      for (int c = 0; c < R / 4; ++c) {
+        const float4 v = rbase[c];
+        x[4 * c] = v.x;
+        x[4 * c + 1] = v.y;
+        x[4 * c + 2] = v.z;
+        x[4 * c + 3] = v.w;
+      }
     }
     wht_reg(x);  // position bits 0..4
 #pragma unroll
@@ -408,7 +424,7 @@ __device__ __forceinline__ void sketch_transform_grp(const float* __restrict__ r
 #pragma unroll
     for (int hw = 0; hw < G; ++hw)
 #pragma unroll
-      for (int q = 0; q < PER; ++q) c[hw][q] = S[hw * SLEN + l0 + q * 64 + lane];
+      for (int q = 0; q < PER; ++q) c[hw][q] = S[hw * plen<LOG_G>() + l0 + q * 64 + lane];
 #pragma unroll
     for (int st = 1; st < G; st <<= 1)
 #pragma unroll
@@ -424,7 +440,7 @@ __device__ __forceinline__ void sketch_transform_grp(const float* __restrict__ r
 #pragma unroll
     for (int hw = 0; hw < G; ++hw)
 #pragma unroll
-      for (int q = 0; q < PER; ++q) S[hw * SLEN + l0 + q * 64 + lane] = c[hw][q];
+      for (int q = 0; q < PER; ++q) S[hw * plen<LOG_G>() + l0 + q * 64 + lane] = c[hw][q];
     __syncthreads();
   }
 #pragma unroll
@@ -544,7 +560,7 @@ __device__ __forceinline__ void grp_unit(const RegJob& j, int u, float* S, float
 #pragma unroll
     for (int r = 0; r < R; ++r) acc[r] = fmaf(xa[r], xb[r], acc[r]);
   }
-  float* Sw = S + w * SLEN;
+  float* Sw = S + w * plen<LOG_G>();
   __syncthreads();
 #pragma unroll
   for (int r = 0; r < R; ++r) Sw[LOG_G == 2 ? f_of4(lane, r) : f_of(lane, r)] = acc[r];
@@ -560,7 +576,7 @@ __device__ __forceinline__ void grp_unit(const RegJob& j, int u, float* S, float
 template <int LOG_G, bool PAIRED>
 __global__ void __launch_bounds__(64 << LOG_G) __attribute__((amdgpu_waves_per_eu(LOG_G == 2 ? 3 : 2, LOG_G == 2 ? 3 : 2)))
 k_compress_fwht_grp(RegJob j, int units) {
-  __shared__ __align__(16) float S[SLEN << LOG_G];
+  __shared__ __align__(16) float S[plen<LOG_G>() << LOG_G];
   __shared__ __align__(16) float Rb[2 * SEG];
   const int G = gridDim.x;
   const int g = blockIdx.x;
@@ -576,7 +592,7 @@ template <int LOG_G>
 __global__ void __launch_bounds__(64 << LOG_G) __attribute__((amdgpu_waves_per_eu(LOG_G == 2 ? 3 : 2, LOG_G == 2 ? 3 : 2)))
 k_compress_fwht_grp_batch(const RegJob* __restrict__ jobs, const int2* __restrict__ items,
                   const int* __restrict__ qstart, unsigned* __restrict__ counters) {
-  __shared__ __align__(16) float S[SLEN << LOG_G];
+  __shared__ __align__(16) float S[plen<LOG_G>() << LOG_G];
   __shared__ __align__(16) float Rb[2 * SEG];
   __shared__ int item;
   const int home = blockIdx.x & 7;
