@@ -358,26 +358,40 @@ __device__ __forceinline__ void sketch_transform_grp(const float* __restrict__ r
   after_scatter();   // the staged row is free again
   if constexpr (LOG_G == 2) {
     // One register pass does the cross-partition WHT together with
-    // position bits 6..8: lane L, register (q << 2) | hw holds position
-    // w 512 + q 64 + L of partition hw.  The results go back to the same
-    // slots (this wave's region, so no other wave is reading them), now as
-    // slice (q << 2 | j) -> row j of the padded layout.
-    const int p0 = w * 512 + lane;
+    // position bits 0..2: lane L, register (c << 2) | hw holds position
+    // w 512 + 8 L + c of partition hw (two 16-byte reads per partition:
+    // the eight positions share a padded row).  The results go back into
+    // this wave's region (rows 16 w .. 16 w + 15 of every partition, which
+    // no other wave reads), permuted for the transposed read: register
+    // (c << 2) | j to row (L >> 5) + 2 c + 16 w, column L & 31 of slice j.
+    const int p0 = w * 512 + 8 * lane;
+    {
+      const int a = int(slot_of<2>(uint32_t(p0)));
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      const int a = int(slot_of<2>(uint32_t(p0 + q * 64)));
-#pragma unroll
-      for (int hw = 0; hw < 4; ++hw) x[(q << 2) | hw] = S[hw * plen<2>() + a];
+      for (int hw = 0; hw < 4; ++hw) {
+        const float4* src = reinterpret_cast<const float4*>(S + hw * plen<2>() + a);
+        const float4 u = src[0], v = src[1];
+        x[(0 << 2) | hw] = u.x;
+        x[(1 << 2) | hw] = u.y;
+        x[(2 << 2) | hw] = u.z;
+        x[(3 << 2) | hw] = u.w;
+        x[(4 << 2) | hw] = v.x;
+        x[(5 << 2) | hw] = v.y;
+        x[(6 << 2) | hw] = v.z;
+        x[(7 << 2) | hw] = v.w;
+      }
     }
     wht_reg(x);
+    {
+      const int wb = ((lane >> 5) + 16 * w) * prow<2>() + (lane & 31);
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      const int a = int(slot_of<2>(uint32_t(p0 + q * 64)));
+      for (int c = 0; c < 8; ++c)
 #pragma unroll
-      for (int jj = 0; jj < 4; ++jj) S[jj * plen<2>() + a] = x[(q << 2) | jj];
+        for (int jj = 0; jj < 4; ++jj) S[jj * plen<2>() + wb + 2 * c * prow<2>()] = x[(c << 2) | jj];
     }
     __syncthreads();  // every wave's region is written
-    // wave w now reads its slice: row `lane` = positions 32 lane + r
+    // wave w now reads its slice: row `lane` holds positions l3..l7 of
+    // (l8, f0, f1, f2, l9, l10) = lane bits 0..5
     {
       const float4* rbase = reinterpret_cast<const float4*>(Sw + lane * prow<2>());
 #pragma unroll
@@ -389,7 +403,7 @@ __device__ __forceinline__ void sketch_transform_grp(const float* __restrict__ r
         x[4 * c + 3] = v.w;
       }
     }
-    wht_reg(x);  // position bits 0..4
+    wht_reg(x);  // position bits 3..7
 #pragma unroll
     for (int jj = 0; jj < R / 2; ++jj) {  // position bit 10 (lane bit 5)
       const auto pq = __builtin_amdgcn_permlane32_swap(__float_as_uint(x[2 * jj]),
@@ -407,7 +421,7 @@ __device__ __forceinline__ void sketch_transform_grp(const float* __restrict__ r
       x[r] = p + q;
       x[r | 2] = p - q;
     }
-    // position bit 5 (lane bit 0): x[L ^ 1] s + x[L] with s = -1 on odd
+    // position bit 8 (lane bit 0): x[L ^ 1] s + x[L] with s = -1 on odd
     // lanes, which therefore hold the negated value; A and B carry the same
     // signs, so their product is unchanged
     const float sgn = (lane & 1) ? -1.f : 1.f;
@@ -478,11 +492,11 @@ __device__ __forceinline__ int f_of(int L, int r) {
 }
 
 // G = 4: frequency of lane L, register r after the merged cross pass
-// (register bits 0, 1 -> f10, f9; 2..4 -> f2..f4; lane bit 0 -> f5,
-// 1..3 -> f6..f8, 4 -> f1, 5 -> f0)
+// (register bits 0, 1 -> f10, f9; 2..4 -> f5..f7; lane bit 0 -> f8,
+// 1..3 -> f0..f2, 4 -> f4, 5 -> f3)
 __device__ __forceinline__ int f_of4(int L, int r) {
-  return ((r & 1) << 10) | (((r >> 1) & 1) << 9) | (r & 0x1c) | ((L & 15) << 5) | (((L >> 4) & 1) << 1) |
-         ((L >> 5) & 1);
+  return ((r & 1) << 10) | (((r >> 1) & 1) << 9) | (((r >> 2) & 7) << 5) | ((L & 1) << 8) |
+         ((L >> 1) & 7) | (((L >> 4) & 1) << 4) | (((L >> 5) & 1) << 3);
 }
 
 // Start copying row[0, len) (len <= SEG) into LDS dst[0, SEG) with
