@@ -12,8 +12,8 @@
 //      1. per lane, DFT-32 over the registers (radix-2 DIF, compile-time
 //         twiddles; register q then holds k2 = bitrev5(q));
 //      2. twiddle by W_2048^(n1 k2) (per-lane table, L1-resident);
-//      3. transpose through LDS (rows of 65 complex: conflict-free, one base
-//         address plus immediates): lane L' gets k2 = L' & 31 and the 32
+//      3. transpose through LDS (rows of 66 complex: 16-byte aligned, read
+//         as conflict-free b128 pairs from one base address plus immediates): lane L' gets k2 = L' & 31 and the 32
 //         values n1 = r' + 32 (L' >> 5);
 //      4. DFT-64 over n1: one radix-2 DIF stage across lanes L' and L'+32
 //         with v_permlane32_swap, a second swap that regroups the halves,
@@ -35,7 +35,7 @@ constexpr int LB2 = 11;          // slice length 2048 (complex)
 constexpr int B2 = 1 << LB2;
 using f2k::R;
 using f2k::fq_of;
-constexpr int SLEN = 2112;       // complex slots per wave: transpose (32 x 65) and dummy slots
+constexpr int SLEN = 2112;       // complex slots per wave: transpose (32 x 66) and dummy slots
 constexpr int SEG = 1024;
 constexpr int RCAP = 8;
 constexpr int MAXSEG = 128;      // m + p <= MAXSEG * SEG = 131072

@@ -7,8 +7,8 @@
 //   1. per lane, DFT-32 over the registers (radix-2 DIF, compile-time
 //      twiddles; register q then holds k2 = bitrev5(q));
 //   2. twiddle by W_2048^(n1 k2) (per-lane table, L1-resident);
-//   3. transpose through LDS (rows of 65 complex: conflict-free, one base
-//      address plus immediates): lane L' gets k2 = L' & 31 and the 32 values
+//   3. transpose through LDS (rows of 66 complex: 16-byte aligned, read
+//      as conflict-free b128 pairs from one base address plus immediates): lane L' gets k2 = L' & 31 and the 32 values
 //      n1 = r' + 32 (L' >> 5);
 //   4. DFT-64 over n1: one radix-2 DIF stage across lanes L' and L'+32 with
 //      v_permlane32_swap, a second swap that regroups the halves, then a
@@ -20,7 +20,10 @@ namespace cm {
 namespace f2k {
 
 constexpr int R = 32;     // complex values per lane
-constexpr int TROW = 65;  // padded transpose row (complex)
+// transpose rows of 64 complex padded to 66 (32 x 66 = 2112 slots, the
+// partition size): 16-byte aligned, so the transposed read is b128 pairs,
+// each 8-lane pass on 32 distinct banks; the row writes stay conflict-free
+constexpr int TROW = 66;
 
 // exp(-2 pi i j / N), rounded to float
 constexpr float W32R[32] = {1.000000000e+00f, 9.807852804e-01f, 9.238795325e-01f, 8.314696123e-01f, 7.071067812e-01f, 5.555702330e-01f, 3.826834324e-01f, 1.950903220e-01f, 0.0f, -1.950903220e-01f, -3.826834324e-01f, -5.555702330e-01f, -7.071067812e-01f, -8.314696123e-01f, -9.238795325e-01f, -9.807852804e-01f, -1.000000000e+00f, -9.807852804e-01f, -9.238795325e-01f, -8.314696123e-01f, -7.071067812e-01f, -5.555702330e-01f, -3.826834324e-01f, -1.950903220e-01f, 0.0f, 1.950903220e-01f, 3.826834324e-01f, 5.555702330e-01f, 7.071067812e-01f, 8.314696123e-01f, 9.238795325e-01f, 9.807852804e-01f};
@@ -120,12 +123,14 @@ __device__ __forceinline__ void dft2048_lds(float2* S, const float2* __restrict_
   for (int q = 0; q < R; ++q) S[brev5(q) * TROW + lane] = make_float2(re[q], im[q]);
   order<WAVE_LOCAL>();
   {
-    const float2* rbase = S + (lane & 31) * TROW + 32 * (lane >> 5);
+    const float4* rbase = reinterpret_cast<const float4*>(S + (lane & 31) * TROW + 32 * (lane >> 5));
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-      const float2 z = rbase[r];
+    for (int r = 0; r < R; r += 2) {
+      const float4 z = rbase[r / 2];
       re[r] = z.x;
       im[r] = z.y;
+      re[r + 1] = z.z;
+      im[r + 1] = z.w;
     }
   }
   order<WAVE_LOCAL>();
