@@ -307,9 +307,49 @@ __device__ __forceinline__ void scatter_cached(const uint32_t (&c)[CA],
   }
 }
 
+// The last WHT stage pairs lane L with lane L ^ 32: v_permlane32_swap brings
+// each pair (p, q) into registers 2jj, 2jj + 1 of one lane.  Its butterfly
+// (p + q, p - q) is not applied: the product of the A and B pairs is
+// accumulated in bilinear form instead (accumulate_pairs), four fmas per
+// pair where transforming both operands and multiplying takes six
+// operations.
+__device__ __forceinline__ void swap_halves(float (&x)[R]) {
+#pragma unroll
+  for (int jj = 0; jj < R / 2; ++jj) {
+    const auto pq = __builtin_amdgcn_permlane32_swap(__float_as_uint(x[2 * jj]),
+                                                     __float_as_uint(x[2 * jj + 1]), false, false);
+    x[2 * jj] = __uint_as_float(pq[0]);
+    x[2 * jj + 1] = __uint_as_float(pq[1]);
+  }
+}
+
+// For pairs (pa, qa) and (pb, qb) before their last butterfly,
+// (pa + qa)(pb + qb) = Sum + Dif and (pa - qa)(pb - qb) = Sum - Dif with
+// Sum = pa pb + qa qb and Dif = pa qb + qa pb: acc[2jj] accumulates Sum and
+// acc[2jj + 1] Dif, and finish_pairs turns them into the two products.
+__device__ __forceinline__ void accumulate_pairs(const float (&a)[R], const float (&b)[R],
+                                                 float (&acc)[R]) {
+#pragma unroll
+  for (int r = 0; r < R; r += 2) {
+    acc[r] = fmaf(a[r + 1], b[r + 1], fmaf(a[r], b[r], acc[r]));
+    acc[r + 1] = fmaf(a[r + 1], b[r], fmaf(a[r], b[r + 1], acc[r + 1]));
+  }
+}
+
+__device__ __forceinline__ void finish_pairs(float (&acc)[R]) {
+#pragma unroll
+  for (int r = 0; r < R; r += 2) {
+    const float sum = acc[r], dif = acc[r + 1];
+    acc[r] = sum + dif;
+    acc[r + 1] = sum - dif;
+  }
+}
+
 // Scatter one row for the G slices of the group, cross-partition WHT, then
-// the register WHT of this wave's slice.  On return lane L, register r holds
-// frequency f_of(L, r) of slice group*G + w.
+// the register WHT of this wave's slice.  On return lane L holds slice
+// group*G + w with every butterfly applied but the last: the butterfly of
+// registers (r, r + 1), r even, gives frequencies f_of(L, r) and
+// f_of(L, r + 1) (f_of4 for G = 4).
 // With `staged`, the row (one segment) is already in Rb; `after_scatter`
 // runs once every wave has finished reading it.
 template <int LOG_G, bool CACHED, typename AfterScatter>
@@ -318,7 +358,7 @@ __device__ __forceinline__ void sketch_transform_grp(const float* __restrict__ r
                                                      const int* __restrict__ segcnt, float* S,
                                                      float* Rb, bool staged, int w, int lane,
                                                      float (&x)[R], const uint32_t (&cache)[CA],
-                                                     AfterScatter after_scatter) {
+                                                     AfterScatter after_scatter, bool drain) {
   constexpr int G = 1 << LOG_G;
   float* Sw = S + w * plen<LOG_G>();
   const int tid = w * 64 + lane;
@@ -389,6 +429,9 @@ __device__ __forceinline__ void sketch_transform_grp(const float* __restrict__ r
 #pragma unroll
         for (int jj = 0; jj < 4; ++jj) S[jj * plen<2>() + wb + 2 * c * prow<2>()] = x[(c << 2) | jj];
     }
+    // with `drain`, this wave's row copies have landed too, so the barrier
+    // also publishes the next rows
+    if (drain) __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();  // every wave's region is written
     // wave w now reads its slice: row `lane` holds positions l3..l7 of
     // (l8, f0, f1, f2, l9, l10) = lane bits 0..5
@@ -405,14 +448,6 @@ __device__ __forceinline__ void sketch_transform_grp(const float* __restrict__ r
     }
     wht_reg(x);  // position bits 3..7
 #pragma unroll
-    for (int jj = 0; jj < R / 2; ++jj) {  // position bit 10 (lane bit 5)
-      const auto pq = __builtin_amdgcn_permlane32_swap(__float_as_uint(x[2 * jj]),
-                                                       __float_as_uint(x[2 * jj + 1]), false, false);
-      const float p = __uint_as_float(pq[0]), q = __uint_as_float(pq[1]);
-      x[2 * jj] = p + q;
-      x[2 * jj + 1] = p - q;
-    }
-#pragma unroll
     for (int r = 0; r < R; ++r) {  // position bit 9 (lane bit 4)
       if (r & 2) continue;
       const auto pq = __builtin_amdgcn_permlane16_swap(__float_as_uint(x[r]), __float_as_uint(x[r | 2]),
@@ -427,6 +462,10 @@ __device__ __forceinline__ void sketch_transform_grp(const float* __restrict__ r
     const float sgn = (lane & 1) ? -1.f : 1.f;
 #pragma unroll
     for (int r = 0; r < R; r += 8) dpp_butterfly8(x + r, sgn);
+    // position bit 10 (lane bit 5): the swap brings each pair into one lane;
+    // its butterfly is left to the caller (lanes L and L ^ 32 have the same
+    // parity, so the pair carries one sign)
+    swap_halves(x);
     return;
   }
   if constexpr (G > 1) {
@@ -455,6 +494,7 @@ __device__ __forceinline__ void sketch_transform_grp(const float* __restrict__ r
     for (int hw = 0; hw < G; ++hw)
 #pragma unroll
       for (int q = 0; q < PER; ++q) S[hw * plen<LOG_G>() + l0 + q * 64 + lane] = c[hw][q];
+    if (drain) __builtin_amdgcn_s_waitcnt(0);  // as above
     __syncthreads();
   }
 #pragma unroll
@@ -477,14 +517,7 @@ __device__ __forceinline__ void sketch_transform_grp(const float* __restrict__ r
   }
   asm volatile("" ::: "memory");
   wht_reg(x);  // index bits 0..4
-#pragma unroll
-  for (int jj = 0; jj < R / 2; ++jj) {  // index bit 5 across lane halves
-    const auto pq = __builtin_amdgcn_permlane32_swap(__float_as_uint(x[2 * jj]),
-                                                     __float_as_uint(x[2 * jj + 1]), false, false);
-    const float p = __uint_as_float(pq[0]), q = __uint_as_float(pq[1]);
-    x[2 * jj] = p + q;
-    x[2 * jj + 1] = p - q;
-  }
+  swap_halves(x);  // index bit 5 across lane halves; butterfly left to the caller
 }
 
 __device__ __forceinline__ int f_of(int L, int r) {
@@ -533,6 +566,7 @@ __device__ __forceinline__ void grp_unit(const RegJob& j, int u, float* S, float
   const int chunk = u / per_chunk;
   const int tg = u - chunk * per_chunk;  // t * (b1 / G) + group
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  static_assert(LOG_G >= 1, "the cross pass barrier publishes the rows of k + 1");
   const int ts = tg * G + w;              // t * b1 + slice
   const int k0 = chunk * j.kper;
   const int k1 = min(j.n, k0 + j.kper);
@@ -554,26 +588,27 @@ __device__ __forceinline__ void grp_unit(const RegJob& j, int u, float* S, float
   if (paired && k0 < k1) {
     copy_row_async<G>(j.AT + int64_t(k0) * j.ld_at, j.m, Rb, tid);
     copy_row_async<G>(j.B + int64_t(k0) * j.ld_b, j.p, Rb + SEG, tid);
+    __builtin_amdgcn_s_waitcnt(0);  // this wave's copies of rows k0 have landed
+    __syncthreads();                // and every wave's
   }
   for (int k = k0; k < k1; ++k) {
     float xa[R], xb[R];
     const float* arow = j.AT + int64_t(k) * j.ld_at;
     const float* brow = j.B + int64_t(k) * j.ld_b;
-    if constexpr (paired) {
-      __builtin_amdgcn_s_waitcnt(0);  // this wave's copies of rows k have landed
-      __syncthreads();                // and every wave's
-    }
+    // rows k + 1 are copied during k: A's after A's scatter, B's after B's;
+    // B's cross pass drains both before its barrier, which thereby also
+    // publishes them for the scatters of k + 1
     const bool more = paired && k + 1 < k1;
     sketch_transform_grp<LOG_G, paired>(arow, j.m, la, sca, S, Rb, paired, w, lane, xa, ca, [&] {
       if (more) copy_row_async<G>(arow + j.ld_at, j.m, Rb, tid);
-    });
+    }, false);
     sketch_transform_grp<LOG_G, paired>(brow, j.p, lb, scb, S, paired ? Rb + SEG : Rb, paired, w, lane, xb,
                                         cb, [&] {
                                           if (more) copy_row_async<G>(brow + j.ld_b, j.p, Rb + SEG, tid);
-                                        });
-#pragma unroll
-    for (int r = 0; r < R; ++r) acc[r] = fmaf(xa[r], xb[r], acc[r]);
+                                        }, more);
+    accumulate_pairs(xa, xb, acc);
   }
+  finish_pairs(acc);
   float* Sw = S + w * plen<LOG_G>();
   __syncthreads();
 #pragma unroll
