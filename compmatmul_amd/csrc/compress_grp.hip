@@ -91,8 +91,10 @@ __host__ __device__ __forceinline__ Layout layout(const RegJob& j) {
 
 // Schedule of list g (g < lists: A's row for slice ts = g, else B's): the
 // keys of partition w = slice mod G, bucket l = h mod 2048, sign modulated
-// by the group bits.  Entry: l << 2 (or a dummy slot 2048 + lane) | key
-// offset in the 2048-key segment << 18 | sign << 31.
+// by the group bits.  Entry: byte offset of the destination in the
+// workgroup's partitions, (w plen + slot_of(l)) << 2 (or w plen + the dummy
+// slot of the lane; below 2^16) | key offset in the 2048-key segment << 18
+// | sign << 31.
 template <int LOG_G>
 __device__ __forceinline__ void grp_lists_body(const RegJob& j, int g) {
   constexpr int G = 1 << LOG_G;
@@ -105,6 +107,8 @@ __device__ __forceinline__ void grp_lists_body(const RegJob& j, int g) {
   const int t = ts >> j.lbs;
   const uint32_t slice = uint32_t(ts & ((1 << j.lbs) - 1));
   const uint32_t w = slice & (G - 1), group = slice >> LOG_G;
+  const uint32_t pbase = w * plen<LOG_G>();
+  static_assert((plen<LOG_G>() << (LOG_G + 2)) <= (1 << 16), "destinations fit 16 bits");
   const int keys = side_b ? j.p : j.m;
   const uint32_t* __restrict__ hh = (side_b ? j.hB : j.hA) + int64_t(t) * keys;
   uint32_t* __restrict__ L = side_b ? lay.LB + ts * lay.cap_b : lay.LA + ts * lay.cap_a;
@@ -149,14 +153,15 @@ __device__ __forceinline__ void grp_lists_body(const RegJob& j, int g) {
       if (r >= 0) {
         const uint32_t h = hbucket(ek[q]);
         const uint32_t neg = hneg(ek[q]) ^ (__popc((h >> (LB2 + LOG_G)) & group) & 1u);
-        L[pos] = (slot_of<LOG_G>(h & (B2 - 1)) << 2) | (uint32_t(q * 64 + lane) << 18) | (neg ? F_SIGN : 0u);
+        L[pos] = ((pbase + slot_of<LOG_G>(h & (B2 - 1))) << 2) | (uint32_t(q * 64 + lane) << 18) |
+                 (neg ? F_SIGN : 0u);
       }
     }
 #pragma unroll
     for (int rr = 0; rr < RCAP; ++rr) {  // pad rounds to whole groups (dummy slots)
       const int full = (nr[rr] + 63) / 64 * 64;
       for (int slot = nr[rr] + lane; slot < full; slot += 64)
-        L[int64_t(base[rr]) * 64 + slot] = dummy_slot<LOG_G>(slot & 63) << 2;
+        L[int64_t(base[rr]) * 64 + slot] = (pbase + dummy_slot<LOG_G>(slot & 63)) << 2;
     }
     if (lane < RCAP) {
       const int64_t row = int64_t(g) * MAXSEG + s0 / SEG;
@@ -241,7 +246,7 @@ __device__ __forceinline__ void scatter_section(const uint32_t* __restrict__ lp,
 
 // Schedule groups kept in registers for a whole unit: the schedule of a
 // (t, slice) list does not depend on k.  Slots past the list hold dummy
-// entries (plain writes of Rb[0] into the pad slot 2048 + lane).
+// entries (plain writes of Rb[0] into the lane's dummy slot of partition w).
 constexpr int CA = 12;
 
 // An empty operand (keys == 0) has no schedule: its counts are never
@@ -249,41 +254,59 @@ constexpr int CA = 12;
 template <int LOG_G>
 __device__ __forceinline__ void load_cache(const uint32_t* __restrict__ lp,
                                            const int* __restrict__ nr, int keys, uint32_t (&c)[CA],
-                                           int lane) {
+                                           int w, int lane) {
   int ng = 0;
   if (keys > 0)
     for (int r = 0; r < RCAP; ++r) ng += nr[r];
 #pragma unroll
-  for (int g = 0; g < CA; ++g) c[g] = g < ng ? lp[g * 64 + lane] : dummy_slot<LOG_G>(lane) << 2;
+  for (int g = 0; g < CA; ++g)
+    c[g] = g < ng ? lp[g * 64 + lane] : (w * plen<LOG_G>() + dummy_slot<LOG_G>(lane)) << 2;
+}
+
+// Section ends of a single-segment list in groups: plain writes below n0,
+// read-add-writes below e1, atomics below ng.  They depend on the list only,
+// so they are read once per unit, into scalar registers.
+struct GroupCounts {
+  int n0, e1, ng;
+};
+
+__device__ __forceinline__ GroupCounts group_counts(const int* __restrict__ nr, int keys) {
+  GroupCounts gc{0, 0, 0};
+  if (keys > 0) {
+    gc.n0 = __builtin_amdgcn_readfirstlane(nr[0]);
+    int e1 = gc.n0;
+    for (int r = 1; r < RCAP - 1; ++r) e1 += nr[r];
+    gc.e1 = __builtin_amdgcn_readfirstlane(e1);
+    gc.ng = __builtin_amdgcn_readfirstlane(e1 + nr[RCAP - 1]);
+  }
+  return gc;
 }
 
 // The scatter of one single-segment list from the cached groups, then any
 // groups past the cache from memory.  Group g is a plain write below nr[0],
 // a read-add-write below nr[0] + (rounds 1 .. RCAP-2), else an LDS atomic;
 // one wave's LDS operations complete in order, as the rounds require.
-__device__ __forceinline__ void scatter_cached(const uint32_t (&c)[CA],
+__device__ __forceinline__ void scatter_cached(uint32_t (&c)[CA],
                                                const uint32_t* __restrict__ lp,
-                                               const int* __restrict__ nr, int keys, float* S,
+                                               const GroupCounts& gc, float* S,
                                                const float* Rb, int lane) {
-  const bool any = keys > 0;
-  const int n0 = any ? nr[0] : 0;
-  int e1 = n0;
-  if (any)
-    for (int r = 1; r < RCAP - 1; ++r) e1 += nr[r];
-  const int ng = any ? e1 + nr[RCAP - 1] : 0;
+  // opaque here, so each group's tests stay scalar compares in the loop
+  // instead of being hoisted out as 24 live condition masks
+  int n0 = gc.n0, e1 = gc.e1, ng = gc.ng;
+  asm volatile("" : "+s"(n0), "+s"(e1), "+s"(ng));
   const char* Rbb = reinterpret_cast<const char*>(Rb);
   char* Sb = reinterpret_cast<char*>(S);
   constexpr int BATCH = 4;  // LDS reads in flight per batch
 #pragma unroll
   for (int g0 = 0; g0 < CA; g0 += BATCH) {
-    // opaque copies: the fields are extracted here, not hoisted out of the
-    // k loop as three live registers per entry
+    // the entries are made opaque in place, so their fields are extracted
+    // here rather than hoisted out of the k loop as three live registers
     uint32_t e[BATCH];
     float v[BATCH];
 #pragma unroll
     for (int q = 0; q < BATCH; ++q) {
+      asm volatile("" : "+v"(c[g0 + q]));
       e[q] = c[g0 + q];
-      asm volatile("" : "+v"(e[q]));
       v[q] = flip(*reinterpret_cast<const float*>(Rbb + ((e[q] >> 16) & 0x1ffcu)), e[q] & F_SIGN);
     }
 #pragma unroll
@@ -357,7 +380,8 @@ __device__ __forceinline__ void sketch_transform_grp(const float* __restrict__ r
                                                      const uint32_t* __restrict__ list,
                                                      const int* __restrict__ segcnt, float* S,
                                                      float* Rb, bool staged, int w, int lane,
-                                                     float (&x)[R], const uint32_t (&cache)[CA],
+                                                     float (&x)[R], uint32_t (&cache)[CA],
+                                                     const GroupCounts& gc,
                                                      AfterScatter after_scatter, bool drain) {
   constexpr int G = 1 << LOG_G;
   float* Sw = S + w * plen<LOG_G>();
@@ -371,7 +395,7 @@ __device__ __forceinline__ void sketch_transform_grp(const float* __restrict__ r
   }
   const int nseg = (keys + SEG - 1) / SEG;
   const uint32_t* __restrict__ lp = list;
-  if constexpr (CACHED) scatter_cached(cache, list, segcnt, keys, Sw, Rb, lane);  // staged, one segment
+  if constexpr (CACHED) scatter_cached(cache, list, gc, S, Rb, lane);  // staged, one segment
   for (int seg = 0; seg < (CACHED ? 0 : nseg); ++seg) {
     if (!staged) {
       const int s0 = seg * SEG;
@@ -387,11 +411,11 @@ __device__ __forceinline__ void sketch_transform_grp(const float* __restrict__ r
     const int* __restrict__ nr = segcnt + seg * RCAP;
     int nw = 0;
     for (int r = 1; r < RCAP - 1; ++r) nw += nr[r];
-    scatter_section<0>(lp, nr[0], Sw, Rb, lane);
+    scatter_section<0>(lp, nr[0], S, Rb, lane);
     lp += int64_t(nr[0]) * 64;
-    scatter_section<1>(lp, nw, Sw, Rb, lane);
+    scatter_section<1>(lp, nw, S, Rb, lane);
     lp += int64_t(nw) * 64;
-    scatter_section<2>(lp, nr[RCAP - 1], Sw, Rb, lane);
+    scatter_section<2>(lp, nr[RCAP - 1], S, Rb, lane);
     lp += int64_t(nr[RCAP - 1]) * 64;
   }
   __syncthreads();  // all partitions complete
@@ -565,7 +589,9 @@ __device__ __forceinline__ void grp_unit(const RegJob& j, int u, float* S, float
   const int per_chunk = lay.lists >> LOG_G;
   const int chunk = u / per_chunk;
   const int tg = u - chunk * per_chunk;  // t * (b1 / G) + group
-  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  // the wave index is uniform: in a scalar register, the schedule counts
+  // derived from it are scalar and each group's kind is a scalar branch
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   static_assert(LOG_G >= 1, "the cross pass barrier publishes the rows of k + 1");
   const int ts = tg * G + w;              // t * b1 + slice
   const int k0 = chunk * j.kper;
@@ -581,9 +607,12 @@ __device__ __forceinline__ void grp_unit(const RegJob& j, int u, float* S, float
   const int tid = threadIdx.x;
   // paired lists are single-segment: keep their first groups in registers
   uint32_t ca[CA], cb[CA];
+  GroupCounts ga{0, 0, 0}, gb{0, 0, 0};
   if constexpr (paired) {
-    load_cache<LOG_G>(la, sca, j.m, ca, lane);
-    load_cache<LOG_G>(lb, scb, j.p, cb, lane);
+    load_cache<LOG_G>(la, sca, j.m, ca, w, lane);
+    load_cache<LOG_G>(lb, scb, j.p, cb, w, lane);
+    ga = group_counts(sca, j.m);
+    gb = group_counts(scb, j.p);
   }
   if (paired && k0 < k1) {
     copy_row_async<G>(j.AT + int64_t(k0) * j.ld_at, j.m, Rb, tid);
@@ -599,11 +628,11 @@ __device__ __forceinline__ void grp_unit(const RegJob& j, int u, float* S, float
     // B's cross pass drains both before its barrier, which thereby also
     // publishes them for the scatters of k + 1
     const bool more = paired && k + 1 < k1;
-    sketch_transform_grp<LOG_G, paired>(arow, j.m, la, sca, S, Rb, paired, w, lane, xa, ca, [&] {
+    sketch_transform_grp<LOG_G, paired>(arow, j.m, la, sca, S, Rb, paired, w, lane, xa, ca, ga, [&] {
       if (more) copy_row_async<G>(arow + j.ld_at, j.m, Rb, tid);
     }, false);
     sketch_transform_grp<LOG_G, paired>(brow, j.p, lb, scb, S, paired ? Rb + SEG : Rb, paired, w, lane, xb,
-                                        cb, [&] {
+                                        cb, gb, [&] {
                                           if (more) copy_row_async<G>(brow + j.ld_b, j.p, Rb + SEG, tid);
                                         }, more);
     accumulate_pairs(xa, xb, acc);
