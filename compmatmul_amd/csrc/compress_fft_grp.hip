@@ -285,7 +285,13 @@ __device__ __forceinline__ void fft_grp_unit(const RegJob& j, int u, float2* S, 
   const int ts0 = u - chunk * per_chunk;            // t * (b1/G) + s0
   const int t = ts0 / (b1 >> LOG_G);
   const int s0 = ts0 - t * (b1 >> LOG_G);
-  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  // The wave index and the section counts are uniform.  The batch kernel
+  // (register staging) keeps them in scalar registers, so its scatter loops
+  // run under scalar control: 2% faster on a 512-product batch.  The same
+  // form measured 4% slower in the single-product kernel, which keeps
+  // per-lane loop control.
+  const int w = ASYNC ? int(threadIdx.x >> 6) : __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
   const int tid = threadIdx.x;
   const int slice = s0 + w * (b1 >> LOG_G);
   const int g = ts0 * G + w;                        // this wave's list
@@ -341,12 +347,15 @@ __device__ __forceinline__ void fft_grp_unit(const RegJob& j, int u, float2* S, 
       const int* __restrict__ nr = sc + seg * RCAP;
       int nw = 0;
       for (int r = 1; r < RCAP - 1; ++r) nw += nr[r];
-      scatter_section<0>(ls + off, cs + off, nr[0], Sw, Rb, lane);
-      off += int64_t(nr[0]) * 64;
-      scatter_section<1>(ls + off, cs + off, nw, Sw, Rb, lane);
-      off += int64_t(nw) * 64;
-      scatter_section<2>(ls + off, cs + off, nr[RCAP - 1], Sw, Rb, lane);
-      off += int64_t(nr[RCAP - 1]) * 64;
+      const int n0 = ASYNC ? nr[0] : __builtin_amdgcn_readfirstlane(nr[0]);
+      const int n1 = ASYNC ? nw : __builtin_amdgcn_readfirstlane(nw);
+      const int n2 = ASYNC ? nr[RCAP - 1] : __builtin_amdgcn_readfirstlane(nr[RCAP - 1]);
+      scatter_section<0>(ls + off, cs + off, n0, Sw, Rb, lane);
+      off += int64_t(n0) * 64;
+      scatter_section<1>(ls + off, cs + off, n1, Sw, Rb, lane);
+      off += int64_t(n1) * 64;
+      scatter_section<2>(ls + off, cs + off, n2, Sw, Rb, lane);
+      off += int64_t(n2) * 64;
     }
     __syncthreads();  // all partitions complete, and every wave is done reading Rb
     if (ASYNC && nseg > 0 && k + 1 < k1)

@@ -411,12 +411,15 @@ __device__ __forceinline__ void sketch_transform_grp(const float* __restrict__ r
     const int* __restrict__ nr = segcnt + seg * RCAP;
     int nw = 0;
     for (int r = 1; r < RCAP - 1; ++r) nw += nr[r];
-    scatter_section<0>(lp, nr[0], S, Rb, lane);
-    lp += int64_t(nr[0]) * 64;
-    scatter_section<1>(lp, nw, S, Rb, lane);
-    lp += int64_t(nw) * 64;
-    scatter_section<2>(lp, nr[RCAP - 1], S, Rb, lane);
-    lp += int64_t(nr[RCAP - 1]) * 64;
+    const int n0 = __builtin_amdgcn_readfirstlane(nr[0]);  // uniform: scalar loop control
+    const int n1 = __builtin_amdgcn_readfirstlane(nw);
+    const int n2 = __builtin_amdgcn_readfirstlane(nr[RCAP - 1]);
+    scatter_section<0>(lp, n0, S, Rb, lane);
+    lp += int64_t(n0) * 64;
+    scatter_section<1>(lp, n1, S, Rb, lane);
+    lp += int64_t(n1) * 64;
+    scatter_section<2>(lp, n2, S, Rb, lane);
+    lp += int64_t(n2) * 64;
   }
   __syncthreads();  // all partitions complete
   after_scatter();   // the staged row is free again
@@ -681,11 +684,17 @@ k_compress_fwht_grp_batch(const RegJob* __restrict__ jobs, const int2* __restric
     for (;;) {
       if (threadIdx.x == 0) item = int(atomicAdd(&counters[q], 1u));
       __syncthreads();
-      const int it = item;
+      const int it = __builtin_amdgcn_readfirstlane(item);  // uniform: the job lives in scalar registers
       __syncthreads();  // every thread has read `item` before it is rewritten
       if (it >= len) break;
       const int2 e = items[base + it];
-      grp_unit<LOG_G, false>(jobs[e.x], e.y, S, Rb);
+      // products whose rows fit one segment take the paired path (rows
+      // copied ahead, schedules in registers), the others stage segments
+      const RegJob& jb = jobs[e.x];
+      if (jb.m <= SEG && jb.p <= SEG)
+        grp_unit<LOG_G, true>(jb, e.y, S, Rb);
+      else
+        grp_unit<LOG_G, false>(jb, e.y, S, Rb);
       __syncthreads();
     }
   }
