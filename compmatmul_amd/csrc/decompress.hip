@@ -160,6 +160,24 @@ __device__ __forceinline__ void stage_row_async(const T* __restrict__ P, T* buf0
   }
 }
 
+// val negated when the sign bits of entries ea and eb differ, as one xor of
+// the IEEE sign bit (exactly what negation does), so the signed value is
+// formed at once rather than the entries being kept live until its use.
+__device__ __forceinline__ float signed_by(float val, uint32_t ea, uint32_t eb) {
+  return __int_as_float(__float_as_int(val) ^ int((ea ^ eb) & 0x80000000u));
+}
+__device__ __forceinline__ double signed_by(double val, uint32_t ea, uint32_t eb) {
+  return __hiloint2double(__double2hiint(val) ^ int((ea ^ eb) & 0x80000000u), __double2loint(val));
+}
+
+// Hash entry i of a read-only table, through the constant address space:
+// at a uniform index this is a scalar load, counted by lgkmcnt, so the waits
+// for it never wait for the row copies in flight (vmcnt), as a vector load
+// issued after them would.
+__device__ __forceinline__ uint32_t hash_entry(const uint32_t* h, int64_t i) {
+  return ((const __attribute__((address_space(4))) uint32_t*)h)[i];
+}
+
 template <typename T, bool FWHT, int D>
 __device__ __forceinline__ void decompress_lds_body(const DecompressArgs& a, int blk) {
   constexpr int ER = rows_per_thread<T, D>();
@@ -184,14 +202,20 @@ __device__ __forceinline__ void decompress_lds_body(const DecompressArgs& a, int
     __syncthreads();                // everyone's, and row t-1 is no longer read
     if (t + 1 < D) stage_row_async<T>(P, buf0, b, t + 1, tid);
     const T* cur = buf0 + (t & 1) * b;
+    const uint32_t* hA = a.hA;
+    asm volatile("" : "+s"(hA));  // row t's entries are loaded here, not all hoisted to the start
 #pragma unroll
     for (int e = 0; e < ER; ++e) {
       const int i = min(i0 + e, a.m - 1);
-      const uint32_t ea = a.hA[int64_t(t) * a.m + i];
+      const uint32_t ea = hash_entry(hA, int64_t(t) * a.m + i);
       const uint32_t h = FWHT ? (hbucket(ea) ^ hbucket(eb[t])) : ((hbucket(ea) + hbucket(eb[t])) & mask);
       const T val = cur[h];
-      v[e][t] = (hneg(ea) ^ hneg(eb[t])) ? -val : val;
+      v[e][t] = signed_by(val, ea, eb[t]);
     }
+    // formed here, after all of the row's gathers are issued: the entries
+    // are not kept live until the median
+#pragma unroll
+    for (int e = 0; e < ER; ++e) asm volatile("" : "+v"(v[e][t]));
   }
   if (!jok) return;
 #pragma unroll
@@ -232,7 +256,6 @@ __device__ __forceinline__ void decompress_ring_body(const DecompressArgs& a, in
   constexpr int NH = 2 * D;  // halves to copy
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int b = a.b;  // b * sizeof(T) == 65536
-  const uint32_t half = uint32_t(b / 2);
   const int tiles = (a.p + LNT - 1) / LNT;
   const int tid = threadIdx.x;
   const int j = int(blk % tiles) * LNT + tid;
@@ -257,17 +280,26 @@ __device__ __forceinline__ void decompress_ring_body(const DecompressArgs& a, in
     __syncthreads();  // every wave's copies landed, and row t-1 is no longer read
     if (2 * t + 3 < NH) stage_half_async<T>(P, smem_raw, b, 2 * t + 3, tid);
     if (2 * t + 4 < NH) stage_half_async<T>(P, smem_raw, b, 2 * t + 4, tid);
-    const unsigned char* lo = smem_raw + ((2 * t) % RING_SLOTS) * RING_HALF;
-    const unsigned char* hi = smem_raw + ((2 * t + 1) % RING_SLOTS) * RING_HALF;
+    // row t's halves sit in slots 2t mod 5 and 2t+1 mod 5: adjacent, so the
+    // byte offset is lo + h sizeof(T), except where slot 4 wraps to slot 0
+    const uint32_t lo = uint32_t((2 * t) % RING_SLOTS) * RING_HALF;
+    const bool wraps = (2 * t) % RING_SLOTS == RING_SLOTS - 1;
+    const uint32_t* hA = a.hA;
+    asm volatile("" : "+s"(hA));  // row t's entries are loaded here, not all hoisted to the start
 #pragma unroll
     for (int e = 0; e < ER; ++e) {
       const int i = min(i0 + e, a.m - 1);
-      const uint32_t ea = a.hA[int64_t(t) * a.m + i];
+      const uint32_t ea = hash_entry(hA, int64_t(t) * a.m + i);
       const uint32_t h = FWHT ? (hbucket(ea) ^ hbucket(eb[t])) : ((hbucket(ea) + hbucket(eb[t])) & mask);
-      const unsigned char* base = h >= half ? hi : lo;
-      const T val = *reinterpret_cast<const T*>(base + (h & (half - 1)) * sizeof(T));
-      v[e][t] = (hneg(ea) ^ hneg(eb[t])) ? -val : val;
+      uint32_t off = lo + h * uint32_t(sizeof(T));
+      if (wraps) off = min(off, off - uint32_t(RING_LDS));  // past slot 4: back to slot 0
+      const T val = *reinterpret_cast<const T*>(smem_raw + off);
+      v[e][t] = signed_by(val, ea, eb[t]);
     }
+    // formed here, after all of the row's gathers are issued: the entries
+    // are not kept live until the median
+#pragma unroll
+    for (int e = 0; e < ER; ++e) asm volatile("" : "+v"(v[e][t]));
   }
   if (!jok) return;
 #pragma unroll
