@@ -233,9 +233,13 @@ __device__ __forceinline__ void decompress_lds_body(const DecompressArgs& a, int
 constexpr int RING_SLOTS = 5;
 constexpr int RING_HALF = 32768;                  // bytes per half row
 constexpr int RING_LDS = RING_SLOTS * RING_HALF;  // 160 KB
-constexpr int RING_ISSUE = RING_HALF / 16 / LNT;  // dwordx4 copies per wave and half (4)
+// dwordx4 copies per wave and half: 4 with 512 threads, 2 with 1024
+template <int NT>
+constexpr int ring_issue() {
+  return RING_HALF / 16 / NT;
+}
 
-template <typename T>
+template <typename T, int NT>
 __device__ __forceinline__ void stage_half_async(const T* __restrict__ P, unsigned char* ring, int b,
                                                  int k, int tid) {
   const unsigned char* src =
@@ -243,22 +247,22 @@ __device__ __forceinline__ void stage_half_async(const T* __restrict__ P, unsign
   unsigned char* dst = ring + (k % RING_SLOTS) * RING_HALF;
   const int wave = tid >> 6, lane = tid & 63;
 #pragma unroll
-  for (int q = 0; q < RING_ISSUE; ++q) {
-    const int c = q * LNT + wave * 64;  // first 16-byte chunk of this wave's copy
+  for (int q = 0; q < ring_issue<NT>(); ++q) {
+    const int c = q * NT + wave * 64;  // first 16-byte chunk of this wave's copy
     __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)(src + (c + lane) * 16),
                                      (__attribute__((address_space(3))) void*)(dst + c * 16), 16, 0, 0);
   }
 }
 
-template <typename T, bool FWHT, int D>
+template <typename T, bool FWHT, int D, int NT>
 __device__ __forceinline__ void decompress_ring_body(const DecompressArgs& a, int blk) {
   constexpr int ER = rows_per_thread<T, D>();
   constexpr int NH = 2 * D;  // halves to copy
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int b = a.b;  // b * sizeof(T) == 65536
-  const int tiles = (a.p + LNT - 1) / LNT;
+  const int tiles = (a.p + NT - 1) / NT;
   const int tid = threadIdx.x;
-  const int j = int(blk % tiles) * LNT + tid;
+  const int j = int(blk % tiles) * NT + tid;
   const int i0 = int(blk / tiles) * ER;
   const bool jok = j < a.p;
   const uint32_t mask = uint32_t(b - 1);
@@ -267,19 +271,19 @@ __device__ __forceinline__ void decompress_ring_body(const DecompressArgs& a, in
 #pragma unroll
   for (int t = 0; t < D; ++t) eb[t] = jok ? a.hB[int64_t(t) * a.p + j] : 0u;
 #pragma unroll
-  for (int k = 0; k < (NH < 3 ? NH : 3); ++k) stage_half_async<T>(P, smem_raw, b, k, tid);
+  for (int k = 0; k < (NH < 3 ? NH : 3); ++k) stage_half_async<T, NT>(P, smem_raw, b, k, tid);
   T v[ER][D];
 #pragma unroll
   for (int t = 0; t < D; ++t) {
-    // halves 0 .. 2t+1 landed: at most one later half (RING_ISSUE copies of
+    // halves 0 .. 2t+1 landed: at most one later half (ring_issue copies of
     // this wave) may still be in flight, none at the last row
     if (t + 1 < D)
-      __builtin_amdgcn_s_waitcnt(0x0070 | RING_ISSUE);  // vmcnt(RING_ISSUE) lgkmcnt(0)
+      __builtin_amdgcn_s_waitcnt(0x0070 | ring_issue<NT>());  // vmcnt(ring_issue) lgkmcnt(0)
     else
       __builtin_amdgcn_s_waitcnt(0x0070);               // vmcnt(0) lgkmcnt(0)
     __syncthreads();  // every wave's copies landed, and row t-1 is no longer read
-    if (2 * t + 3 < NH) stage_half_async<T>(P, smem_raw, b, 2 * t + 3, tid);
-    if (2 * t + 4 < NH) stage_half_async<T>(P, smem_raw, b, 2 * t + 4, tid);
+    if (2 * t + 3 < NH) stage_half_async<T, NT>(P, smem_raw, b, 2 * t + 3, tid);
+    if (2 * t + 4 < NH) stage_half_async<T, NT>(P, smem_raw, b, 2 * t + 4, tid);
     // row t's halves sit in slots 2t mod 5 and 2t+1 mod 5: adjacent, so the
     // byte offset is lo + h sizeof(T), except where slot 4 wraps to slot 0
     const uint32_t lo = uint32_t((2 * t) % RING_SLOTS) * RING_HALF;
@@ -309,9 +313,9 @@ __device__ __forceinline__ void decompress_ring_body(const DecompressArgs& a, in
   }
 }
 
-template <typename T, bool FWHT, int D>
-__global__ void __launch_bounds__(LNT) k_decompress_ring(DecompressArgs a) {
-  decompress_ring_body<T, FWHT, D>(a, blockIdx.x);
+template <typename T, bool FWHT, int D, int NT>
+__global__ void __launch_bounds__(NT) k_decompress_ring(DecompressArgs a) {
+  decompress_ring_body<T, FWHT, D, NT>(a, blockIdx.x);
 }
 
 template <typename T, bool FWHT, int D>
@@ -362,20 +366,38 @@ __global__ void __launch_bounds__(NT) k_decompress_any(DecompressArgs a) {
   }
 }
 
+// 1024 threads leave 128 registers per thread: every ring instantiation fits
+// but FP32 at d = 10, whose 12 rows x 10 values plus hash entries spill.
+template <typename T, int D>
+constexpr bool ring_wide_fits() {
+  return !(sizeof(T) == 4 && D == 10);
+}
+
+// The ring kernel with NT threads: false when the 160 KB of LDS is refused.
+template <typename T, bool FWHT, int D, int NT>
+bool launch_ring(const DecompressArgs& a, hipStream_t s) {
+  if (hipFuncSetAttribute(reinterpret_cast<const void*>(&k_decompress_ring<T, FWHT, D, NT>),
+                          hipFuncAttributeMaxDynamicSharedMemorySize, RING_LDS) != hipSuccess)
+    return false;
+  constexpr int er = rows_per_thread<T, D>();
+  const int64_t blocks = int64_t((a.p + NT - 1) / NT) * ((a.m + er - 1) / er);
+  k_decompress_ring<T, FWHT, D, NT><<<dim3(unsigned(blocks)), dim3(NT), RING_LDS, s>>>(a);
+  return true;
+}
+
 template <typename T, bool FWHT>
 hipError_t launch_t(const DecompressArgs& a, hipStream_t s) {
   if (a.d <= 16 && size_t(a.b) * sizeof(T) == 65536) {  // 64 KB rows: the half-row ring
+    // with at least 1024 columns: 1024 threads, so each workgroup's staging
+    // of the d rows serves twice the entries
+    const bool wide = a.p >= 1024;
     switch (a.d) {
 #define CM_D(N)                                                                               \
   case N:                                                                                     \
-    if (hipFuncSetAttribute(reinterpret_cast<const void*>(&k_decompress_ring<T, FWHT, N>),    \
-                            hipFuncAttributeMaxDynamicSharedMemorySize, RING_LDS) != hipSuccess) \
-      break;                                                                                  \
-    k_decompress_ring<T, FWHT, N><<<dim3(unsigned(int64_t((a.p + LNT - 1) / LNT) *           \
-                                                 ((a.m + rows_per_thread<T, N>() - 1) /      \
-                                                  rows_per_thread<T, N>()))),                \
-                                    dim3(LNT), RING_LDS, s>>>(a);                             \
-    return hipGetLastError();
+    if (wide && ring_wide_fits<T, N>() ? launch_ring<T, FWHT, N, 1024>(a, s)                  \
+                                       : launch_ring<T, FWHT, N, LNT>(a, s))                  \
+      return hipGetLastError();                                                               \
+    break;
       CM_D(1) CM_D(2) CM_D(3) CM_D(4) CM_D(5) CM_D(6) CM_D(7) CM_D(8)
       CM_D(9) CM_D(10) CM_D(11) CM_D(12) CM_D(13) CM_D(14) CM_D(15) CM_D(16)
 #undef CM_D

@@ -131,6 +131,26 @@ def test_decompress_f32_64k_rows(variant, d):
     sk.free()
 
 
+@pytest.mark.parametrize("variant", ["fft", "fwht"])
+@pytest.mark.parametrize("dtype,b,d", [(np.float32, 1 << 14, 1), (np.float32, 1 << 14, 10),
+                                       (np.float32, 1 << 14, 15), (np.float32, 1 << 14, 16),
+                                       (np.float64, 1 << 13, 9), (np.float64, 1 << 13, 16)])
+def test_decompress_64k_rows_wide(variant, dtype, b, d):
+    # 64 KB sketch rows with at least 1024 columns: the 1024-thread ring
+    # kernel (d = 10 in FP32 keeps 512 threads); 1300 columns leave a ragged
+    # last tile and 37 rows a ragged last row block
+    print(f"{variant} d={d} b={b} {np.dtype(dtype).name} p=1300", flush=True)
+    rng = np.random.default_rng(200 + d)
+    AT = rng.standard_normal((16, 37)).astype(dtype)
+    B = rng.standard_normal((16, 1300)).astype(dtype)
+    sk = cma.compress(variant, AT, B, d, b, seed=7 + d, dtype=dtype)
+    vals = sk.values()
+    hA, hB = sk.hashes()
+    want = orc.decompress(variant, vals, hA, hB)
+    assert np.array_equal(sk.decompress().astype(np.float64), want)
+    sk.free()
+
+
 def test_decompress_into_device_buffer():
     rng = np.random.default_rng(1)
     AT = rng.standard_normal((30, 40)).astype(np.float32)
