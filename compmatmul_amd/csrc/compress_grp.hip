@@ -560,23 +560,34 @@ __device__ __forceinline__ int f_of4(int L, int r) {
 }
 
 // Start copying row[0, len) (len <= SEG) into LDS dst[0, SEG) with
-// global_load_lds, 4 bytes per lane and no registers: wave w of the
-// workgroup copies elements q 64G + 64 w + lane; lanes past the row copy its
-// last element into slots the schedule never reads.
+// global_load_lds, 4 bytes per lane and no registers: wave w (uniform) of
+// the workgroup copies elements q 64G + 64 w + lane; lanes past the row copy
+// its last element into slots the schedule never reads.  A full row needs no
+// clamp: each copy is the scalar base row + q 64G + 64 w and the lane's
+// offset, with no vector arithmetic.
 template <int G>
-__device__ __forceinline__ void copy_row_async(const float* __restrict__ row, int len, float* dst,
-                                               int tid) {
+__device__ __forceinline__ void copy_row_async(const float* __restrict__ row, int len, float* dst, int w,
+                                               int lane) {
   if (len <= 0) return;
-  const int wbase = tid & ~63;
-  // opaque per call, so the per-lane offsets are recomputed here (two VALU
+  if (len == SEG) {
+#pragma unroll
+    for (int q = 0; q < SEG / (64 * G); ++q) {
+      const int e0 = q * 64 * G + w * 64;
+      const float* rq = row + e0;  // uniform; the unsigned lane offset allows scalar-base addressing
+      __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)(rq + uint32_t(lane)),
+                                       (__attribute__((address_space(3))) void*)(dst + e0), 4, 0, 0);
+    }
+    return;
+  }
+  // opaque per call, so the clamped offsets are recomputed here (two VALU
   // each) instead of being hoisted out of the k loop and kept live
-  int lane_id = tid;
+  int lane_id = w * 64 + lane;
   asm volatile("" : "+v"(lane_id));
 #pragma unroll
   for (int q = 0; q < SEG / (64 * G); ++q) {
     const int e = q * 64 * G + lane_id;
     __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)(row + min(e, len - 1)),
-                                     (__attribute__((address_space(3))) void*)(dst + q * 64 * G + wbase), 4,
+                                     (__attribute__((address_space(3))) void*)(dst + q * 64 * G + w * 64), 4,
                                      0, 0);
   }
 }
@@ -618,8 +629,8 @@ __device__ __forceinline__ void grp_unit(const RegJob& j, int u, float* S, float
     gb = group_counts(scb, j.p);
   }
   if (paired && k0 < k1) {
-    copy_row_async<G>(j.AT + int64_t(k0) * j.ld_at, j.m, Rb, tid);
-    copy_row_async<G>(j.B + int64_t(k0) * j.ld_b, j.p, Rb + SEG, tid);
+    copy_row_async<G>(j.AT + int64_t(k0) * j.ld_at, j.m, Rb, w, lane);
+    copy_row_async<G>(j.B + int64_t(k0) * j.ld_b, j.p, Rb + SEG, w, lane);
     __builtin_amdgcn_s_waitcnt(0);  // this wave's copies of rows k0 have landed
     __syncthreads();                // and every wave's
   }
@@ -632,11 +643,11 @@ __device__ __forceinline__ void grp_unit(const RegJob& j, int u, float* S, float
     // publishes them for the scatters of k + 1
     const bool more = paired && k + 1 < k1;
     sketch_transform_grp<LOG_G, paired>(arow, j.m, la, sca, S, Rb, paired, w, lane, xa, ca, ga, [&] {
-      if (more) copy_row_async<G>(arow + j.ld_at, j.m, Rb, tid);
+      if (more) copy_row_async<G>(arow + j.ld_at, j.m, Rb, w, lane);
     }, false);
     sketch_transform_grp<LOG_G, paired>(brow, j.p, lb, scb, S, paired ? Rb + SEG : Rb, paired, w, lane, xb,
                                         cb, gb, [&] {
-                                          if (more) copy_row_async<G>(brow + j.ld_b, j.p, Rb + SEG, tid);
+                                          if (more) copy_row_async<G>(brow + j.ld_b, j.p, Rb + SEG, w, lane);
                                         }, more);
     accumulate_pairs(xa, xb, acc);
   }
