@@ -256,7 +256,18 @@ template <int G>
 __device__ __forceinline__ void copy_seg0_async(const float* __restrict__ ar, int m,
                                                 const float* __restrict__ br, int sl, float* dst,
                                                 int tid) {
-  const int wbase = tid & ~63;
+  // the wave's base, uniform: the LDS destinations (M0) are scalar
+  const int wbase = __builtin_amdgcn_readfirstlane(tid & ~63);
+  if (m >= SEG) {  // the whole segment comes from A: no clamp, no select
+#pragma unroll
+    for (int q = 0; q < SEG / (64 * G); ++q) {
+      const int e0 = q * 64 * G + wbase;
+      const float* rq = ar + e0;  // uniform; the lane offset is unsigned
+      __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)(rq + uint32_t(tid & 63)),
+                                       (__attribute__((address_space(3))) void*)(dst + e0), 4, 0, 0);
+    }
+    return;
+  }
   // opaque per call, so the per-lane addresses are recomputed here instead
   // of being hoisted out of the k loop and kept live
   int lane_id = tid;

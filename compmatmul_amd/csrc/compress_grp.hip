@@ -386,12 +386,17 @@ __device__ __forceinline__ void sketch_transform_grp(const float* __restrict__ r
   constexpr int G = 1 << LOG_G;
   float* Sw = S + w * plen<LOG_G>();
   const int tid = w * 64 + lane;
-  {
-    float4* S4 = reinterpret_cast<float4*>(Sw);
-    constexpr int Z4 = (LOG_G == 2 ? plen<2>() : B2) / 4;  // float4 to clear
+  if constexpr (LOG_G == 2) {
+    // the 32 data columns of the 64 padded rows (the pad columns hold only
+    // dummy slots, never read): lane L clears columns 4 (L & 7) .. + 3 of
+    // rows 8 q + (L >> 3)
+    float4* z = reinterpret_cast<float4*>(Sw + (lane >> 3) * prow<2>() + (lane & 7) * 4);
 #pragma unroll
-    for (int q = 0; q < (Z4 + 63) / 64; ++q)
-      if (q * 64 + lane < Z4) S4[q * 64 + lane] = make_float4(0.f, 0.f, 0.f, 0.f);
+    for (int q = 0; q < 8; ++q) z[q * 8 * prow<2>() / 4] = make_float4(0.f, 0.f, 0.f, 0.f);
+  } else {
+    float4* S4 = reinterpret_cast<float4*>(Sw);
+#pragma unroll
+    for (int q = 0; q < B2 / 4 / 64; ++q) S4[q * 64 + lane] = make_float4(0.f, 0.f, 0.f, 0.f);
   }
   const int nseg = (keys + SEG - 1) / SEG;
   const uint32_t* __restrict__ lp = list;
