@@ -296,7 +296,7 @@ __device__ __forceinline__ void scatter_cached(uint32_t (&c)[CA],
   asm volatile("" : "+s"(n0), "+s"(e1), "+s"(ng));
   const char* Rbb = reinterpret_cast<const char*>(Rb);
   char* Sb = reinterpret_cast<char*>(S);
-  constexpr int BATCH = 4;  // LDS reads in flight per batch
+  constexpr int BATCH = 6;  // LDS reads in flight per batch (4: 1% slower)
 #pragma unroll
   for (int g0 = 0; g0 < CA; g0 += BATCH) {
     // the entries are made opaque in place, so their fields are extracted
