@@ -438,7 +438,7 @@ int build_plan(int count, const cm_product* ps, unsigned flags, int grain, const
     bt->n_comb = int(comb_blocks.size());
     bt->red_lds = 0;
     for (int i = 0; i < count; ++i)
-      bt->red_lds = std::max(bt->red_lds, size_t(ps[i].variant == CM_FWHT ? 1 : 2) * sizeof(float) << cm::kRegLb2);
+      bt->red_lds = std::max(bt->red_lds, cm::reduce_lds_bytes(ps[i].variant == CM_FWHT, sizeof(float), cm::kRegLb2));
 
     // compress work items: chunks to the least-loaded of eight queues
     std::vector<int2> items_host[kKinds];

@@ -59,6 +59,25 @@ __device__ void lds_wht2(T* sa, T* sb, int b2, int tid) {
   }
 }
 
+// As lds_dif, with the twiddles from an LDS table tws[k] = W_b2^k, k < b2/2
+// (the same values: W_b^(pos b / 2len) = W_b2^(pos b2 / 2len)), so no stage
+// waits on a global load per butterfly.
+template <typename T, bool CONJ>
+__device__ void lds_dif_tab(cpx<T>* z, const cpx<T>* tws, int b2, int lb2, int tid) {
+  for (int len = b2 >> 1; len >= 1; len >>= 1) {
+    const int lstride = lb2 - __ffs(len);  // log2(b2 / (2 len))
+    for (int q = tid; q < (b2 >> 1); q += NT) {
+      const int pos = q & (len - 1);
+      const int j = ((q & ~(len - 1)) << 1) | pos;
+      const cpx<T> x0 = z[j], x1 = z[j + len];
+      const cpx<T> w = tws[pos << lstride];
+      z[j] = cadd(x0, x1);
+      z[j + len] = CONJ ? cmulc(csub(x0, x1), w) : cmul(csub(x0, x1), w);
+    }
+    __syncthreads();
+  }
+}
+
 // in-LDS radix-2 decimation-in-frequency DFT of length b2 (natural order in,
 // bit-reversed order out).  tw holds W_b^j for j < b; conj selects W^-1.
 template <typename T, bool CONJ>
@@ -254,13 +273,27 @@ __device__ __forceinline__ void reduce_body(const ReduceArgs& a, int blk) {
   const int tid = threadIdx.x;
   const int64_t chunk_stride = int64_t(a.d) * b1 * b2;
   const int64_t off = (int64_t(t) * b1 + slice) * b2;
+  // with the chunks already summed (nchunks 1), the loads of eight elements
+  // are issued before their stores; the general loop sums chunk by chunk
+  constexpr int U = 8;
   if constexpr (FWHT) {
     T* s = reinterpret_cast<T*>(smem_raw);
     const T* __restrict__ P = static_cast<const T*>(a.partial);
-    for (int e = tid; e < b2; e += NT) {
-      ACC sum = ACC(0);
-      for (int c = 0; c < a.nchunks; ++c) sum += ACC(P[c * chunk_stride + off + e]);
-      s[e] = T(sum);
+    if (a.nchunks == 1) {
+      for (int e0 = tid; e0 < b2; e0 += U * NT) {
+        T v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) v[u] = e0 + u * NT < b2 ? P[off + e0 + u * NT] : T(0);
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+          if (e0 + u * NT < b2) s[e0 + u * NT] = v[u];
+      }
+    } else {
+      for (int e = tid; e < b2; e += NT) {
+        ACC sum = ACC(0);
+        for (int c = 0; c < a.nchunks; ++c) sum += ACC(P[c * chunk_stride + off + e]);
+        s[e] = T(sum);
+      }
     }
     __syncthreads();
     lds_wht2<T>(s, nullptr, b2, tid);
@@ -270,17 +303,35 @@ __device__ __forceinline__ void reduce_body(const ReduceArgs& a, int blk) {
     cpx<T>* z = reinterpret_cast<cpx<T>*>(smem_raw);
     const cpx<T>* __restrict__ P = static_cast<const cpx<T>*>(a.partial);
     const cpx<T>* __restrict__ tw = static_cast<const cpx<T>*>(a.tw);
-    for (int e = tid; e < b2; e += NT) {
-      ACC sx = ACC(0), sy = ACC(0);
-      for (int c = 0; c < a.nchunks; ++c) {
-        const cpx<T> v = P[c * chunk_stride + off + e];
-        sx += ACC(v.x);
-        sy += ACC(v.y);
+    if (a.nchunks == 1) {
+      for (int e0 = tid; e0 < b2; e0 += U * NT) {
+        cpx<T> v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) v[u] = e0 + u * NT < b2 ? P[off + e0 + u * NT] : cpx<T>{T(0), T(0)};
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+          if (e0 + u * NT < b2) z[e0 + u * NT] = v[u];
       }
-      z[e] = {T(sx), T(sy)};
+    } else {
+      for (int e = tid; e < b2; e += NT) {
+        ACC sx = ACC(0), sy = ACC(0);
+        for (int c = 0; c < a.nchunks; ++c) {
+          const cpx<T> v = P[c * chunk_stride + off + e];
+          sx += ACC(v.x);
+          sy += ACC(v.y);
+        }
+        z[e] = {T(sx), T(sy)};
+      }
     }
-    __syncthreads();
-    lds_dif<T, true>(z, b2, a.lb, tw, tid);
+    if (reduce_twiddle_table(sizeof(T), a.lb2)) {
+      cpx<T>* tws = z + b2;  // W_b2^k = W_b^(k b / b2), k < b2 / 2
+      for (int k = tid; k < (b2 >> 1); k += NT) tws[k] = tw[int64_t(k) << (a.lb - a.lb2)];
+      __syncthreads();
+      lds_dif_tab<T, true>(z, tws, b2, a.lb2, tid);
+    } else {
+      __syncthreads();
+      lds_dif<T, true>(z, b2, a.lb, tw, tid);
+    }
     cpx<T>* V = static_cast<cpx<T>*>(a.V) + off;
     for (int e = tid; e < b2; e += NT) V[bitrev(e, a.lb2)] = z[e];
   }
@@ -309,22 +360,36 @@ __device__ __forceinline__ void combine_body(const CombineArgs& a, int64_t idx) 
   for (int64_t f = 0; f < (b >> a.lb_out); ++f) {
     const uint32_t h = ho + uint32_t(f << a.lb_out);
     const uint32_t lo = h & uint32_t(b2 - 1), hi = h >> a.lb2;
+    // the loads of eight slices are issued before their additions (in
+    // slice order, as before)
+    constexpr int U = 8;
     if constexpr (FWHT) {
       const T* V = static_cast<const T*>(a.V) + int64_t(t) * b;
       T s = T(0);
-      for (int sl = 0; sl < b1; ++sl) {
-        const T v = V[int64_t(sl) * b2 + lo];
-        s += (__popc(uint32_t(sl) & hi) & 1) ? -v : v;
+      for (int sl0 = 0; sl0 < b1; sl0 += U) {
+        T v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) v[u] = sl0 + u < b1 ? V[int64_t(sl0 + u) * b2 + lo] : T(0);
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+          if (sl0 + u < b1) s += (__popc(uint32_t(sl0 + u) & hi) & 1) ? -v[u] : v[u];
       }
       total += s * (T(1) / T(b));
     } else {
       const cpx<T>* V = static_cast<const cpx<T>*>(a.V) + int64_t(t) * b;
       const cpx<T>* tw = static_cast<const cpx<T>*>(a.tw);
       T im = T(0);
-      for (int sl = 0; sl < b1; ++sl) {
-        const cpx<T> v = V[int64_t(sl) * b2 + lo];
-        const cpx<T> w = tw[(uint64_t(sl) * h) & uint64_t(b - 1)];
-        im += v.y * w.x - v.x * w.y;  // Im(v * conj(w))
+      for (int sl0 = 0; sl0 < b1; sl0 += U) {
+        cpx<T> v[U], w[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const bool ok = sl0 + u < b1;
+          v[u] = ok ? V[int64_t(sl0 + u) * b2 + lo] : cpx<T>{T(0), T(0)};
+          w[u] = ok ? tw[(uint64_t(sl0 + u) * h) & uint64_t(b - 1)] : cpx<T>{T(0), T(0)};
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+          if (sl0 + u < b1) im += v[u].y * w[u].x - v[u].x * w[u].y;  // Im(v * conj(w))
       }
       total += im * (T(1) / T(2 * b));
     }
@@ -401,7 +466,7 @@ hipError_t reduce_t(const ReduceArgs& a0, hipStream_t s) {
     a.nchunks = 1;
   }
   const int b1 = 1 << (a.lb - a.lb2);
-  const size_t lds = (FWHT ? 1 : 2) * sizeof(T) * (size_t(1) << a.lb2);
+  const size_t lds = reduce_lds_bytes(FWHT, sizeof(T), a.lb2);
   const unsigned grid = unsigned(a.d * b1);
   if (wide)
     k_reduce_inverse<T, double, FWHT><<<dim3(grid), dim3(NT), lds, s>>>(a);

@@ -165,6 +165,19 @@ hipError_t launch_combine(const CombineArgs& a, hipStream_t s);
 // the per-job counts from finish_batch_blocks() (returns the combine count).
 struct FinishJob { ReduceArgs r; CombineArgs c; };
 int finish_batch_blocks(const FinishJob& j, int* sum_blocks, int* red_blocks);
+// The FFT reduce keeps a half-length twiddle table W_b2^k, k < b2/2, in LDS
+// beside the complex slice when both fit the default 64 KB.
+__host__ __device__ inline bool reduce_twiddle_table(size_t real_bytes, int lb2) {
+  return (2 * real_bytes << lb2) + (lb2 > 0 ? 2 * real_bytes << (lb2 - 1) : 0) <= 65536;
+}
+// LDS bytes of one reduce workgroup: the slice (complex for FFT) and the
+// twiddle table where it is used.
+inline size_t reduce_lds_bytes(bool fwht, size_t real_bytes, int lb2) {
+  if (fwht) return real_bytes << lb2;
+  const size_t z = 2 * real_bytes << lb2;
+  return reduce_twiddle_table(real_bytes, lb2) && lb2 > 0 ? z + (2 * real_bytes << (lb2 - 1)) : z;
+}
+
 hipError_t launch_finish_batch(const FinishJob* jobs, const int2* sum_blocks, int n_sum,
                                const int2* red_blocks, int n_red, size_t red_lds,
                                const int2* comb_blocks, int n_comb, hipStream_t s);
