@@ -95,12 +95,23 @@ __host__ __device__ __forceinline__ Layout layout(const RegJob& j) {
 // workgroup's partitions, (w plen + slot_of(l)) << 2 (or w plen + the dummy
 // slot of the lane; below 2^16) | key offset in the 2048-key segment << 18
 // | sign << 31.
+// LW waves build one list: wave bw takes keys [bw SEG/LW, (bw+1) SEG/LW)
+// of each segment.  A key's round is its occurrence index among the keys of
+// its bucket in key order: the count from earlier segments, plus the counts
+// of the earlier waves, plus its rank in its own wave (one wave's LDS
+// atomics complete in program order).  Within a round, keys are placed in
+// key order: each wave's run starts after the earlier waves' keys of that
+// round.  The list is the same as one wave walking the keys in order.
+constexpr int LW = 4;
+
 template <int LOG_G>
 __device__ __forceinline__ void grp_lists_body(const RegJob& j, int g) {
   constexpr int G = 1 << LOG_G;
-  constexpr int Q = SEG / 64;  // keys per lane and segment
-  __shared__ uint32_t cnt[B2];
-  __shared__ int nr[RCAP];
+  constexpr int KW = SEG / LW;   // keys per wave and segment
+  constexpr int Q = KW / 64;     // keys per lane and segment
+  __shared__ uint32_t cnt[B2];     // occurrences in earlier segments
+  __shared__ uint32_t cw[LW][B2];  // occurrences in this segment, per wave
+  __shared__ int nrw[LW][RCAP];    // this segment's keys per wave and round
   const Layout lay = layout(j);
   const bool side_b = g >= lay.lists;
   const int ts = side_b ? g - lay.lists : g;
@@ -112,39 +123,61 @@ __device__ __forceinline__ void grp_lists_body(const RegJob& j, int g) {
   const int keys = side_b ? j.p : j.m;
   const uint32_t* __restrict__ hh = (side_b ? j.hB : j.hA) + int64_t(t) * keys;
   uint32_t* __restrict__ L = side_b ? lay.LB + ts * lay.cap_b : lay.LA + ts * lay.cap_a;
-  const int lane = threadIdx.x;
-  for (int e = lane; e < B2; e += 64) cnt[e] = 0;
+  const int tid = threadIdx.x;
+  const int bw = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
+  for (int e = tid; e < B2; e += 64 * LW) cnt[e] = 0;
+  for (int e = tid; e < LW * B2; e += 64 * LW) (&cw[0][0])[e] = 0;
   int out = 0;
   for (int s0 = 0; s0 < keys; s0 += SEG) {
     const int sl = min(SEG, keys - s0);
-    // the whole segment's hash entries in flight at once
+    // this wave's hash entries of the segment in flight at once
     uint32_t ek[Q];
 #pragma unroll
-    for (int q = 0; q < Q; ++q) ek[q] = q * 64 + lane < sl ? hh[s0 + q * 64 + lane] : 0u;
-    if (lane < RCAP) nr[lane] = 0;
+    for (int q = 0; q < Q; ++q) {
+      const int kk = bw * KW + q * 64 + lane;
+      ek[q] = kk < sl ? hh[s0 + kk] : 0u;
+    }
+    if (lane < RCAP) nrw[bw][lane] = 0;
     __syncthreads();
-    // round = occurrence of the bucket so far, in key order (the wave's LDS
-    // atomics complete in program order), -1 outside this partition
-    int rk[Q];
+    int loc[Q];  // rank within this wave, -1 outside this partition
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
       const uint32_t h = hbucket(ek[q]);
+      loc[q] = -1;
+      if (bw * KW + q * 64 + lane < sl && ((h >> LB2) & (G - 1)) == w)
+        loc[q] = int(atomicAdd(&cw[bw][h & (B2 - 1)], 1u));
+    }
+    __syncthreads();
+    int rk[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
       rk[q] = -1;
-      if (q * 64 + lane < sl && ((h >> LB2) & (G - 1)) == w) {
-        rk[q] = min(int(atomicAdd(&cnt[h & (B2 - 1)], 1u)), RCAP - 1);
-        atomicAdd(&nr[rk[q]], 1);
+      if (loc[q] >= 0) {
+        const uint32_t b = hbucket(ek[q]) & (B2 - 1);
+        int r = int(cnt[b]) + loc[q];
+        for (int w2 = 0; w2 < bw; ++w2) r += int(cw[w2][b]);
+        rk[q] = min(r, RCAP - 1);
+        atomicAdd(&nrw[bw][rk[q]], 1);
       }
     }
     __syncthreads();
     static_assert(RCAP == 8, "round_slot places rounds 0..7");
-    int base[RCAP];
-    int64_t run[RCAP];  // next slot of each round
+    int base[RCAP], tot[RCAP];
+    int64_t run[RCAP];  // next slot of each round for this wave
     int acc = out;
 #pragma unroll
     for (int r = 0; r < RCAP; ++r) {
+      int before = 0, all = 0;
+#pragma unroll
+      for (int w2 = 0; w2 < LW; ++w2) {
+        const int n = nrw[w2][r];
+        before += w2 < bw ? n : 0;
+        all += n;
+      }
       base[r] = acc;
-      run[r] = int64_t(acc) * 64;
-      acc += (nr[r] + 63) / 64;
+      tot[r] = all;
+      run[r] = int64_t(acc) * 64 + before;
+      acc += (all + 63) / 64;
     }
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
@@ -153,30 +186,42 @@ __device__ __forceinline__ void grp_lists_body(const RegJob& j, int g) {
       if (r >= 0) {
         const uint32_t h = hbucket(ek[q]);
         const uint32_t neg = hneg(ek[q]) ^ (__popc((h >> (LB2 + LOG_G)) & group) & 1u);
-        L[pos] = ((pbase + slot_of<LOG_G>(h & (B2 - 1))) << 2) | (uint32_t(q * 64 + lane) << 18) |
+        L[pos] = ((pbase + slot_of<LOG_G>(h & (B2 - 1))) << 2) | (uint32_t(bw * KW + q * 64 + lane) << 18) |
                  (neg ? F_SIGN : 0u);
       }
     }
 #pragma unroll
     for (int rr = 0; rr < RCAP; ++rr) {  // pad rounds to whole groups (dummy slots)
-      const int full = (nr[rr] + 63) / 64 * 64;
-      for (int slot = nr[rr] + lane; slot < full; slot += 64)
+      const int full = (tot[rr] + 63) / 64 * 64;
+      for (int slot = tot[rr] + tid; slot < full; slot += 64 * LW)
         L[int64_t(base[rr]) * 64 + slot] = (pbase + dummy_slot<LOG_G>(slot & 63)) << 2;
     }
-    if (lane < RCAP) {
+    if (tid < RCAP) {
+      int all = 0;
+#pragma unroll
+      for (int w2 = 0; w2 < LW; ++w2) all += nrw[w2][tid];
       const int64_t row = int64_t(g) * MAXSEG + s0 / SEG;
-      lay.ngroups[row * RCAP + lane] = (nr[lane] + 63) / 64;
+      lay.ngroups[row * RCAP + tid] = (all + 63) / 64;
     }
     out = acc;
-    __syncthreads();
+    __syncthreads();  // every rank and count of the segment is read
+    for (int e = tid; e < B2; e += 64 * LW) {  // fold the segment into cnt
+      uint32_t c = cnt[e];
+#pragma unroll
+      for (int w2 = 0; w2 < LW; ++w2) {
+        c += cw[w2][e];
+        cw[w2][e] = 0;
+      }
+      cnt[e] = c;
+    }
   }
 }
 
 template <int LOG_G>
-__global__ void __launch_bounds__(64) k_grp_lists(RegJob j) { grp_lists_body<LOG_G>(j, blockIdx.x); }
+__global__ void __launch_bounds__(64 * LW) k_grp_lists(RegJob j) { grp_lists_body<LOG_G>(j, blockIdx.x); }
 
 template <int LOG_G>
-__global__ void __launch_bounds__(64) k_grp_lists_batch(const RegJob* __restrict__ jobs,
+__global__ void __launch_bounds__(64 * LW) k_grp_lists_batch(const RegJob* __restrict__ jobs,
                                                           const int2* __restrict__ blocks) {
   const int2 e = blocks[blockIdx.x];
   grp_lists_body<LOG_G>(jobs[e.x], e.y);
@@ -731,7 +776,7 @@ int slots_t() {
 template <int LOG_G>
 hipError_t launch_t(const RegJob& j, hipStream_t s) {
   const int lists = j.d << j.lbs;
-  k_grp_lists<LOG_G><<<dim3(unsigned(2 * lists)), dim3(64), 0, s>>>(j);
+  k_grp_lists<LOG_G><<<dim3(unsigned(2 * lists)), dim3(64 * LW), 0, s>>>(j);
   const int units = j.nchunks * (lists >> LOG_G);
   const int grid = (units + 7) / 8 * 8;
   if (j.m <= SEG && j.p <= SEG)
@@ -765,9 +810,9 @@ hipError_t launch_fwht_grp_lists_batch(int log_g, const RegJob* jobs, const int2
                                           hipStream_t s) {
   if (nblocks <= 0) return hipSuccess;
   if (log_g == 2)
-    k_grp_lists_batch<2><<<dim3(unsigned(nblocks)), dim3(64), 0, s>>>(jobs, blocks);
+    k_grp_lists_batch<2><<<dim3(unsigned(nblocks)), dim3(64 * LW), 0, s>>>(jobs, blocks);
   else
-    k_grp_lists_batch<1><<<dim3(unsigned(nblocks)), dim3(64), 0, s>>>(jobs, blocks);
+    k_grp_lists_batch<1><<<dim3(unsigned(nblocks)), dim3(64 * LW), 0, s>>>(jobs, blocks);
   return hipGetLastError();
 }
 
