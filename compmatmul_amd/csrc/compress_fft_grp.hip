@@ -62,12 +62,20 @@ __host__ __device__ __forceinline__ Layout layout(const RegJob& j) {
 // [0, m + p): the keys of partition hw = w, bucket l = h mod 2048,
 // coefficient s W_b^(h s0) (times i for B).  Entry: l << 3 (byte offset of
 // a complex, or a dummy slot 2048 + lane) | key offset in the segment << 18.
+// LW waves build one list, as in the FWHT slice-group builder: wave bw takes
+// keys [bw SEG/LW, (bw+1) SEG/LW) of each segment; ranks and round runs are
+// offset by the earlier waves' counts, so the list is the one a single wave
+// walking the keys in order would build.
+constexpr int LW = 4;
+
 template <int LOG_G>
 __device__ __forceinline__ void fft_grp_lists_body(const RegJob& j, int g) {
   constexpr int G = 1 << LOG_G;
-  constexpr int Q = SEG / 64;  // keys per lane and segment
-  __shared__ uint32_t cnt[B2];
-  __shared__ int nr[RCAP];
+  constexpr int KW = SEG / LW;   // keys per wave and segment
+  constexpr int Q = KW / 64;     // keys per lane and segment
+  __shared__ uint32_t cnt[B2];     // occurrences in earlier segments
+  __shared__ uint32_t cw[LW][B2];  // occurrences in this segment, per wave
+  __shared__ int nrw[LW][RCAP];    // this segment's keys per wave and round
   const Layout lay = layout(j);
   const int b1 = 1 << j.lbs;
   const int t = g >> j.lbs;
@@ -81,42 +89,62 @@ __device__ __forceinline__ void fft_grp_lists_body(const RegJob& j, int g) {
   const uint32_t* __restrict__ hB = j.hB + int64_t(t) * p;
   uint32_t* __restrict__ Ls = lay.L + g * lay.cap;
   float2* __restrict__ Cs = lay.C + g * lay.cap;
-  const int lane = threadIdx.x;
-  for (int e = lane; e < B2; e += 64) cnt[e] = 0;
+  const int tid = threadIdx.x;
+  const int bw = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
+  for (int e = tid; e < B2; e += 64 * LW) cnt[e] = 0;
+  for (int e = tid; e < LW * B2; e += 64 * LW) (&cw[0][0])[e] = 0;
   int out = 0;
   for (int base0 = 0; base0 < keys; base0 += SEG) {
     const int sl = min(SEG, keys - base0);
-    // the whole segment's hash entries in flight at once
+    // this wave's hash entries of the segment in flight at once
     uint32_t ek[Q];
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
-      const int key = base0 + q * 64 + lane;
-      ek[q] = q * 64 + lane < sl ? (key < m ? hA[key] : hB[key - m]) : 0u;
+      const int kk = bw * KW + q * 64 + lane;
+      const int key = base0 + kk;
+      ek[q] = kk < sl ? (key < m ? hA[key] : hB[key - m]) : 0u;
     }
-    if (lane < RCAP) nr[lane] = 0;
+    if (lane < RCAP) nrw[bw][lane] = 0;
     __syncthreads();
-    // round = occurrence of the bucket so far, in key order (the wave's LDS
-    // atomics complete in program order), -1 outside this partition
-    int rk[Q];
+    int loc[Q];  // rank within this wave, -1 outside this partition
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
       const uint32_t h = hbucket(ek[q]);
+      loc[q] = -1;
+      if (bw * KW + q * 64 + lane < sl && ((h >> LB2) & (G - 1)) == w)
+        loc[q] = int(atomicAdd(&cw[bw][h & (B2 - 1)], 1u));
+    }
+    __syncthreads();
+    int rk[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
       rk[q] = -1;
-      if (q * 64 + lane < sl && ((h >> LB2) & (G - 1)) == w) {
-        rk[q] = min(int(atomicAdd(&cnt[h & (B2 - 1)], 1u)), RCAP - 1);
-        atomicAdd(&nr[rk[q]], 1);
+      if (loc[q] >= 0) {
+        const uint32_t b = hbucket(ek[q]) & (B2 - 1);
+        int r = int(cnt[b]) + loc[q];
+        for (int w2 = 0; w2 < bw; ++w2) r += int(cw[w2][b]);
+        rk[q] = min(r, RCAP - 1);
+        atomicAdd(&nrw[bw][rk[q]], 1);
       }
     }
     __syncthreads();
     static_assert(RCAP == 8, "round_slot places rounds 0..7");
-    int base[RCAP];
-    int64_t run[RCAP];  // next slot of each round
+    int base[RCAP], tot[RCAP];
+    int64_t run[RCAP];  // next slot of each round for this wave
     int acc = out;
 #pragma unroll
     for (int r = 0; r < RCAP; ++r) {
+      int before = 0, all = 0;
+#pragma unroll
+      for (int w2 = 0; w2 < LW; ++w2) {
+        const int n = nrw[w2][r];
+        before += w2 < bw ? n : 0;
+        all += n;
+      }
       base[r] = acc;
-      run[r] = int64_t(acc) * 64;
-      acc += (nr[r] + 63) / 64;
+      tot[r] = all;
+      run[r] = int64_t(acc) * 64 + before;
+      acc += (all + 63) / 64;
     }
     // every key's twiddle W_b^(h s0), all loads in flight at once (inside
     // the per-key branch below they would be issued one after another)
@@ -126,15 +154,16 @@ __device__ __forceinline__ void fft_grp_lists_body(const RegJob& j, int g) {
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
       const int r = rk[q];
+      const int kk = bw * KW + q * 64 + lane;
       uint32_t ent = 0;
       float2 coef = make_float2(0.f, 0.f);
       if (r >= 0) {
-        const bool isb = base0 + q * 64 + lane >= m;
+        const bool isb = base0 + kk >= m;
         const uint32_t h = hbucket(ek[q]);
         const float2 wv = wq[q];
         const float sg = hneg(ek[q]) ? -1.f : 1.f;
         coef = isb ? make_float2(-sg * wv.y, sg * wv.x) : make_float2(sg * wv.x, sg * wv.y);
-        ent = ((h & (B2 - 1)) << 3) | (uint32_t(q * 64 + lane) << 18);
+        ent = ((h & (B2 - 1)) << 3) | (uint32_t(kk) << 18);
       }
       const int64_t pos = round_slot(r, lane, run);
       if (r >= 0) {
@@ -144,27 +173,39 @@ __device__ __forceinline__ void fft_grp_lists_body(const RegJob& j, int g) {
     }
 #pragma unroll
     for (int rr = 0; rr < RCAP; ++rr) {  // pad rounds to whole groups (dummy slots)
-      const int full = (nr[rr] + 63) / 64 * 64;
-      for (int slot = nr[rr] + lane; slot < full; slot += 64) {
+      const int full = (tot[rr] + 63) / 64 * 64;
+      for (int slot = tot[rr] + tid; slot < full; slot += 64 * LW) {
         const int64_t pos = int64_t(base[rr]) * 64 + slot;
         Ls[pos] = uint32_t(B2 + (slot & 63)) << 3;
         Cs[pos] = make_float2(0.f, 0.f);
       }
     }
-    if (lane < RCAP) {
+    if (tid < RCAP) {
+      int all = 0;
+#pragma unroll
+      for (int w2 = 0; w2 < LW; ++w2) all += nrw[w2][tid];
       const int64_t row = int64_t(g) * MAXSEG + base0 / SEG;
-      lay.segcnt[row * RCAP + lane] = (nr[lane] + 63) / 64;
+      lay.segcnt[row * RCAP + tid] = (all + 63) / 64;
     }
     out = acc;
-    __syncthreads();
+    __syncthreads();  // every rank and count of the segment is read
+    for (int e = tid; e < B2; e += 64 * LW) {  // fold the segment into cnt
+      uint32_t c = cnt[e];
+#pragma unroll
+      for (int w2 = 0; w2 < LW; ++w2) {
+        c += cw[w2][e];
+        cw[w2][e] = 0;
+      }
+      cnt[e] = c;
+    }
   }
 }
 
 template <int LOG_G>
-__global__ void __launch_bounds__(64) k_fft_grp_lists(RegJob j) { fft_grp_lists_body<LOG_G>(j, blockIdx.x); }
+__global__ void __launch_bounds__(64 * LW) k_fft_grp_lists(RegJob j) { fft_grp_lists_body<LOG_G>(j, blockIdx.x); }
 
 template <int LOG_G>
-__global__ void __launch_bounds__(64) k_fft_grp_lists_batch(const RegJob* __restrict__ jobs,
+__global__ void __launch_bounds__(64 * LW) k_fft_grp_lists_batch(const RegJob* __restrict__ jobs,
                                                           const int2* __restrict__ blocks) {
   const int2 e = blocks[blockIdx.x];
   fft_grp_lists_body<LOG_G>(jobs[e.x], e.y);
@@ -475,7 +516,7 @@ int slots_t() {
 template <int LOG_G>
 hipError_t launch_t(const RegJob& j, hipStream_t s) {
   const int lists = j.d << j.lbs;
-  k_fft_grp_lists<LOG_G><<<dim3(unsigned(lists)), dim3(64), 0, s>>>(j);
+  k_fft_grp_lists<LOG_G><<<dim3(unsigned(lists)), dim3(64 * LW), 0, s>>>(j);
   const int units = j.nchunks * (lists >> LOG_G);
   const int grid = (units + 7) / 8 * 8;
   k_compress_fft_grp<LOG_G><<<dim3(unsigned(grid)), dim3(64 << LOG_G), 0, s>>>(j, units);
@@ -506,9 +547,9 @@ hipError_t launch_fft_grp_lists_batch(int log_g, const RegJob* jobs, const int2*
                                           hipStream_t s) {
   if (nblocks <= 0) return hipSuccess;
   if (log_g == 2)
-    k_fft_grp_lists_batch<2><<<dim3(unsigned(nblocks)), dim3(64), 0, s>>>(jobs, blocks);
+    k_fft_grp_lists_batch<2><<<dim3(unsigned(nblocks)), dim3(64 * LW), 0, s>>>(jobs, blocks);
   else
-    k_fft_grp_lists_batch<1><<<dim3(unsigned(nblocks)), dim3(64), 0, s>>>(jobs, blocks);
+    k_fft_grp_lists_batch<1><<<dim3(unsigned(nblocks)), dim3(64 * LW), 0, s>>>(jobs, blocks);
   return hipGetLastError();
 }
 
