@@ -246,6 +246,22 @@ __device__ __forceinline__ void dpp_butterfly8(float* x, float s) {
       : "v"(s));
 }
 
+// As dpp_butterfly8 with partner lane L ^ 2 (quad_perm [2, 3, 0, 1]).
+__device__ __forceinline__ void dpp_butterfly8_x2(float* x, float s) {
+  asm volatile(
+      "s_nop 1\n\t"
+      "v_fmac_f32_dpp %0, %0, %8 quad_perm:[2,3,0,1] row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f32_dpp %1, %1, %8 quad_perm:[2,3,0,1] row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f32_dpp %2, %2, %8 quad_perm:[2,3,0,1] row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f32_dpp %3, %3, %8 quad_perm:[2,3,0,1] row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f32_dpp %4, %4, %8 quad_perm:[2,3,0,1] row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f32_dpp %5, %5, %8 quad_perm:[2,3,0,1] row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f32_dpp %6, %6, %8 quad_perm:[2,3,0,1] row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f32_dpp %7, %7, %8 quad_perm:[2,3,0,1] row_mask:0xf bank_mask:0xf"
+      : "+v"(x[0]), "+v"(x[1]), "+v"(x[2]), "+v"(x[3]), "+v"(x[4]), "+v"(x[5]), "+v"(x[6]), "+v"(x[7])
+      : "v"(s));
+}
+
 __device__ __forceinline__ void wht_reg(float (&x)[R]) {
 #pragma unroll
   for (int s = 1; s < R; s <<= 1) {
@@ -475,13 +491,17 @@ __device__ __forceinline__ void sketch_transform_grp(const float* __restrict__ r
   after_scatter();   // the staged row is free again
   if constexpr (LOG_G == 2) {
     // One register pass does the cross-partition WHT together with
-    // position bits 0..2: lane L, register (c << 2) | hw holds position
-    // w 512 + 8 L + c of partition hw (two 16-byte reads per partition:
-    // the eight positions share a padded row).  The results go back into
-    // this wave's region (rows 16 w .. 16 w + 15 of every partition, which
-    // no other wave reads), permuted for the transposed read: register
-    // (c << 2) | j to row (L >> 5) + 2 c + 16 w, column L & 31 of slice j.
-    const int p0 = w * 512 + 8 * lane;
+    // position bits 0..2: lane L, register (c << 2) | hw holds position p
+    // of partition hw, with p bits 0..2 = c, bits 3, 4, 5 = L bits 0, 1, 2,
+    // bit 6 = w bit 0, bits 7, 8, 9 = L bits 3, 4, 5, bit 10 = w bit 1 (two
+    // 16-byte reads per partition: the eight positions share a padded row,
+    // and each pass of 8 lanes covers 32 distinct banks).  The results go
+    // back into this wave's region (the rows whose bits 1 and 5 are w's,
+    // which no other wave reads), permuted for the transposed read: register
+    // (c << 2) | j to row (L >> 5) + 2 (w & 1) + 4 c + 32 (w >> 1), column
+    // L & 31 of slice j.
+    const int p0 = (lane & 3) * 8 + ((lane >> 2) & 1) * 32 + (w & 1) * 64 + ((lane >> 3) & 7) * 128 +
+                   (w >> 1) * 1024;
     {
       const int a = int(slot_of<2>(uint32_t(p0)));
 #pragma unroll
@@ -500,18 +520,18 @@ __device__ __forceinline__ void sketch_transform_grp(const float* __restrict__ r
     }
     wht_reg(x);
     {
-      const int wb = ((lane >> 5) + 16 * w) * prow<2>() + (lane & 31);
+      const int wb = ((lane >> 5) + 2 * (w & 1) + 32 * (w >> 1)) * prow<2>() + (lane & 31);
 #pragma unroll
       for (int c = 0; c < 8; ++c)
 #pragma unroll
-        for (int jj = 0; jj < 4; ++jj) S[jj * plen<2>() + wb + 2 * c * prow<2>()] = x[(c << 2) | jj];
+        for (int jj = 0; jj < 4; ++jj) S[jj * plen<2>() + wb + 4 * c * prow<2>()] = x[(c << 2) | jj];
     }
     // with `drain`, this wave's row copies have landed too, so the barrier
     // also publishes the next rows
     if (drain) __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();  // every wave's region is written
-    // wave w now reads its slice: row `lane` holds positions l3..l7 of
-    // (l8, f0, f1, f2, l9, l10) = lane bits 0..5
+    // wave w now reads its slice: row `lane` holds positions l3, l4, l5,
+    // l7, l8 of (l9, l6, f0, f1, f2, l10) = lane bits 0..5
     {
       const float4* rbase = reinterpret_cast<const float4*>(Sw + lane * prow<2>());
 #pragma unroll
@@ -523,22 +543,17 @@ __device__ __forceinline__ void sketch_transform_grp(const float* __restrict__ r
         x[4 * c + 3] = v.w;
       }
     }
-    wht_reg(x);  // position bits 3..7
-#pragma unroll
-    for (int r = 0; r < R; ++r) {  // position bit 9 (lane bit 4)
-      if (r & 2) continue;
-      const auto pq = __builtin_amdgcn_permlane16_swap(__float_as_uint(x[r]), __float_as_uint(x[r | 2]),
-                                                       false, false);
-      const float p = __uint_as_float(pq[0]), q = __uint_as_float(pq[1]);
-      x[r] = p + q;
-      x[r | 2] = p - q;
-    }
-    // position bit 8 (lane bit 0): x[L ^ 1] s + x[L] with s = -1 on odd
-    // lanes, which therefore hold the negated value; A and B carry the same
-    // signs, so their product is unchanged
+    wht_reg(x);  // position bits 3, 4, 5, 7, 8
+    // position bit 9 (lane bit 0): x[L ^ 1] s + x[L] with s = -1 on odd
+    // lanes, which therefore hold the negated value; likewise position bit
+    // 6 (lane bit 1) with x[L ^ 2].  A and B carry the same signs, so their
+    // product is unchanged
     const float sgn = (lane & 1) ? -1.f : 1.f;
 #pragma unroll
     for (int r = 0; r < R; r += 8) dpp_butterfly8(x + r, sgn);
+    const float sgn2 = (lane & 2) ? -1.f : 1.f;
+#pragma unroll
+    for (int r = 0; r < R; r += 8) dpp_butterfly8_x2(x + r, sgn2);
     // position bit 10 (lane bit 5): the swap brings each pair into one lane;
     // its butterfly is left to the caller (lanes L and L ^ 32 have the same
     // parity, so the pair carries one sign)
@@ -602,11 +617,12 @@ __device__ __forceinline__ int f_of(int L, int r) {
 }
 
 // G = 4: frequency of lane L, register r after the merged cross pass
-// (register bits 0, 1 -> f10, f9; 2..4 -> f5..f7; lane bit 0 -> f8,
-// 1..3 -> f0..f2, 4 -> f4, 5 -> f3)
+// (register bits 0..4 -> f10, f4, f5, f7, f8; lane bits 0..5 -> f9, f6,
+// f0, f1, f2, f3)
 __device__ __forceinline__ int f_of4(int L, int r) {
-  return ((r & 1) << 10) | (((r >> 1) & 1) << 9) | (((r >> 2) & 7) << 5) | ((L & 1) << 8) |
-         ((L >> 1) & 7) | (((L >> 4) & 1) << 4) | (((L >> 5) & 1) << 3);
+  return ((r & 1) << 10) | (((r >> 1) & 1) << 4) | (((r >> 2) & 1) << 5) | (((r >> 3) & 1) << 7) |
+         (((r >> 4) & 1) << 8) | ((L & 1) << 9) | (((L >> 1) & 1) << 6) | ((L >> 2) & 7) |
+         (((L >> 5) & 1) << 3);
 }
 
 // Start copying row[0, len) (len <= SEG) into LDS dst[0, SEG) with
