@@ -386,10 +386,18 @@ __device__ __forceinline__ void fft_grp_unit(const RegJob& j, int u, float2* S, 
         const int base0 = seg * SEG;
         const int sl = min(SEG, keys - base0);
         float v[SEG / (64 * G)];
+        if (sl == SEG && (base0 + SEG <= j.m || base0 >= j.m)) {
+          // a full segment from one operand: a uniform base and the lane's
+          // unsigned offset, no clamp and no select
+          const float* src = base0 >= j.m ? br + (base0 - j.m) : ar + base0;
 #pragma unroll
-        for (int q = 0; q < SEG / (64 * G); ++q) {
-          const int key = base0 + min(q * 64 * G + tid, sl - 1);
-          v[q] = key < j.m ? ar[key] : br[key - j.m];
+          for (int q = 0; q < SEG / (64 * G); ++q) v[q] = (src + q * 64 * G)[uint32_t(tid)];
+        } else {
+#pragma unroll
+          for (int q = 0; q < SEG / (64 * G); ++q) {
+            const int key = base0 + min(q * 64 * G + tid, sl - 1);
+            v[q] = key < j.m ? ar[key] : br[key - j.m];
+          }
         }
         __syncthreads();  // every wave is done with the previous segment
 #pragma unroll
@@ -422,7 +430,7 @@ __device__ __forceinline__ void fft_grp_unit(const RegJob& j, int u, float2* S, 
       // constant W_{32 G}^(q jj) = W_128^(q jj 4 / G): no load inside the loop
       float2 twb[G];
 #pragma unroll
-      for (int jj = 1; jj < G; ++jj) twb[jj] = tw[int64_t((l0 + lane) * jj) << tw_shift];
+      for (int jj = 1; jj < G; ++jj) twb[jj] = tw[uint32_t(((l0 + lane) * jj) << tw_shift)];  // index < b
 #pragma unroll 1  // unrolled, all positions' loads would be live at once
       for (int q = 0; q < PER; ++q) {
         const int l = l0 + q * 64 + lane;
@@ -491,7 +499,7 @@ k_compress_fft_grp_batch(const RegJob* __restrict__ jobs, const int2* __restrict
     for (;;) {
       if (threadIdx.x == 0) item = int(atomicAdd(&counters[q], 1u));
       __syncthreads();
-      const int it = item;
+      const int it = __builtin_amdgcn_readfirstlane(item);  // uniform: the job lives in scalar registers
       __syncthreads();  // every thread has read `item` before it is rewritten
       if (it >= len) break;
       const int2 e = items[base + it];

@@ -96,14 +96,14 @@ __device__ __forceinline__ void dft2048_lds(float2* S, const float2* __restrict_
   // per call instead of 32; the other twiddles are products A[j] B[i].  The
   // table address is laundered so the compiler reloads these (L1-resident)
   // instead of pinning them across the k loop.
-  asm volatile("" : "+v"(t2));
+  asm volatile("" : "+s"(t2));
   float2 A[8], B[4];
   A[0] = make_float2(1.f, 0.f);
   B[0] = A[0];
 #pragma unroll
-  for (int jj = 1; jj < 8; ++jj) A[jj] = t2[cbrev5(jj) * 64 + lane];
+  for (int jj = 1; jj < 8; ++jj) A[jj] = (t2 + cbrev5(jj) * 64)[uint32_t(lane)];
 #pragma unroll
-  for (int ii = 1; ii < 4; ++ii) B[ii] = t2[cbrev5(8 * ii) * 64 + lane];
+  for (int ii = 1; ii < 4; ++ii) B[ii] = (t2 + cbrev5(8 * ii) * 64)[uint32_t(lane)];
 #pragma unroll
   for (int q = 1; q < R; ++q) {  // q = 0 is k2 = 0: W = 1
     const int k2 = cbrev5(q), jj = k2 & 7, ii = k2 >> 3;
