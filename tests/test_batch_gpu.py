@@ -29,6 +29,8 @@ SHAPES = [
     ("fwht", 1, 3, 1, 1, 2, None),           # b = 2
     ("fft", 3, 5, 4, 16, 4, None),           # tiny b, d = 16
     ("fft", 90, 40, 0, 3, 1 << 12, None),    # empty C
+    ("fwht", 3000, 150, 500, 9, 1 << 13, None),   # rows past one segment: staged path, G = 4
+    ("fwht", 700, 64, 2500, 5, 1 << 12, None),    # rows past one segment: staged path, G = 2
 ]
 SEED = 13794
 
