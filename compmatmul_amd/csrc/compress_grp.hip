@@ -360,6 +360,9 @@ __device__ __forceinline__ void scatter_cached(uint32_t (&c)[CA],
   constexpr int BATCH = 6;  // LDS reads in flight per batch (4: 1% slower)
 #pragma unroll
   for (int g0 = 0; g0 < CA; g0 += BATCH) {
+    // a batch wholly past the list is skipped (one scalar branch: short
+    // lists of small products); within a batch the reads stay together
+    if (g0 >= ng) break;
     // the entries are made opaque in place, so their fields are extracted
     // here rather than hoisted out of the k loop as three live registers
     uint32_t e[BATCH];
