@@ -9,12 +9,14 @@
 //
 //  * Element n = L + 64 r (lane L, register r < 32) after the sketch is read
 //    from LDS.  Four-step DFT with n = n1 + 64 n2, k = k2 + 32 k1:
-//      1. per lane, DFT-32 over the registers (radix-2 DIF, compile-time
-//         twiddles; register q then holds k2 = bitrev5(q));
+//      1. per lane, DFT-32 over the registers (DIF: one radix-2^2 step,
+//         three radix-2 stages, compile-time twiddles; register q then
+//         holds k2 = bitrev5(q));
 //      2. twiddle by W_2048^(n1 k2) (per-lane table, L1-resident);
 //      3. transpose through LDS (rows of 66 complex: 16-byte aligned, read
-//         as conflict-free b128 pairs from one base address plus immediates): lane L' gets k2 = L' & 31 and the 32
-//         values n1 = r' + 32 (L' >> 5);
+//         as conflict-free b128 pairs from one base address plus
+//         immediates): lane L' gets k2 = L' & 31 and the 32 values
+//         n1 = r' + 32 (L' >> 5);
 //      4. DFT-64 over n1: one radix-2 DIF stage across lanes L' and L'+32
 //         with v_permlane32_swap, a second swap that regroups the halves,
 //         then a DFT-32 per lane (lanes < 32 give even k1, lanes >= 32 odd).

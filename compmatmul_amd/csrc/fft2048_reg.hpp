@@ -4,11 +4,13 @@
 //
 // Element n = L + 64 r (lane L, register r < 32) as read from LDS.  Four-step
 // DFT with n = n1 + 64 n2, k = k2 + 32 k1:
-//   1. per lane, DFT-32 over the registers (radix-2 DIF, compile-time
-//      twiddles; register q then holds k2 = bitrev5(q));
+//   1. per lane, DFT-32 over the registers (DIF: one radix-2^2 step, three
+//      radix-2 stages, compile-time twiddles; register q then holds
+//      k2 = bitrev5(q));
 //   2. twiddle by W_2048^(n1 k2) (per-lane table, L1-resident);
 //   3. transpose through LDS (rows of 66 complex: 16-byte aligned, read
-//      as conflict-free b128 pairs from one base address plus immediates): lane L' gets k2 = L' & 31 and the 32 values
+//      as conflict-free b128 pairs from one base address plus
+//      immediates): lane L' gets k2 = L' & 31 and the 32 values
 //      n1 = r' + 32 (L' >> 5);
 //   4. DFT-64 over n1: one radix-2 DIF stage across lanes L' and L'+32 with
 //      v_permlane32_swap, a second swap that regroups the halves, then a
@@ -32,10 +34,63 @@ constexpr float W64R[64] = {1.000000000e+00f, 9.951847267e-01f, 9.807852804e-01f
 constexpr float W64I[64] = {0.0f, -9.801714033e-02f, -1.950903220e-01f, -2.902846773e-01f, -3.826834324e-01f, -4.713967368e-01f, -5.555702330e-01f, -6.343932842e-01f, -7.071067812e-01f, -7.730104534e-01f, -8.314696123e-01f, -8.819212643e-01f, -9.238795325e-01f, -9.569403357e-01f, -9.807852804e-01f, -9.951847267e-01f, -1.000000000e+00f, -9.951847267e-01f, -9.807852804e-01f, -9.569403357e-01f, -9.238795325e-01f, -8.819212643e-01f, -8.314696123e-01f, -7.730104534e-01f, -7.071067812e-01f, -6.343932842e-01f, -5.555702330e-01f, -4.713967368e-01f, -3.826834324e-01f, -2.902846773e-01f, -1.950903220e-01f, -9.801714033e-02f, 0.0f, 9.801714033e-02f, 1.950903220e-01f, 2.902846773e-01f, 3.826834324e-01f, 4.713967368e-01f, 5.555702330e-01f, 6.343932842e-01f, 7.071067812e-01f, 7.730104534e-01f, 8.314696123e-01f, 8.819212643e-01f, 9.238795325e-01f, 9.569403357e-01f, 9.807852804e-01f, 9.951847267e-01f, 1.000000000e+00f, 9.951847267e-01f, 9.807852804e-01f, 9.569403357e-01f, 9.238795325e-01f, 8.819212643e-01f, 8.314696123e-01f, 7.730104534e-01f, 7.071067812e-01f, 6.343932842e-01f, 5.555702330e-01f, 4.713967368e-01f, 3.826834324e-01f, 2.902846773e-01f, 1.950903220e-01f, 9.801714033e-02f};
 
 
-// in-register radix-2 DIF DFT-32: register q ends holding X[bitrev5(q)]
+// x W_32^e for an exponent known after unrolling: quarter turns are exact
+// moves, the rest one complex product
+__device__ __forceinline__ void twiddle32(float& xr, float& xi, int e) {
+  e &= 31;
+  if (e == 0) return;
+  if (e == 8) {  // -i
+    const float t = xr;
+    xr = xi;
+    xi = -t;
+    return;
+  }
+  if (e == 16) {
+    xr = -xr;
+    xi = -xi;
+    return;
+  }
+  if (e == 24) {  // +i
+    const float t = xr;
+    xr = -xi;
+    xi = t;
+    return;
+  }
+  const float wr = W32R[e], wi = W32I[e];
+  const float nr = fmaf(xr, wr, -xi * wi);
+  xi = fmaf(xr, wi, xi * wr);
+  xr = nr;
+}
+
+// in-register DIF DFT-32: register q ends holding X[bitrev5(q)].  The
+// first two radix-2 stages run as one radix-2^2 step: for a, b, c, d =
+// x[j], x[j+8], x[j+16], x[j+24], p = a + c, q = b + d, r = a - c, t = b - d,
+// the outputs are p + q, (p - q) W^2j, (r - i t) W^j and (r + i t) W^3j,
+// 20 general twiddle products where two radix-2 stages take 26.
 __device__ __forceinline__ void dft32(float (&re)[R], float (&im)[R]) {
 #pragma unroll
-  for (int half = R / 2; half >= 1; half >>= 1) {
+  for (int j = 0; j < 8; ++j) {
+    const float pr = re[j] + re[j + 16], pi = im[j] + im[j + 16];
+    const float rr = re[j] - re[j + 16], ri = im[j] - im[j + 16];
+    const float qr = re[j + 8] + re[j + 24], qi = im[j + 8] + im[j + 24];
+    const float tr = re[j + 8] - re[j + 24], ti = im[j + 8] - im[j + 24];
+    re[j] = pr + qr;
+    im[j] = pi + qi;
+    float ar = pr - qr, ai = pi - qi;
+    twiddle32(ar, ai, 2 * j);
+    re[j + 8] = ar;
+    im[j + 8] = ai;
+    float br = rr + ti, bi = ri - tr;  // r - i t
+    twiddle32(br, bi, j);
+    re[j + 16] = br;
+    im[j + 16] = bi;
+    float cr = rr - ti, ci = ri + tr;  // r + i t
+    twiddle32(cr, ci, 3 * j);
+    re[j + 24] = cr;
+    im[j + 24] = ci;
+  }
+#pragma unroll
+  for (int half = R / 8; half >= 1; half >>= 1) {
 #pragma unroll
     for (int blk = 0; blk < R; blk += 2 * half) {
 #pragma unroll
