@@ -252,6 +252,51 @@ __device__ __forceinline__ void scatter_section(const uint32_t* __restrict__ lp,
   }
 }
 
+// A segment's whole schedule in one batch when it has at most SEGB groups:
+// the entries and coefficients of all three sections are loaded together
+// (one load latency per segment instead of one per section), then each
+// group is written by its kind: plain below n0, read-add-write below
+// n0 + n1, atomic after.  The counts are uniform, so the tests are scalar.
+constexpr int SEGB = 12;
+
+__device__ __forceinline__ void scatter_segment(const uint32_t* __restrict__ lp,
+                                                const float2* __restrict__ cp, int n0, int n1,
+                                                int ng, float2* S, const float* Rb, int lane) {
+  const char* Rbb = reinterpret_cast<const char*>(Rb);
+  char* Sb = reinterpret_cast<char*>(S);
+  uint32_t ent[SEGB];
+  float2 cf[SEGB];
+#pragma unroll
+  for (int g = 0; g < SEGB; ++g) {
+    ent[g] = 0u;
+    cf[g] = make_float2(0.f, 0.f);
+    if (g < ng) {
+      ent[g] = lp[g * 64 + lane];
+      cf[g] = cp[g * 64 + lane];
+    }
+  }
+  float v[SEGB];
+#pragma unroll
+  for (int g = 0; g < SEGB; ++g)
+    v[g] = g < ng ? *reinterpret_cast<const float*>(Rbb + ((ent[g] >> 16) & 0x1ffcu)) : 0.f;
+#pragma unroll
+  for (int g = 0; g < SEGB; ++g) {
+    if (g >= ng) continue;
+    const float2 c = make_float2(v[g] * cf[g].x, v[g] * cf[g].y);
+    float2* d = reinterpret_cast<float2*>(Sb + (ent[g] & 0xfff8u));
+    if (g < n0) {
+      *d = c;
+    } else if (g < n0 + n1) {
+      const float2 o = *d;
+      *d = make_float2(o.x + c.x, o.y + c.y);
+    } else {
+      float* df = reinterpret_cast<float*>(d);
+      __hip_atomic_fetch_add(df, c.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      __hip_atomic_fetch_add(df + 1, c.y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+  }
+}
+
 // W_128^k = exp(-2 pi i k / 128) for k < 32, rounded from double
 __constant__ float2 kW128[32] = {
     {1.0f, -0.0f}, {0.99879545620517241f, -0.049067674327418015f},
@@ -410,6 +455,16 @@ __device__ __forceinline__ void fft_grp_unit(const RegJob& j, int u, float2* S, 
       const int n0 = ASYNC ? nr[0] : __builtin_amdgcn_readfirstlane(nr[0]);
       const int n1 = ASYNC ? nw : __builtin_amdgcn_readfirstlane(nw);
       const int n2 = ASYNC ? nr[RCAP - 1] : __builtin_amdgcn_readfirstlane(nr[RCAP - 1]);
+      if (ASYNC) {
+        // the single-product kernel: a short segment schedule in one batch
+        const int u0 = __builtin_amdgcn_readfirstlane(n0), u1 = __builtin_amdgcn_readfirstlane(n1);
+        const int ung = __builtin_amdgcn_readfirstlane(n0 + n1 + n2);
+        if (ung <= SEGB) {
+          scatter_segment(ls + off, cs + off, u0, u1, ung, Sw, Rb, lane);
+          off += int64_t(ung) * 64;
+          continue;
+        }
+      }
       scatter_section<0>(ls + off, cs + off, n0, Sw, Rb, lane);
       off += int64_t(n0) * 64;
       scatter_section<1>(ls + off, cs + off, n1, Sw, Rb, lane);
