@@ -455,8 +455,8 @@ __device__ __forceinline__ void fft_grp_unit(const RegJob& j, int u, float2* S, 
       const int n0 = ASYNC ? nr[0] : __builtin_amdgcn_readfirstlane(nr[0]);
       const int n1 = ASYNC ? nw : __builtin_amdgcn_readfirstlane(nw);
       const int n2 = ASYNC ? nr[RCAP - 1] : __builtin_amdgcn_readfirstlane(nr[RCAP - 1]);
-      if (ASYNC) {
-        // the single-product kernel: a short segment schedule in one batch
+      {
+        // a short segment schedule in one batch
         const int u0 = __builtin_amdgcn_readfirstlane(n0), u1 = __builtin_amdgcn_readfirstlane(n1);
         const int ung = __builtin_amdgcn_readfirstlane(n0 + n1 + n2);
         if (ung <= SEGB) {
