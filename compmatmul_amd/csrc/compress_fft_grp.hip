@@ -297,6 +297,21 @@ __device__ __forceinline__ void scatter_segment(const uint32_t* __restrict__ lp,
   }
 }
 
+// Section counts (plain, read-add-write, atomic groups) of the first
+// SEGC segments, read once per unit: they do not depend on k.
+constexpr int SEGC = 2;
+struct SegCounts {
+  int n0[SEGC], n1[SEGC], n2[SEGC];
+};
+
+__device__ __forceinline__ void seg_counts(const int* __restrict__ nr, int& n0, int& n1, int& n2) {
+  int nw = 0;
+  for (int r = 1; r < RCAP - 1; ++r) nw += nr[r];
+  n0 = __builtin_amdgcn_readfirstlane(nr[0]);
+  n1 = __builtin_amdgcn_readfirstlane(nw);
+  n2 = __builtin_amdgcn_readfirstlane(nr[RCAP - 1]);
+}
+
 // W_128^k = exp(-2 pi i k / 128) for k < 32, rounded from double
 __constant__ float2 kW128[32] = {
     {1.0f, -0.0f}, {0.99879545620517241f, -0.049067674327418015f},
@@ -414,6 +429,12 @@ __device__ __forceinline__ void fft_grp_unit(const RegJob& j, int u, float2* S, 
   // scatter of the previous segment (an LDS copy could only start after it)
   if (ASYNC && k0 < k1 && nseg > 0)
     copy_seg0_async<G>(j.AT + int64_t(k0) * j.ld_at, j.m, j.B + int64_t(k0) * j.ld_b, min(SEG, keys), Rb, tid);
+  SegCounts scn;
+#pragma unroll
+  for (int q = 0; q < SEGC; ++q) {
+    scn.n0[q] = scn.n1[q] = scn.n2[q] = 0;
+    if (q < nseg) seg_counts(sc + q * RCAP, scn.n0[q], scn.n1[q], scn.n2[q]);
+  }
   for (int k = k0; k < k1; ++k) {
     const float* ar = j.AT + int64_t(k) * j.ld_at;
     const float* br = j.B + int64_t(k) * j.ld_b;
@@ -449,21 +470,25 @@ __device__ __forceinline__ void fft_grp_unit(const RegJob& j, int u, float2* S, 
         for (int q = 0; q < SEG / (64 * G); ++q) Rb[q * 64 * G + tid] = v[q];
         __syncthreads();
       }
-      const int* __restrict__ nr = sc + seg * RCAP;
-      int nw = 0;
-      for (int r = 1; r < RCAP - 1; ++r) nw += nr[r];
-      const int n0 = ASYNC ? nr[0] : __builtin_amdgcn_readfirstlane(nr[0]);
-      const int n1 = ASYNC ? nw : __builtin_amdgcn_readfirstlane(nw);
-      const int n2 = ASYNC ? nr[RCAP - 1] : __builtin_amdgcn_readfirstlane(nr[RCAP - 1]);
-      {
-        // a short segment schedule in one batch
-        const int u0 = __builtin_amdgcn_readfirstlane(n0), u1 = __builtin_amdgcn_readfirstlane(n1);
-        const int ung = __builtin_amdgcn_readfirstlane(n0 + n1 + n2);
-        if (ung <= SEGB) {
-          scatter_segment(ls + off, cs + off, u0, u1, ung, Sw, Rb, lane);
-          off += int64_t(ung) * 64;
-          continue;
-        }
+      int n0, n1, n2;  // uniform: from the per-unit counts (a select chain) or read
+      if (seg < SEGC) {
+        n0 = scn.n0[0];
+        n1 = scn.n1[0];
+        n2 = scn.n2[0];
+#pragma unroll
+        for (int q = 1; q < SEGC; ++q)
+          if (seg == q) {
+            n0 = scn.n0[q];
+            n1 = scn.n1[q];
+            n2 = scn.n2[q];
+          }
+      } else {
+        seg_counts(sc + seg * RCAP, n0, n1, n2);
+      }
+      if (n0 + n1 + n2 <= SEGB) {  // a short segment schedule in one batch
+        scatter_segment(ls + off, cs + off, n0, n1, n0 + n1 + n2, Sw, Rb, lane);
+        off += int64_t(n0 + n1 + n2) * 64;
+        continue;
       }
       scatter_section<0>(ls + off, cs + off, n0, Sw, Rb, lane);
       off += int64_t(n0) * 64;
