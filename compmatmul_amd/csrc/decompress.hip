@@ -12,6 +12,8 @@
 // register selection network (d <= 16) or a rank count (larger d).
 // Recovery appends (i, j, value) for every estimate > tau through a global
 // counter; the host sorts the list by (i, j).
+#include <type_traits>
+
 #include "device_common.hpp"
 #include "kernels.hpp"
 
@@ -21,6 +23,18 @@ namespace {
 
 constexpr int NT = 256;
 constexpr int ROWS = 8;
+
+// f(integral_constant<int, I>) for I = I0 .. N-1: a loop whose index is a
+// constant in every copy of the body, however large the body grows (an
+// unroll pragma gives up past a size threshold, and then arrays indexed by
+// the loop variable leave registers for scratch memory)
+template <int I0, int N, typename F>
+__device__ __forceinline__ void static_for(F&& f) {
+  if constexpr (I0 < N) {
+    f(std::integral_constant<int, I0>{});
+    static_for<I0 + 1, N>(f);
+  }
+}
 
 __device__ __forceinline__ float vmin(float a, float b) { return fminf(a, b); }
 __device__ __forceinline__ float vmax(float a, float b) { return fmaxf(a, b); }
@@ -39,7 +53,9 @@ struct CmpNet {
   unsigned char a[64], b[64];
 };
 
-constexpr CmpNet median_net(int D) {
+// Batcher's odd-even merge sort of D values (comparator c orders a[c] before
+// b[c]), padded to the next power of two as above.
+constexpr CmpNet sort_net(int D) {
   CmpNet full{0, {}, {}};
   int N = 1;
   while (N < D) N *= 2;
@@ -52,6 +68,11 @@ constexpr CmpNet median_net(int D) {
             full.b[full.n] = static_cast<unsigned char>(i + j + k);
             ++full.n;
           }
+  return full;
+}
+
+constexpr CmpNet median_net(int D) {
+  const CmpNet full = sort_net(D);
   bool need[16] = {};
   bool keep[64] = {};
   need[D / 2] = true;
@@ -83,6 +104,84 @@ __device__ __forceinline__ T median_network(T (&v)[D]) {
     v[k] = hi;
   }
   return v[D / 2];
+}
+
+// x clamped to [lo, hi] (lo <= hi): the middle one of the three values,
+// returned unchanged (one v_med3_f32 in FP32)
+__device__ __forceinline__ float vclamp(float x, float lo, float hi) {
+  return __builtin_amdgcn_fmed3f(x, lo, hi);
+}
+__device__ __forceinline__ double vclamp(double x, double lo, double hi) {
+  return fmax(lo, fmin(x, hi));
+}
+
+// Streaming form of the same selection, for kernels that receive the D values
+// of an entry one repetition at a time.  Element D/2 of the ascending order
+// is the K-th largest value, K = D - D/2, so only the K largest values seen
+// so far are kept, in descending order in a[0..K-1]:
+// - values 0..K-1 are stored, then sorted with Batcher's network of K;
+// - value t >= K, the q-th of the R = D - K later values, enters as
+//   a[k] = clamp(x, a[k], a[k-1]) for k = K-1 down to 1, and a[0] = max(a[0], x).
+//   A value moves at most one place per later insertion, so only positions
+//   k >= K-1-(R-q) can still reach a[K-1]; the others are left stale and
+//   never read again.
+// Every step moves values without arithmetic, so a[K-1] at the end is
+// exactly element D/2 of the sorted values.  D = 15: 19 comparators and 28
+// clamps (the pruned network takes 49 comparators).  The selection work then
+// runs between the row copies instead of after the last one.  Used in FP32
+// for D >= 10, where it needs no more registers than the network (FP64 has
+// no med3 instruction, and at smaller D the compiler's schedule for it costs
+// occupancy or spills).
+template <typename T, int D>
+constexpr bool streamed() {
+  return sizeof(T) == 4 && D >= 10;
+}
+
+// values kept per entry: the K largest when streamed, else all D
+template <typename T, int D>
+constexpr int keep_count() {
+  return streamed<T, D>() ? D - D / 2 : D;
+}
+
+// value t of an entry into its kept values a
+template <typename T, int D, int t, int N>
+__device__ __forceinline__ void select_push(T (&a)[N], T x) {
+  static_assert(N == keep_count<T, D>(), "kept values per entry");
+  if constexpr (!streamed<T, D>()) {
+    a[t] = x;
+    return;
+  }
+  constexpr int K = keep_count<T, D>(), R = D - K;
+  if constexpr (t < K) {
+    a[t] = x;
+    if constexpr (t == K - 1) {
+      constexpr CmpNet net = sort_net(K);
+#pragma unroll
+      for (int c = 0; c < net.n; ++c) {
+        const int i = net.a[c], k = net.b[c];
+        const T hi = vmax(a[i], a[k]);
+        const T lo = vmin(a[i], a[k]);
+        a[i] = hi;
+        a[k] = lo;
+      }
+    }
+  } else {
+    constexpr int q = t - K + 1;
+    constexpr int lo = K - 1 - (R - q) > 0 ? K - 1 - (R - q) : 0;
+#pragma unroll
+    for (int k = K - 1; k >= (lo > 1 ? lo : 1); --k) a[k] = vclamp(x, a[k], a[k - 1]);
+    if constexpr (lo == 0) a[0] = vmax(a[0], x);
+  }
+}
+
+// element D/2 of an entry's sorted values, from its kept values
+template <typename T, int D, int N>
+__device__ __forceinline__ T select_result(T (&a)[N]) {
+  static_assert(N == keep_count<T, D>(), "kept values per entry");
+  if constexpr (streamed<T, D>())
+    return a[keep_count<T, D>() - 1];
+  else
+    return median_network<T, D>(a);
 }
 
 template <typename T, bool FWHT>
@@ -195,33 +294,36 @@ __device__ __forceinline__ void decompress_lds_body(const DecompressArgs& a, int
 #pragma unroll
   for (int t = 0; t < D; ++t) eb[t] = jok ? a.hB[int64_t(t) * a.p + j] : 0u;
   stage_row_async<T>(P, buf0, b, 0, tid);
-  T v[ER][D];
-#pragma unroll
-  for (int t = 0; t < D; ++t) {
+  T v[ER][keep_count<T, D>()];
+  static_for<0, D>([&](auto tc) {
+    constexpr int t = decltype(tc)::value;
     __builtin_amdgcn_s_waitcnt(0);  // this thread's copies of row t are done
     __syncthreads();                // everyone's, and row t-1 is no longer read
     if (t + 1 < D) stage_row_async<T>(P, buf0, b, t + 1, tid);
     const T* cur = buf0 + (t & 1) * b;
     const uint32_t* hA = a.hA;
     asm volatile("" : "+s"(hA));  // row t's entries are loaded here, not all hoisted to the start
+    T x[ER];
 #pragma unroll
     for (int e = 0; e < ER; ++e) {
       const int i = min(i0 + e, a.m - 1);
       const uint32_t ea = hash_entry(hA, int64_t(t) * a.m + i);
       const uint32_t h = FWHT ? (hbucket(ea) ^ hbucket(eb[t])) : ((hbucket(ea) + hbucket(eb[t])) & mask);
       const T val = cur[h];
-      v[e][t] = signed_by(val, ea, eb[t]);
+      x[e] = signed_by(val, ea, eb[t]);
     }
     // formed here, after all of the row's gathers are issued: the entries
-    // are not kept live until the median
+    // are not kept live until the selection
 #pragma unroll
-    for (int e = 0; e < ER; ++e) asm volatile("" : "+v"(v[e][t]));
-  }
+    for (int e = 0; e < ER; ++e) asm volatile("" : "+v"(x[e]));
+#pragma unroll
+    for (int e = 0; e < ER; ++e) select_push<T, D, t>(v[e], x[e]);
+  });
   if (!jok) return;
 #pragma unroll
   for (int e = 0; e < ER; ++e) {
     if (i0 + e >= a.m) break;
-    emit<T>(a, i0 + e, j, median_network<T, D>(v[e]));
+    emit<T>(a, i0 + e, j, select_result<T, D>(v[e]));
   }
 }
 
@@ -272,9 +374,9 @@ __device__ __forceinline__ void decompress_ring_body(const DecompressArgs& a, in
   for (int t = 0; t < D; ++t) eb[t] = jok ? a.hB[int64_t(t) * a.p + j] : 0u;
 #pragma unroll
   for (int k = 0; k < (NH < 3 ? NH : 3); ++k) stage_half_async<T, NT>(P, smem_raw, b, k, tid);
-  T v[ER][D];
-#pragma unroll
-  for (int t = 0; t < D; ++t) {
+  T v[ER][keep_count<T, D>()];
+  static_for<0, D>([&](auto tc) {
+    constexpr int t = decltype(tc)::value;
     // halves 0 .. 2t+1 landed: at most one later half (ring_issue copies of
     // this wave) may still be in flight, none at the last row
     if (t + 1 < D)
@@ -290,6 +392,7 @@ __device__ __forceinline__ void decompress_ring_body(const DecompressArgs& a, in
     const bool wraps = (2 * t) % RING_SLOTS == RING_SLOTS - 1;
     const uint32_t* hA = a.hA;
     asm volatile("" : "+s"(hA));  // row t's entries are loaded here, not all hoisted to the start
+    T x[ER];
 #pragma unroll
     for (int e = 0; e < ER; ++e) {
       const int i = min(i0 + e, a.m - 1);
@@ -298,18 +401,21 @@ __device__ __forceinline__ void decompress_ring_body(const DecompressArgs& a, in
       uint32_t off = lo + h * uint32_t(sizeof(T));
       if (wraps) off = min(off, off - uint32_t(RING_LDS));  // past slot 4: back to slot 0
       const T val = *reinterpret_cast<const T*>(smem_raw + off);
-      v[e][t] = signed_by(val, ea, eb[t]);
+      x[e] = signed_by(val, ea, eb[t]);
     }
     // formed here, after all of the row's gathers are issued: the entries
-    // are not kept live until the median
+    // are not kept live until the selection, which overlaps the copies in
+    // flight instead of following the last row
 #pragma unroll
-    for (int e = 0; e < ER; ++e) asm volatile("" : "+v"(v[e][t]));
-  }
+    for (int e = 0; e < ER; ++e) asm volatile("" : "+v"(x[e]));
+#pragma unroll
+    for (int e = 0; e < ER; ++e) select_push<T, D, t>(v[e], x[e]);
+  });
   if (!jok) return;
 #pragma unroll
   for (int e = 0; e < ER; ++e) {
     if (i0 + e >= a.m) break;
-    emit<T>(a, i0 + e, j, median_network<T, D>(v[e]));
+    emit<T>(a, i0 + e, j, select_result<T, D>(v[e]));
   }
 }
 

@@ -114,6 +114,29 @@ def test_median_rank_any_d(d):
         assert np.array_equal(sk.decompress(), orc.decompress(variant, vals, hA, hB))
 
 
+@pytest.mark.parametrize("d", list(range(1, 17)))
+@pytest.mark.parametrize("ties", [False, True])
+def test_median_rank_f32(d, ties):
+    # FP32 entries from d >= 10 on keep only the d - d/2 largest values seen
+    # so far; small integer inputs make many equal values (and zeros of both
+    # signs), which the selection must still place at rank d/2
+    print(f"d={d} ties={ties}", flush=True)
+    rng = np.random.default_rng(300 + d)
+    if ties:
+        AT = rng.integers(-2, 3, (6, 45)).astype(np.float32)
+        B = rng.integers(-2, 3, (6, 38)).astype(np.float32)
+    else:
+        AT = rng.standard_normal((20, 45)).astype(np.float32)
+        B = rng.standard_normal((20, 38)).astype(np.float32)
+    for variant in ("fft", "fwht"):
+        sk = cma.compress(variant, AT, B, d, 128, seed=d, dtype=np.float32)
+        vals = sk.values()
+        hA, hB = sk.hashes()
+        want = orc.decompress(variant, vals.astype(np.float64), hA, hB)
+        assert np.array_equal(sk.decompress().astype(np.float64), want), variant
+        sk.free()
+
+
 @pytest.mark.parametrize("variant", ["fft", "fwht"])
 @pytest.mark.parametrize("d", [1, 2, 7, 15, 16])
 def test_decompress_f32_64k_rows(variant, d):
