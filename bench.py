@@ -411,8 +411,8 @@ def bench_serve(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--plain", action="store_true", help="time the unfused path instead")
     ap.add_argument("--workload", default=DEFAULT_WORKLOAD,
@@ -427,6 +427,9 @@ def main():
     ap.add_argument("--hbm-gb", type=int, default=64, help="serve: server HBM budget (GiB)")
     ap.add_argument("--plain-baseline", action="store_true",
                     help="serve: also replay the arrivals through the unfused path")
+    ap.add_argument("--kernel-times", action="store_true",
+                    help="time every kernel class with HIP events (default: only the "
+                         "dominant one, since each event pair slows the stream)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copies of MIB MiB (the known-size "
                          "kernel that scales FETCH_SIZE / WRITE_SIZE counter readings)")
@@ -471,7 +474,8 @@ def main():
     for _ in range(args.warmup):
         step()
     cma.synchronize()
-    cma.timing_enable(True)
+    kname = "plain_transform" if args.plain else "compress"
+    cma.timing_enable(True, only=None if args.kernel_times else kname)
     cma.timing_reset()
     cma.synchronize()
     t0 = time.perf_counter()
@@ -479,7 +483,6 @@ def main():
         step()
     cma.synchronize()
     elapsed = time.perf_counter() - t0
-    kname = "plain_transform" if args.plain else "compress"
     k_ms, k_n = cma.timing_get(kname)
     totals = {k: cma.timing_get(k) for k in ("hash", "compress", "reduce", "combine",
                                             "decompress", "plain_sketch", "plain_transform",

@@ -76,6 +76,7 @@ def lib():
             ctypes.c_int, ctypes.c_int, ctypes.c_uint64, ctypes.c_int, ctypes.c_int]
     L.cm_gemm.argtypes = [ctypes.c_int] * 3 + [_vp] * 3
     L.cm_timing_enable.argtypes = [ctypes.c_int]
+    L.cm_timing_only.argtypes = [ctypes.c_char_p]
     L.cm_timing_get.argtypes = [ctypes.c_char_p, _f64p, _i64p]
     L.cm_batch_plan.argtypes = [ctypes.c_int, _vp, ctypes.c_uint, ctypes.c_int,
                                 ctypes.POINTER(_vp)]
@@ -280,7 +281,10 @@ def synchronize():
     check(lib().cm_synchronize())
 
 
-def timing_enable(on=True):
+def timing_enable(on=True, only=None):
+    """Time kernel launches with HIP events; `only` restricts the timing to one
+    kernel class (e.g. "compress"), so the other launches run without events."""
+    check(lib().cm_timing_only(only.encode() if only else None))
     check(lib().cm_timing_enable(1 if on else 0))
 
 

@@ -63,6 +63,7 @@ struct State {
   size_t pinned_n = 0;
   std::map<std::pair<int, int>, void*> twiddles;  // (log2 b, f64) -> table
   bool timing = false;
+  std::string timing_only;  // empty: every kernel class
   struct Rec {
     std::string name;
     hipEvent_t a, b;
@@ -138,13 +139,14 @@ hipEvent_t get_event() {
 template <class F>
 int timed(const char* name, F&& launch) {
   hipEvent_t a = nullptr, b = nullptr;
-  if (g.timing) {
+  const bool on = g.timing && (g.timing_only.empty() || g.timing_only == name);
+  if (on) {
     a = get_event();
     b = get_event();
     hipEventRecord(a, g.stream);
   }
   const hipError_t e = launch();
-  if (g.timing) {
+  if (on) {
     hipEventRecord(b, g.stream);
     g.pending.push_back({name, a, b});
   }
@@ -922,6 +924,13 @@ int cm_timing_enable(int on) {
   std::lock_guard<std::recursive_mutex> lk(g_mu);
   resolve_timing();
   g.timing = on != 0;
+  return CM_OK;
+}
+
+int cm_timing_only(const char* name) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  resolve_timing();
+  g.timing_only = name ? name : "";
   return CM_OK;
 }
 

@@ -158,6 +158,11 @@ int cm_synchronize(void);
  * launch count of the kernel class `name` ("compress", "reduce", "combine",
  * "decompress", "hash", "plain_sketch", "plain_transform", "plain_mac"). */
 int cm_timing_enable(int on);
+/* Restricts the timing to the kernel class `name` (NULL or "" restores every
+ * class).  Each event pair stalls the stream briefly, so a throughput
+ * measurement times only its dominant class.  No reference counterpart: it
+ * serves the measurement harness, as cm_timing_enable does. */
+int cm_timing_only(const char* name);
 int cm_timing_reset(void);
 int cm_timing_get(const char* name, double* total_ms, int64_t* launches);
 

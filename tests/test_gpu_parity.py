@@ -301,3 +301,23 @@ def test_fft_register_kernel_matches_lds_kernel(m, n, p, b):
     assert rel_l2(v1, want) <= 1e-5
     assert rel_l2(v2, want) <= 1e-5
     assert rel_l2(v3, want) <= 1e-5  # single-wave kernel
+
+
+def test_timing_only_one_class():
+    # events bracket only the named kernel class; None restores every class
+    AT = np.random.default_rng(5).standard_normal((64, 96)).astype(np.float32)
+    B = np.random.default_rng(6).standard_normal((64, 80)).astype(np.float32)
+    try:
+        for only, want in (("compress", {"compress"}), (None, {"compress", "hash", "decompress"})):
+            cma.timing_enable(True, only=only)
+            cma.timing_reset()
+            sk = cma.compress("fwht", AT, B, 5, 1 << 14, seed=2, dtype=np.float32)
+            sk.decompress()
+            timed = {k for k in ("compress", "hash", "reduce", "combine", "decompress")
+                     if cma.timing_get(k)[1] > 0}
+            assert want <= timed and ("compress" in timed)
+            if only:
+                assert timed == {only}
+            sk.free()
+    finally:
+        cma.timing_enable(False)
