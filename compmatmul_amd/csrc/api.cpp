@@ -131,7 +131,7 @@ hipEvent_t get_event() {
     return e;
   }
   hipEvent_t e = nullptr;
-  hipEventCreate(&e);
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;  // that launch goes untimed
   return e;
 }
 
@@ -139,16 +139,24 @@ hipEvent_t get_event() {
 template <class F>
 int timed(const char* name, F&& launch) {
   hipEvent_t a = nullptr, b = nullptr;
-  const bool on = g.timing && (g.timing_only.empty() || g.timing_only == name);
+  bool on = g.timing && (g.timing_only.empty() || g.timing_only == name);
   if (on) {
     a = get_event();
     b = get_event();
-    hipEventRecord(a, g.stream);
+    on = a && b && hipEventRecord(a, g.stream) == hipSuccess;
+    if (!on) {
+      if (a) g.event_pool.push_back(a);
+      if (b) g.event_pool.push_back(b);
+    }
   }
   const hipError_t e = launch();
   if (on) {
-    hipEventRecord(b, g.stream);
-    g.pending.push_back({name, a, b});
+    if (hipEventRecord(b, g.stream) == hipSuccess) {
+      g.pending.push_back({name, a, b});
+    } else {
+      g.event_pool.push_back(a);
+      g.event_pool.push_back(b);
+    }
   }
   if (e != hipSuccess) return fail(CM_EDEVICE, std::string(name) + " launch failed: " + hipGetErrorString(e));
   return CM_OK;
@@ -156,10 +164,10 @@ int timed(const char* name, F&& launch) {
 
 void resolve_timing() {
   if (g.pending.empty()) return;
-  hipStreamSynchronize(g.stream);
+  (void)hipStreamSynchronize(g.stream);
   for (auto& r : g.pending) {
     float ms = 0.f;
-    hipEventElapsedTime(&ms, r.a, r.b);
+    (void)hipEventElapsedTime(&ms, r.a, r.b);
     auto& t = g.totals[r.name];
     t.first += ms;
     t.second += 1;
@@ -169,7 +177,6 @@ void resolve_timing() {
   g.pending.clear();
 }
 
-bool is_pow2(int64_t x) { return x > 0 && (x & (x - 1)) == 0; }
 int ilog2(int64_t x) {
   int l = 0;
   while ((int64_t(1) << (l + 1)) <= x) ++l;
@@ -757,9 +764,9 @@ int cm_hash_arrays(int family, int d, int b, uint64_t seed, int m, int p, uint32
     if (size_t(d) * m) CM_HIP(hipMemcpyAsync(outA, dA, size_t(d) * m * 4, hipMemcpyDeviceToHost, g.stream));
     if (size_t(d) * p) CM_HIP(hipMemcpyAsync(outB, dB, size_t(d) * p * 4, hipMemcpyDeviceToHost, g.stream));
   }
-  hipStreamSynchronize(g.stream);
-  if (tA.p) hipFree(tA.p);
-  if (tB.p) hipFree(tB.p);
+  (void)hipStreamSynchronize(g.stream);
+  if (tA.p) (void)hipFree(tA.p);
+  if (tB.p) (void)hipFree(tB.p);
   return rc;
 }
 
@@ -860,10 +867,10 @@ int cm_sketch_hashes(const cm_sketch* sk, uint32_t* outA, uint32_t* outB) {
 void cm_sketch_free(cm_sketch* sk) {
   if (!sk) return;
   std::lock_guard<std::recursive_mutex> lk(g_mu);
-  if (g.stream) hipStreamSynchronize(g.stream);
-  if (sk->hA) hipFree(sk->hA);
-  if (sk->hB) hipFree(sk->hB);
-  if (sk->P) hipFree(sk->P);
+  if (g.stream) (void)hipStreamSynchronize(g.stream);
+  if (sk->hA) (void)hipFree(sk->hA);
+  if (sk->hB) (void)hipFree(sk->hB);
+  if (sk->P) (void)hipFree(sk->P);
   delete sk;
 }
 
@@ -878,7 +885,7 @@ int cm_dev_alloc(size_t bytes, void** ptr) {
 
 int cm_dev_free(void* ptr) {
   std::lock_guard<std::recursive_mutex> lk(g_mu);
-  if (g.stream) hipStreamSynchronize(g.stream);
+  if (g.stream) (void)hipStreamSynchronize(g.stream);
   if (ptr) CM_HIP(hipFree(ptr));
   return CM_OK;
 }

@@ -687,7 +687,6 @@ __device__ __forceinline__ void grp_unit(const RegJob& j, int u, float* S, float
 #pragma unroll
   for (int r = 0; r < R; ++r) acc[r] = 0.f;
   constexpr bool paired = PAIRED;
-  const int tid = threadIdx.x;
   // paired lists are single-segment: keep their first groups in registers
   uint32_t ca[CA], cb[CA];
   GroupCounts ga{0, 0, 0}, gb{0, 0, 0};

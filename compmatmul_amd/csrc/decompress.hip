@@ -107,12 +107,9 @@ __device__ __forceinline__ T median_network(T (&v)[D]) {
 }
 
 // x clamped to [lo, hi] (lo <= hi): the middle one of the three values,
-// returned unchanged (one v_med3_f32 in FP32)
+// returned unchanged (one v_med3_f32)
 __device__ __forceinline__ float vclamp(float x, float lo, float hi) {
   return __builtin_amdgcn_fmed3f(x, lo, hi);
-}
-__device__ __forceinline__ double vclamp(double x, double lo, double hi) {
-  return fmax(lo, fmin(x, hi));
 }
 
 // Streaming form of the same selection, for kernels that receive the D values

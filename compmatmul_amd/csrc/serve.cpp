@@ -149,6 +149,7 @@ struct cm_server {
   uint64_t next_ticket = 1;
   std::vector<Slot> slots;
   std::thread thread;
+  int device = 0;  // the library's device at creation; the dispatcher thread selects it
   bool stop = false;
   cm_server_stats stats{};
 };
@@ -414,7 +415,7 @@ bool launch_wave(cm_server* sv, Slot& w) {
     if (rc) err = cm_last_error();
   }
   if (rc != CM_OK) {
-    hipStreamSynchronize(w.s);
+    (void)hipStreamSynchronize(w.s);
     if (w.bt) cm::batch_destroy(w.bt);
     w.bt = nullptr;
     std::lock_guard<std::mutex> lk2(sv->mu);
@@ -435,7 +436,7 @@ bool launch_wave(cm_server* sv, Slot& w) {
 }
 
 void dispatcher(cm_server* sv) {
-  hipSetDevice(0);
+  (void)hipSetDevice(sv->device);  // the device the pool and streams were made on
   for (;;) {
     bool progress = false;
     for (Slot& w : sv->slots) {
@@ -488,6 +489,7 @@ int cm_server_create(size_t hbm_bytes, int max_waves, int max_wave_products, int
   int rc = cm::rt::init();
   if (rc) return rc;
   auto sv = std::make_unique<cm_server>();
+  if (hipGetDevice(&sv->device) != hipSuccess) return error(CM_EDEVICE, "no current device");
   sv->budget = align_up(hbm_bytes);
   sv->max_waves = max_waves;
   sv->max_wave = max_wave_products;
@@ -502,10 +504,10 @@ int cm_server_create(size_t hbm_bytes, int max_waves, int max_wave_products, int
     if (hipStreamCreateWithFlags(&w.s, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&w.done, hipEventDisableTiming) != hipSuccess) {
       for (Slot& x : sv->slots) {
-        if (x.s) hipStreamDestroy(x.s);
-        if (x.done) hipEventDestroy(x.done);
+        if (x.s) (void)hipStreamDestroy(x.s);
+        if (x.done) (void)hipEventDestroy(x.done);
       }
-      hipFree(sv->base);
+      (void)hipFree(sv->base);
       return error(CM_EDEVICE, "stream or event creation failed");
     }
   }
@@ -673,12 +675,12 @@ int cm_server_destroy(cm_server* sv) {
     std::lock_guard<std::recursive_mutex> lk(cm::rt::mutex());
     for (Slot& w : sv->slots) {
       if (w.s) {
-        hipStreamSynchronize(w.s);
-        hipStreamDestroy(w.s);
+        (void)hipStreamSynchronize(w.s);
+        (void)hipStreamDestroy(w.s);
       }
-      if (w.done) hipEventDestroy(w.done);
+      if (w.done) (void)hipEventDestroy(w.done);
     }
-    if (sv->base) hipFree(sv->base);
+    if (sv->base) (void)hipFree(sv->base);
   }
   delete sv;
   return CM_OK;
