@@ -177,8 +177,24 @@ class Sketch:
         self.handle = _vp()
         self.variant, self.dtype, self.d, self.b, self.m, self.p = variant, dtype, d, b, m, p
 
+    def _check_out(self, out):
+        """ValueError unless `out` can take the m x p estimate in this dtype."""
+        if out.shape != (self.m, self.p):
+            raise ValueError(f"out has shape {out.shape}, expected {(self.m, self.p)}")
+        if np.dtype(out.dtype) != np.dtype(self.dtype):
+            raise ValueError(f"out has dtype {out.dtype}, expected {np.dtype(self.dtype)}")
+        if isinstance(out, np.ndarray):
+            if not out.flags.c_contiguous:
+                raise ValueError("out must be C-contiguous")
+            if not out.flags.writeable:
+                raise ValueError("out must be writeable")
+        elif not isinstance(out, DeviceArray):
+            raise ValueError("out must be a numpy array or a DeviceArray")
+
     def decompress(self, out=None):
         """Median-of-d estimate of C (m x p): numpy array or DeviceArray `out`."""
+        if out is not None:
+            self._check_out(out)
         if isinstance(out, DeviceArray):
             check(lib().cm_decompress(self.handle, out.ptr, self.p, DEVICE))
             return out

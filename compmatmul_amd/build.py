@@ -7,7 +7,8 @@
 * ``multiply``: the command-line tool (cli/), and ``tests/cpp/tests_runner``,
   the C++ known-answer test executable, both linked against the library.
 
-Usage: ``python -m compmatmul_amd.build`` (or ``build()`` from Python).
+Usage: ``python -m compmatmul_amd.build`` (or ``build()`` from Python);
+``--changed`` recompiles only the sources changed since their objects.
 """
 import concurrent.futures as cf
 import os
@@ -33,7 +34,7 @@ HIP_FLAGS = [
     "-Wall", "-Wno-unused-command-line-argument",
 ]
 
-SOURCES = ["hash.hip", "compress.hip", "compress_reg.hip", "compress_grp.hip", "compress_fft_reg.hip", "compress_fft_grp.hip", "decompress.hip",
+SOURCES = ["hash.hip", "balance.hip", "compress.hip", "compress_reg.hip", "compress_grp.hip", "compress_fft_reg.hip", "compress_fft_grp.hip", "decompress.hip",
            "plain.hip", "sparse.hip", "fft_mfma.hip", "api.cpp", "batch.cpp", "serve.cpp", "host_hash.cpp"]
 
 
@@ -44,19 +45,26 @@ def _run(cmd):
     return r
 
 
-def _compile(src):
+def _compile(src, changed_only=False):
     out = os.path.join(OBJ, os.path.splitext(src)[0] + ".o")
     path = os.path.join(CSRC, src)
+    if changed_only and os.path.exists(out):
+        # skip an object newer than its source and every shared header
+        deps = [path] + [os.path.join(CSRC, h) for h in os.listdir(CSRC) if h.endswith(".hpp")]
+        deps.append(os.path.join(ROOT, "include", "compmatmul_amd.h"))
+        if os.path.getmtime(out) > max(os.path.getmtime(d) for d in deps):
+            return out
     lang = ["-x", "hip"] if src.endswith(".hip") else []
     _run([HIPCC] + HIP_FLAGS + lang + ["-c", path, "-o", out])
     return out
 
 
-def build(verbose=False):
+def build(verbose=False, changed_only=False):
+    """Compile and link everything; changed_only=True reuses up-to-date objects."""
     os.makedirs(OBJ, exist_ok=True)
     srcs = [s for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
     with cf.ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
-        objs = list(ex.map(_compile, srcs))
+        objs = list(ex.map(lambda s: _compile(s, changed_only), srcs))
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", LIB + ".tmp"] + objs)
     os.replace(LIB + ".tmp", LIB)
     import pybind11
@@ -82,5 +90,5 @@ def build(verbose=False):
 
 
 if __name__ == "__main__":
-    build(verbose=True)
+    build(verbose=True, changed_only="--changed" in sys.argv)
     sys.exit(0)

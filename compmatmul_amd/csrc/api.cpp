@@ -57,7 +57,7 @@ struct DevBuf {
 struct State {
   int device = -1;
   hipStream_t stream = nullptr;
-  DevBuf partial, V, params, tables, AT, B, C, counter, oi, oj, ov, plainA, plainB, entries;
+  DevBuf partial, V, params, tables, AT, B, C, counter, oi, oj, ov, plainA, plainB, entries, scale;
   void* fft_t2 = nullptr;  // W_2048^(L bitrev5(q)) table of the FFT register kernel
   void* pinned = nullptr;
   size_t pinned_n = 0;
@@ -380,9 +380,15 @@ int compress_device(cm_sketch* sk, const void* AT, int64_t ld_at, const void* B,
   const size_t cplx = fwht ? 1 : 2;
   const int max_lb2 = cm::compress_lds_max_lb2(fwht, f64);
   const void* tw = nullptr;
+  // FFT: per-k power-of-two balance of the pair-packed rows, computed by
+  // the first launch of the compress stage (balance.hip)
+  cm::PairScaleArgs psa{f64, AT, ld_at, B, ld_b, m, n, p, nullptr};
+  auto with_scales = [&]() { return fwht ? hipSuccess : cm::launch_pair_scale(psa, g.stream); };
   if (!fwht) {
     int rc = twiddles(lb, f64, &tw);
     if (rc) return rc;
+    if ((rc = ensure_buf(g.scale, std::max<size_t>(1, size_t(n) * 2 * esz)))) return rc;
+    psa.out = g.scale.p;
   }
   if (flags & CM_FLAG_PLAIN) {
     if (lb > max_lb2 + (fwht ? 1 : 0))
@@ -410,10 +416,16 @@ int compress_device(cm_sketch* sk, const void* AT, int64_t ld_at, const void* B,
       pa.lb = lb;
       pa.t = t;
       pa.tw = tw;
+      pa.scale = psa.out;
       pa.SA = g.plainA.p;
       pa.SB = g.plainB.p;
       pa.acc = static_cast<char*>(g.partial.p) + size_t(t) * b * cplx * esz;
-      if ((rc = timed("plain_sketch", [&] { return cm::launch_plain_sketch(pa, g.stream); }))) return rc;
+      if ((rc = timed("plain_sketch", [&] {
+             // the balance scales once, ahead of the first repetition's sketch
+             const hipError_t e = t == 0 ? with_scales() : hipSuccess;
+             return e != hipSuccess ? e : cm::launch_plain_sketch(pa, g.stream);
+           })))
+        return rc;
       if ((rc = timed("plain_transform", [&] { return cm::launch_plain_transform(pa, g.stream); }))) return rc;
       if ((rc = timed("plain_mac", [&] { return cm::launch_plain_mac(pa, g.stream); }))) return rc;
     }
@@ -458,10 +470,15 @@ int compress_device(cm_sketch* sk, const void* AT, int64_t ld_at, const void* B,
     fa.rows = rows;
     fa.nchunks = nchunks;
     fa.tw = twb;
+    fa.scale = static_cast<const float2*>(psa.out);
     fa.Z0 = static_cast<float2*>(g.plainA.p);
     fa.Z1 = static_cast<float2*>(g.plainB.p);
     fa.partial = static_cast<float*>(g.partial.p);
-    if ((rc = timed("compress", [&] { return cm::launch_fft_bf16_compress(fa, g.stream); }))) return rc;
+    if ((rc = timed("compress", [&] {
+           const hipError_t e = with_scales();
+           return e != hipSuccess ? e : cm::launch_fft_bf16_compress(fa, g.stream);
+         })))
+      return rc;
     const int f64_reduce = (flags & CM_FLAG_F64_REDUCE) || n > 16384;
     cm::ReduceArgs ra{0, 0, f64_reduce, g.partial.p, nchunks, d, lb, cm::kRegLb2, twb, g.V.p};
     if ((rc = timed("reduce", [&] { return cm::launch_reduce_inverse(ra, g.stream); }))) return rc;
@@ -521,8 +538,11 @@ int compress_device(cm_sketch* sk, const void* AT, int64_t ld_at, const void* B,
     j.lists = g.entries.p;
     j.tw = twv;
     j.t2 = t2;
+    j.scale = static_cast<const float2*>(psa.out);
     j.partial = static_cast<float*>(g.partial.p);
     if ((rc = timed("compress", [&] {
+           const hipError_t e = with_scales();
+           if (e != hipSuccess) return e;
            if (grp) return fwht ? cm::launch_fwht_grp(j, g.stream) : cm::launch_fft_grp(j, g.stream);
            return fwht ? cm::launch_fwht_reg(j, g.stream) : cm::launch_fft_reg(j, g.stream);
          })))
@@ -567,8 +587,13 @@ int compress_device(cm_sketch* sk, const void* AT, int64_t ld_at, const void* B,
   caa.nchunks = int(nchunks);
   caa.kper = kper;
   caa.tw = tw;
+  caa.scale = psa.out;
   caa.partial = g.partial.p;
-  if ((rc = timed("compress", [&] { return cm::launch_compress_lds(caa, g.stream); }))) return rc;
+  if ((rc = timed("compress", [&] {
+         const hipError_t e = with_scales();
+         return e != hipSuccess ? e : cm::launch_compress_lds(caa, g.stream);
+       })))
+    return rc;
   const int f64_reduce = (flags & CM_FLAG_F64_REDUCE) || n > 16384;
   cm::ReduceArgs ra{fwht, f64, f64_reduce, g.partial.p, int(nchunks), d, lb, lb2, tw, g.V.p};
   if ((rc = timed("reduce", [&] { return cm::launch_reduce_inverse(ra, g.stream); }))) return rc;
@@ -639,8 +664,11 @@ int decompress_locked(const cm_sketch* sk, void* C, int64_t ldc, int mem, int re
   const size_t esz = sk->f64 ? 8 : 4;
   int rc;
   if (C && mem == CM_HOST) {
-    if ((rc = ensure_buf(g.C, std::max<size_t>(1, size_t(sk->m) * ldc * esz)))) return rc;
+    // the device copy is dense (pitch p); the copy back writes only the m x p
+    // window of the caller's matrix, so columns p .. ldc-1 stay untouched
+    if ((rc = ensure_buf(g.C, std::max<size_t>(1, size_t(sk->m) * sk->p * esz)))) return rc;
     da.C = g.C.p;
+    da.ldc = sk->p;
   } else {
     da.C = C;
   }
@@ -659,9 +687,9 @@ int decompress_locked(const cm_sketch* sk, void* C, int64_t ldc, int mem, int re
     da.ov = g.ov.p;
   }
   if ((rc = timed("decompress", [&] { return cm::launch_decompress(da, g.stream); }))) return rc;
-  if (C && mem == CM_HOST && sk->m > 0) {
-    const size_t bytes = (size_t(sk->m - 1) * ldc + sk->p) * esz;
-    CM_HIP(hipMemcpyAsync(C, g.C.p, bytes, hipMemcpyDeviceToHost, g.stream));
+  if (C && mem == CM_HOST && sk->m > 0 && sk->p > 0) {
+    CM_HIP(hipMemcpy2DAsync(C, size_t(ldc) * esz, g.C.p, size_t(sk->p) * esz, size_t(sk->p) * esz,
+                            size_t(sk->m), hipMemcpyDeviceToHost, g.stream));
     CM_HIP(hipStreamSynchronize(g.stream));
   }
   return CM_OK;
@@ -797,6 +825,7 @@ int cm_decompress(const cm_sketch* sk, void* C, int64_t ldc, int mem) {
   if (!sk) return fail(CM_EINVAL, "null sketch");
   if (!C && int64_t(sk->m) * sk->p > 0) return fail(CM_EINVAL, "null output pointer");
   if (ldc < sk->p) return fail(CM_EINVAL, "ldc smaller than the number of columns");
+  if (mem != CM_HOST && mem != CM_DEVICE) return fail(CM_EINVAL, "mem must be CM_HOST or CM_DEVICE");
   std::lock_guard<std::recursive_mutex> lk(g_mu);
   return decompress_locked(sk, C, ldc, mem, 0, 0.0, 0);
 }

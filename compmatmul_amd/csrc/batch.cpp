@@ -4,6 +4,8 @@
 // one device arena, and upload the tables that map workgroups and work items
 // to products:
 //   hash        a range of workgroups per product          (k_hash_batch)
+//   balance     FFT products: a workgroup per four inner indices
+//                                                         (k_pair_scale_batch)
 //   schedules   one workgroup per (product, scatter list)  (k_*_lists_batch)
 //   compress    (product, unit) work items in eight XCD queues per variant,
 //               drained by one persistent launch per variant
@@ -58,6 +60,9 @@ struct cm_batch {
   cm::HashArgs* hash = nullptr;
   int2* hash_blocks = nullptr;
   int n_hash_blocks = 0;
+  cm::PairScaleArgs* pscale = nullptr;  // FFT products' balance scales (balance.hip)
+  int2* pscale_blocks = nullptr;
+  int n_pscale_blocks = 0;
   // per compress kernel kind (kind_of): FFT then FWHT, 1, 2 or 4 waves
   static constexpr int kKinds = 6;
   cm::RegJob* jobs[kKinds] = {};
@@ -270,6 +275,8 @@ int build_plan(int count, const cm_product* ps, unsigned flags, int grain, const
   // host tables
   std::vector<cm::HashArgs> hash(count);
   std::vector<int2> hash_blocks;
+  std::vector<cm::PairScaleArgs> pscale;
+  std::vector<int2> pscale_blocks;
   std::vector<cm::RegJob> jobs[kKinds];
   for (int k = 0; k < kKinds; ++k) jobs[k].resize(njobs[k]);
   std::vector<int2> list_blocks[kKinds];
@@ -297,7 +304,7 @@ int build_plan(int count, const cm_product* ps, unsigned flags, int grain, const
   for (int pass = 0; pass < 2; ++pass) {
     cv.off = 0;
     bt->pp.assign(count, {});
-    std::vector<void*> lists(count), partial(count), V(count);
+    std::vector<void*> lists(count), partial(count), V(count), scale(count);
     std::vector<const uint64_t*> mulp(count);
     std::vector<const uint32_t*> tabp(count);
     for (int i = 0; i < count; ++i) {
@@ -321,6 +328,7 @@ int build_plan(int count, const cm_product* ps, unsigned flags, int grain, const
       const size_t chunk_floats = (size_t(q.d) << hp.lbv[i]) * (fw ? 1 : 2);
       partial[i] = cv.take<float>(size_t(std::max(1, hp.nchunks[i])) * chunk_floats);
       V[i] = cv.take<float>(chunk_floats);
+      scale[i] = fw ? nullptr : cv.take<float2>(size_t(q.n));
       if (!hp.mul[i].empty() && hp.mul[i].size() > 256) mulp[i] = cv.take<uint64_t>(hp.mul[i].size());
       if (!hp.tabs[i].empty()) tabp[i] = cv.take<uint32_t>(hp.tabs[i].size());
     }
@@ -329,12 +337,15 @@ int build_plan(int count, const cm_product* ps, unsigned flags, int grain, const
     // tables (filled below on the second pass, uploaded from one host image)
     const size_t tables_begin = (cv.off + 255) & ~size_t(255);
     bt->hash = cv.take<cm::HashArgs>(size_t(count));
+    bt->pscale = cv.take<cm::PairScaleArgs>(size_t(count));
     bt->finish = cv.take<cm::FinishJob>(size_t(count));
     bt->dec = cv.take<cm::DecompressArgs>(size_t(count));
     for (int k = 0; k < kKinds; ++k) bt->jobs[k] = cv.take<cm::RegJob>(size_t(njobs[k]));
 
     // per-product tables and block maps
     hash_blocks.clear();
+    pscale.clear();
+    pscale_blocks.clear();
     sum_blocks.clear();
     red_blocks.clear();
     comb_blocks.clear();
@@ -386,7 +397,14 @@ int build_plan(int count, const cm_product* ps, unsigned flags, int grain, const
       j.lists = lists[i];
       j.tw = tw[i];
       j.t2 = t2;
+      j.scale = static_cast<const float2*>(scale[i]);
       j.partial = static_cast<float*>(partial[i]);
+      if (!fw && q.n > 0) {
+        const cm::PairScaleArgs pa{0, q.AT, q.ld_at, q.B, q.ld_b, q.m, q.n, q.p, scale[i]};
+        const int nb = cm::pair_scale_blocks(pa);
+        for (int g = 0; g < nb; ++g) pscale_blocks.push_back(make_int2(int(pscale.size()), g));
+        pscale.push_back(pa);
+      }
       const int nl = cm::reg_lists_count(fw, q.d, lb);
       for (int g = 0; g < nl; ++g) list_blocks[hp.kind[i]].push_back(make_int2(hp.jobidx[i], g));
 
@@ -426,6 +444,8 @@ int build_plan(int count, const cm_product* ps, unsigned flags, int grain, const
     }
     bt->hash_blocks = cv.take<int2>(hash_blocks.size());
     bt->n_hash_blocks = int(hash_blocks.size());
+    bt->pscale_blocks = cv.take<int2>(pscale_blocks.size());
+    bt->n_pscale_blocks = int(pscale_blocks.size());
     for (int k = 0; k < kKinds; ++k) {
       bt->list_blocks[k] = cv.take<int2>(list_blocks[k].size());
       bt->n_list_blocks[k] = int(list_blocks[k].size());
@@ -537,6 +557,8 @@ int build_plan(int count, const cm_product* ps, unsigned flags, int grain, const
       std::memcpy(img.data() + (static_cast<const char*>(dev) - cv.base - tables_begin), src, bytes);
     };
     put(bt->hash, hash.data(), hash.size() * sizeof(cm::HashArgs));
+    put(bt->pscale, pscale.data(), pscale.size() * sizeof(cm::PairScaleArgs));
+    put(bt->pscale_blocks, pscale_blocks.data(), pscale_blocks.size() * sizeof(int2));
     put(bt->finish, fin.data(), fin.size() * sizeof(cm::FinishJob));
     put(bt->dec, dec.data(), dec.size() * sizeof(cm::DecompressArgs));
     for (int k = 0; k < kKinds; ++k) {
@@ -590,6 +612,11 @@ int enqueue(cm_batch* bt, bool untimed) {
   if ((rc = run("hash", [&] { return cm::launch_hash_batch(bt->hash, bt->hash_blocks, bt->n_hash_blocks, s); })))
     return rc;
   if ((rc = run("zero", [&] { return cm::launch_zero_u32(bt->counters, bt->counter_words, s); })))
+    return rc;
+  if (bt->n_pscale_blocks &&
+      (rc = run("balance", [&] {
+         return cm::launch_pair_scale_batch(bt->pscale, bt->pscale_blocks, bt->n_pscale_blocks, s);
+       })))
     return rc;
   for (int k = 0; k < kKinds; ++k) {
     if (!bt->n_items[k] && !bt->n_list_blocks[k]) continue;

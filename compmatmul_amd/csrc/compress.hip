@@ -21,7 +21,9 @@
 // * Two real transforms per complex FFT.  z = S a + i S' b has spectrum
 //   Z = A + iB, and A[f]B[f] = (Z[f]^2 - conj(Z[-f])^2)/(4i).  Summed over k
 //   this is linear in Q[f] = sum_k Z_k[f]^2, so only Q is accumulated and the
-//   inverse reduces to p[h] = Im(IDFT_b(Q)[h]) / (2b).
+//   inverse reduces to p[h] = Im(IDFT_b(Q)[h]) / (2b).  Rows a_k and b_k are
+//   scaled by the power-of-two pair (2^e_k, 2^-e_k) of balance.hip first, so
+//   the cancellation costs no accuracy when |a_k| and |b_k| differ.
 // * The spectral accumulator lives in registers of the owning workgroup for
 //   its whole chunk of k; chunks are summed afterwards in a fixed order
 //   (reduce kernel), replacing the reference's per-t omp locks.
@@ -163,14 +165,16 @@ __global__ void __launch_bounds__(NT) k_compress_lds(CompressArgs a) {
     cpx<T> acc[MAXR];
 #pragma unroll
     for (int r = 0; r < MAXR; ++r) acc[r] = {T(0), T(0)};
+    const cpx<T>* __restrict__ bal = static_cast<const cpx<T>*>(a.scale);
     for (int k = k0; k < k1; ++k) {
       for (int e = tid; e < b2; e += NT) z[e] = {T(0), T(0)};
       __syncthreads();
       const T* __restrict__ ar = AT + int64_t(k) * a.ld_at;
       const T* __restrict__ br = B + int64_t(k) * a.ld_b;
+      const cpx<T> bk = bal[k];  // power-of-two balance of row k (balance.hip)
       for (int i = tid; i < a.m; i += NT) {
         const uint32_t e = hA[i], h = hbucket(e);
-        const T v = apply_sign(ar[i], hneg(e));
+        const T v = apply_sign(ar[i], hneg(e)) * bk.x;
         const cpx<T> w = tw[(uint64_t(h) * uint64_t(slice)) & bmask];
         cpx<T>* dst = &z[h & (b2 - 1)];
         atomicAdd(&dst->x, v * w.x);
@@ -178,7 +182,7 @@ __global__ void __launch_bounds__(NT) k_compress_lds(CompressArgs a) {
       }
       for (int j = tid; j < a.p; j += NT) {
         const uint32_t e = hB[j], h = hbucket(e);
-        const T v = apply_sign(br[j], hneg(e));
+        const T v = apply_sign(br[j], hneg(e)) * bk.y;
         const cpx<T> w = tw[(uint64_t(h) * uint64_t(slice)) & bmask];
         cpx<T>* dst = &z[h & (b2 - 1)];
         atomicAdd(&dst->x, -v * w.y);  // i * v * w

@@ -15,7 +15,10 @@
 // single-wave kernel), the waves then run the cross-partition step on a
 // share of the positions l each, and wave j runs the 2048-point register
 // DFT (fft2048_reg.hpp) of slice s_j and accumulates Q += z^2.  The partial
-// spectra keep the [chunk][t][slice][2048] layout.
+// spectra keep the [chunk][t][slice][2048] layout.  Every value of row k
+// enters the scatter times its power-of-two balance scale (balance.hip):
+// register-staged segments are scaled as they are staged, the first segment
+// (copied straight into LDS) as it is scattered.
 #include "device_common.hpp"
 #include "fft2048_reg.hpp"
 #include "kernels.hpp"
@@ -211,12 +214,22 @@ __global__ void __launch_bounds__(64 * LW) k_fft_grp_lists_batch(const RegJob* _
   fft_grp_lists_body<LOG_G>(jobs[e.x], e.y);
 }
 
+// Balance scale of a staged value at byte offset `off` of the segment: the
+// staged rows of register-staged segments already carry it (SC = 0); a
+// segment copied straight into LDS carries raw values, and its keys below
+// byte offset `ab` are A's (scale bal.x), the others B's (SC = 1).
+template <int SC>
+__device__ __forceinline__ float staged_scale(uint32_t off, float2 bal, uint32_t ab) {
+  if constexpr (SC == 0) return 1.f;
+  return off < ab ? bal.x : bal.y;
+}
+
 // KIND 0 plain writes, 1 read-add-writes, 2 LDS atomics (as in the
 // single-wave kernel)
-template <int KIND>
+template <int KIND, int SC>
 __device__ __forceinline__ void scatter_section(const uint32_t* __restrict__ lp,
                                                 const float2* __restrict__ cp, int ng, float2* S,
-                                                const float* Rb, int lane) {
+                                                const float* Rb, int lane, float2 bal, uint32_t ab) {
   const char* Rbb = reinterpret_cast<const char*>(Rb);
   char* Sb = reinterpret_cast<char*>(S);
   for (int g0 = 0; g0 < ng; g0 += GB) {
@@ -231,7 +244,11 @@ __device__ __forceinline__ void scatter_section(const uint32_t* __restrict__ lp,
       cf[q] = cp[pos];
     }
 #pragma unroll
-    for (int q = 0; q < GB; ++q) v[q] = *reinterpret_cast<const float*>(Rbb + ((ent[q] >> 16) & 0x1ffcu));
+    for (int q = 0; q < GB; ++q) {
+      const uint32_t o = (ent[q] >> 16) & 0x1ffcu;
+      v[q] = *reinterpret_cast<const float*>(Rbb + o);
+      if (SC) v[q] *= staged_scale<SC>(o, bal, ab);
+    }
 #pragma unroll
     for (int q = 0; q < GB; ++q) {
       if (q < nb) {
@@ -259,9 +276,11 @@ __device__ __forceinline__ void scatter_section(const uint32_t* __restrict__ lp,
 // n0 + n1, atomic after.  The counts are uniform, so the tests are scalar.
 constexpr int SEGB = 12;
 
+template <int SC>
 __device__ __forceinline__ void scatter_segment(const uint32_t* __restrict__ lp,
                                                 const float2* __restrict__ cp, int n0, int n1,
-                                                int ng, float2* S, const float* Rb, int lane) {
+                                                int ng, float2* S, const float* Rb, int lane,
+                                                float2 bal, uint32_t ab) {
   const char* Rbb = reinterpret_cast<const char*>(Rb);
   char* Sb = reinterpret_cast<char*>(S);
   uint32_t ent[SEGB];
@@ -277,8 +296,11 @@ __device__ __forceinline__ void scatter_segment(const uint32_t* __restrict__ lp,
   }
   float v[SEGB];
 #pragma unroll
-  for (int g = 0; g < SEGB; ++g)
-    v[g] = g < ng ? *reinterpret_cast<const float*>(Rbb + ((ent[g] >> 16) & 0x1ffcu)) : 0.f;
+  for (int g = 0; g < SEGB; ++g) {
+    const uint32_t o = (ent[g] >> 16) & 0x1ffcu;
+    v[g] = g < ng ? *reinterpret_cast<const float*>(Rbb + o) : 0.f;
+    if (SC) v[g] *= staged_scale<SC>(o, bal, ab);
+  }
 #pragma unroll
   for (int g = 0; g < SEGB; ++g) {
     if (g >= ng) continue;
@@ -435,9 +457,13 @@ __device__ __forceinline__ void fft_grp_unit(const RegJob& j, int u, float2* S, 
     scn.n0[q] = scn.n1[q] = scn.n2[q] = 0;
     if (q < nseg) seg_counts(sc + q * RCAP, scn.n0[q], scn.n1[q], scn.n2[q]);
   }
+  // A's keys of the first segment end at this byte offset (for the raw
+  // values the LDS copy brings)
+  const uint32_t ab0 = uint32_t(min(j.m, SEG)) * 4u;
   for (int k = k0; k < k1; ++k) {
     const float* ar = j.AT + int64_t(k) * j.ld_at;
     const float* br = j.B + int64_t(k) * j.ld_b;
+    const float2 bal = j.scale[k];  // power-of-two balance of row k (balance.hip)
     {
       float4* S4 = reinterpret_cast<float4*>(Sw);
 #pragma unroll
@@ -456,13 +482,14 @@ __device__ __forceinline__ void fft_grp_unit(const RegJob& j, int u, float2* S, 
           // a full segment from one operand: a uniform base and the lane's
           // unsigned offset, no clamp and no select
           const float* src = base0 >= j.m ? br + (base0 - j.m) : ar + base0;
+          const float sc = base0 >= j.m ? bal.y : bal.x;
 #pragma unroll
-          for (int q = 0; q < SEG / (64 * G); ++q) v[q] = (src + q * 64 * G)[uint32_t(tid)];
+          for (int q = 0; q < SEG / (64 * G); ++q) v[q] = (src + q * 64 * G)[uint32_t(tid)] * sc;
         } else {
 #pragma unroll
           for (int q = 0; q < SEG / (64 * G); ++q) {
             const int key = base0 + min(q * 64 * G + tid, sl - 1);
-            v[q] = key < j.m ? ar[key] : br[key - j.m];
+            v[q] = key < j.m ? ar[key] * bal.x : br[key - j.m] * bal.y;
           }
         }
         __syncthreads();  // every wave is done with the previous segment
@@ -485,16 +512,30 @@ __device__ __forceinline__ void fft_grp_unit(const RegJob& j, int u, float2* S, 
       } else {
         seg_counts(sc + seg * RCAP, n0, n1, n2);
       }
+      if (ASYNC && seg == 0) {  // raw values from the LDS copy: scaled here
+        if (n0 + n1 + n2 <= SEGB) {
+          scatter_segment<1>(ls + off, cs + off, n0, n1, n0 + n1 + n2, Sw, Rb, lane, bal, ab0);
+          off += int64_t(n0 + n1 + n2) * 64;
+          continue;
+        }
+        scatter_section<0, 1>(ls + off, cs + off, n0, Sw, Rb, lane, bal, ab0);
+        off += int64_t(n0) * 64;
+        scatter_section<1, 1>(ls + off, cs + off, n1, Sw, Rb, lane, bal, ab0);
+        off += int64_t(n1) * 64;
+        scatter_section<2, 1>(ls + off, cs + off, n2, Sw, Rb, lane, bal, ab0);
+        off += int64_t(n2) * 64;
+        continue;
+      }
       if (n0 + n1 + n2 <= SEGB) {  // a short segment schedule in one batch
-        scatter_segment(ls + off, cs + off, n0, n1, n0 + n1 + n2, Sw, Rb, lane);
+        scatter_segment<0>(ls + off, cs + off, n0, n1, n0 + n1 + n2, Sw, Rb, lane, bal, 0u);
         off += int64_t(n0 + n1 + n2) * 64;
         continue;
       }
-      scatter_section<0>(ls + off, cs + off, n0, Sw, Rb, lane);
+      scatter_section<0, 0>(ls + off, cs + off, n0, Sw, Rb, lane, bal, 0u);
       off += int64_t(n0) * 64;
-      scatter_section<1>(ls + off, cs + off, n1, Sw, Rb, lane);
+      scatter_section<1, 0>(ls + off, cs + off, n1, Sw, Rb, lane, bal, 0u);
       off += int64_t(n1) * 64;
-      scatter_section<2>(ls + off, cs + off, n2, Sw, Rb, lane);
+      scatter_section<2, 0>(ls + off, cs + off, n2, Sw, Rb, lane, bal, 0u);
       off += int64_t(n2) * 64;
     }
     __syncthreads();  // all partitions complete, and every wave is done reading Rb

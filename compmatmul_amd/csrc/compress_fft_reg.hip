@@ -25,6 +25,8 @@
 //  * The scatter of the m + p nonzeros follows a per-(t, slice) schedule of
 //    conflict-free plain LDS writes with per-item complex coefficients
 //    s * W_b^(h*slice) (times i for B), built once per product.
+//  * Rows k of AT and B are staged times their power-of-two balance scales
+//    (balance.hip), which keep the pair packing accurate at any |a_k|/|b_k|.
 #include "device_common.hpp"
 #include "fft2048_reg.hpp"
 #include "kernels.hpp"
@@ -168,17 +170,18 @@ __global__ void __launch_bounds__(64) k_fft_lists_batch(const RegJob* __restrict
 }
 
 // ---------------------------------------------------------------- transforms
-// stage keys [seg*SEG, seg*SEG + SEG) of the concatenated (A row, B row)
+// stage keys [seg*SEG, seg*SEG + SEG) of the concatenated (A row, B row),
+// A's values times the balance scale bal.x and B's times bal.y
 __device__ __forceinline__ void stage_rows(const float* __restrict__ ar,
                                            const float* __restrict__ br, int m, int keys,
-                                           int seg, float* Rb, int lane) {
+                                           int seg, float2 bal, float* Rb, int lane) {
   const int s0 = seg * SEG;
   const int sl = min(SEG, keys - s0);
   float v[SEG / 64];
 #pragma unroll
   for (int q = 0; q < SEG / 64; ++q) {
     const int key = s0 + min(q * 64 + lane, sl - 1);
-    v[q] = key < m ? ar[key] : br[key - m];
+    v[q] = key < m ? ar[key] * bal.x : br[key - m] * bal.y;
   }
 #pragma unroll
   for (int q = 0; q < SEG / 64; ++q) Rb[q * 64 + lane] = v[q];
@@ -237,8 +240,8 @@ __device__ __forceinline__ void sketch_fft(const float* __restrict__ ar,
                                            const uint32_t* __restrict__ list,
                                            const float2* __restrict__ coef,
                                            const int* __restrict__ segcnt,
-                                           const float2* __restrict__ t2, float2* S, float* Rb,
-                                           int lane, float (&re)[R], float (&im)[R]) {
+                                           const float2* __restrict__ t2, float2 bal, float2* S,
+                                           float* Rb, int lane, float (&re)[R], float (&im)[R]) {
   float4* S4 = reinterpret_cast<float4*>(S);
 #pragma unroll
   for (int j = 0; j < B2 / 2 / 64; ++j) S4[j * 64 + lane] = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -247,7 +250,7 @@ __device__ __forceinline__ void sketch_fft(const float* __restrict__ ar,
   for (int seg = 0; seg < nseg; ++seg) {
     const int* __restrict__ nr = segcnt + seg * RCAP;
     __syncthreads();
-    stage_rows(ar, br, m, keys, seg, Rb, lane);
+    stage_rows(ar, br, m, keys, seg, bal, Rb, lane);
     __syncthreads();
     int nw = 0;
     for (int r = 1; r < RCAP - 1; ++r) nw += nr[r];
@@ -283,8 +286,8 @@ __device__ __forceinline__ void fft_unit(const RegJob& j, int u, float2* S, floa
   for (int r = 0; r < R; ++r) qr[r] = qi[r] = 0.f;
   for (int k = k0; k < k1; ++k) {
     float re[R], im[R];
-    sketch_fft(AT + int64_t(k) * j.ld_at, B + int64_t(k) * j.ld_b, j.m, keys, ls, cs, sc, t2, S,
-               Rb, lane, re, im);
+    sketch_fft(AT + int64_t(k) * j.ld_at, B + int64_t(k) * j.ld_b, j.m, keys, ls, cs, sc, t2,
+               j.scale[k], S, Rb, lane, re, im);
 #pragma unroll
     for (int r = 0; r < R; ++r) {  // Q += z^2
       qr[r] += re[r] * re[r] - im[r] * im[r];

@@ -61,23 +61,28 @@ __global__ void __launch_bounds__(256) k_zero_rows(float4* __restrict__ z, int64
     z[e] = make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
-// pair-packed sketch of row k0 + blockIdx.x into row blockIdx.x of Z
+// pair-packed sketch of row k0 + blockIdx.x into row blockIdx.x of Z, the
+// rows times their power-of-two balance scales (balance.hip)
 __global__ void __launch_bounds__(256) k_sketch_rows(const float* __restrict__ AT, int64_t ld_at,
                                                      const float* __restrict__ B, int64_t ld_b,
                                                      const uint32_t* __restrict__ hA,
                                                      const uint32_t* __restrict__ hB, int m, int p,
-                                                     int k0, int lb, float2* __restrict__ Z) {
+                                                     int k0, int lb, const float2* __restrict__ scale,
+                                                     float2* __restrict__ Z) {
   const int r = blockIdx.x;
   float2* z = Z + (int64_t(r) << lb);
   const float* ar = AT + int64_t(k0 + r) * ld_at;
   const float* br = B + int64_t(k0 + r) * ld_b;
+  const float2 bal = scale[k0 + r];
   for (int i = threadIdx.x; i < m; i += 256) {
     const uint32_t e = hA[i];
-    atomicAdd(&z[hbucket(e)].x, hneg(e) ? -ar[i] : ar[i]);
+    const float v = ar[i] * bal.x;
+    atomicAdd(&z[hbucket(e)].x, hneg(e) ? -v : v);
   }
   for (int j = threadIdx.x; j < p; j += 256) {
     const uint32_t e = hB[j];
-    atomicAdd(&z[hbucket(e)].y, hneg(e) ? -br[j] : br[j]);
+    const float v = br[j] * bal.y;
+    atomicAdd(&z[hbucket(e)].y, hneg(e) ? -v : v);
   }
 }
 
@@ -183,7 +188,7 @@ hipError_t launch_fft_bf16_compress(const Bf16FftArgs& a, hipStream_t s) {
       k_zero_rows<<<dim3(unsigned(std::min<int64_t>(4096, (n4 + 255) / 256))), dim3(256), 0, s>>>(
           reinterpret_cast<float4*>(bufs[0]), n4);
       k_sketch_rows<<<dim3(unsigned(rows)), dim3(256), 0, s>>>(a.AT, a.ld_at, a.B, a.ld_b, hA, hB, a.m,
-                                                              a.p, k0, a.lb, bufs[0]);
+                                                              a.p, k0, a.lb, a.scale, bufs[0]);
       const int waves = int(rows * (N >> 9));
       for (int st = 0; st < stages; ++st)
         k_fft16_stage<<<dim3(unsigned((waves + 3) / 4)), dim3(256), 0, s>>>(

@@ -28,6 +28,23 @@ hipError_t launch_hash(const HashArgs& a, hipStream_t s);
 int hash_batch_blocks(const HashArgs& a);
 hipError_t launch_hash_batch(const HashArgs* jobs, const int2* blocks, int nblocks, hipStream_t s);
 
+// ------------------------------------------------------------- FFT balance
+// Per inner index k, the power-of-two scales (2^e_k, 2^-e_k) applied to row
+// k of AT and row k of B before pair packing, which equalise the two halves
+// of z_k = S a_k + i S' b_k (both 0 when either row is zero); balance.hip.
+struct PairScaleArgs {
+  int f64;                  // element type of AT, B and out
+  const void* AT; int64_t ld_at;
+  const void* B; int64_t ld_b;
+  int m, n, p;
+  void* out;                // n pairs (scale of a_k, scale of b_k)
+};
+int pair_scale_blocks(const PairScaleArgs& a);
+hipError_t launch_pair_scale(const PairScaleArgs& a, hipStream_t s);
+// Many FP32 products: blocks[g] = (job, workgroup within the job).
+hipError_t launch_pair_scale_batch(const PairScaleArgs* jobs, const int2* blocks, int nblocks,
+                                   hipStream_t s);
+
 // ------------------------------------------------------------- compress
 // Sliced fused compress: the spectrum of length b = b1*b2 is cut into b1
 // slices of b2 frequencies; one workgroup owns (repetition t, slice, chunk
@@ -41,6 +58,7 @@ struct CompressArgs {
   int m, n, p, d, lb, lb2;
   int nchunks, kper;
   const void* tw;      // FFT: W_b^j, j < b, as (re, im) pairs of the element type
+  const void* scale;   // FFT: per k, (scale of row k of AT, of row k of B) (balance.hip)
   void* partial;       // [nchunks][d][b1][b2] (x2 for complex)
 };
 int compress_lds_max_lb2(int fwht, int f64);
@@ -64,6 +82,7 @@ struct RegJob {
   void* lists;              // scatter schedules (*_reg_lists_bytes())
   const void* tw;           // FFT: W_b^j (float2), j < max(b, 2048)
   const void* t2;           // FFT: W_2048^(L * bitrev5(q)) at [q * 64 + L]
+  const float2* scale;      // FFT: per k, (scale of row k of AT, of row k of B) (balance.hip)
   float* partial;           // [nchunks][d][b1][2048] float (float2 for FFT)
 };
 inline int reg_lbs(int lb) { return lb > kRegLb2 ? lb - kRegLb2 : 0; }
@@ -133,6 +152,7 @@ struct Bf16FftArgs {
   int m, n, p, d, lb;
   int rows, nchunks;       // inner indices per chunk, chunks
   const void* tw;          // W_b^j (float2)
+  const float2* scale;     // per k, (scale of row k of AT, of row k of B) (balance.hip)
   float2* Z0; float2* Z1;  // rows x b complex each
   float* partial;          // [nchunks][d][b1][2048] complex
 };
@@ -212,6 +232,7 @@ struct PlainArgs {
   int m, n, p, d, lb;
   int t;               // repetition handled by this call
   const void* tw;
+  const void* scale;   // FFT: per k, (scale of row k of AT, of row k of B) (balance.hip)
   void* SA; void* SB;  // n x b (x2 for complex) sketches / spectra
   void* acc;           // b (x2 for complex) spectrum of repetition t
 };

@@ -46,10 +46,13 @@ __global__ void __launch_bounds__(NT) k_plain_sketch(PlainArgs a) {
     for (int e = tid; e < b; e += NT) S[e] = s[e];
   } else {
     cpx<T>* z = reinterpret_cast<cpx<T>*>(smem_raw);
+    const cpx<T> bal = static_cast<const cpx<T>*>(a.scale)[k];  // balance.hip
     for (int e = tid; e < b; e += NT) z[e] = {T(0), T(0)};
     __syncthreads();
-    for (int i = tid; i < a.m; i += NT) atomicAdd(&z[hbucket(hA[i])].x, apply_sign(ar[i], hneg(hA[i])));
-    for (int j = tid; j < a.p; j += NT) atomicAdd(&z[hbucket(hB[j])].y, apply_sign(br[j], hneg(hB[j])));
+    for (int i = tid; i < a.m; i += NT)
+      atomicAdd(&z[hbucket(hA[i])].x, apply_sign(ar[i] * bal.x, hneg(hA[i])));
+    for (int j = tid; j < a.p; j += NT)
+      atomicAdd(&z[hbucket(hB[j])].y, apply_sign(br[j] * bal.y, hneg(hB[j])));
     __syncthreads();
     cpx<T>* Z = static_cast<cpx<T>*>(a.SA) + int64_t(k) * b;
     for (int e = tid; e < b; e += NT) Z[e] = z[e];

@@ -321,3 +321,18 @@ def test_timing_only_one_class():
             sk.free()
     finally:
         cma.timing_enable(False)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_decompress_host_ldc_keeps_padding(dtype):
+    # a caller decompressing into a sub-block of a wider host matrix: only
+    # the m x p window is written, columns p .. ldc-1 keep their contents
+    m, p, pad = 37, 29, 11
+    AT, B, _ = workloads.planted(m, 40, p, 4, 4.0, 17, dtype)
+    sk = cma.compress("fft", AT, B, 5, 1 << 12, seed=3, dtype=dtype)
+    dense = sk.decompress()
+    wide = np.full((m, p + pad), -7.25, dtype)
+    cma.check(cma.lib().cm_decompress(sk.handle, wide.ctypes.data, p + pad, cma.HOST))
+    assert np.array_equal(wide[:, :p], dense)
+    assert np.all(wide[:, p:] == dtype(-7.25))
+    sk.free()

@@ -95,3 +95,23 @@ def test_dropin_validation_messages_without_gpu():
         mod.matmul_fwht(C, A, A, 3, 32, 1, "foo")
     with pytest.raises(TypeError):
         mod.matmul_fwht(C, A, A, 3, 32, -1)
+
+
+@pytest.mark.parametrize("bad", [
+    np.empty((5, 4)),                        # wrong shape
+    np.empty((4, 5), np.float32),            # wrong dtype for a float64 sketch
+    np.empty((5, 4)).T,                      # right shape, not C-contiguous
+])
+def test_sketch_decompress_rejects_bad_out(bad):
+    # the checks run before any library call, so no kernel is launched
+    sk = cma.Sketch(cma.FFT, np.float64, 3, 64, 4, 5)
+    with pytest.raises(ValueError):
+        sk.decompress(out=bad)
+
+
+def test_sketch_decompress_rejects_readonly_out():
+    sk = cma.Sketch(cma.FWHT, np.float32, 3, 64, 4, 5)
+    out = np.empty((4, 5), np.float32)
+    out.flags.writeable = False
+    with pytest.raises(ValueError):
+        sk.decompress(out=out)
