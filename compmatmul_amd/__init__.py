@@ -75,6 +75,8 @@ def lib():
         getattr(L, name).argtypes = [ctypes.c_int] * 3 + [_vp] * 3 + [
             ctypes.c_int, ctypes.c_int, ctypes.c_uint64, ctypes.c_int, ctypes.c_int]
     L.cm_gemm.argtypes = [ctypes.c_int] * 3 + [_vp] * 3
+    L.cm_matmul_noalloc.argtypes = [ctypes.c_int] * 5 + [_vp, ctypes.c_int64] * 3 + [
+        ctypes.c_int, ctypes.c_int, ctypes.c_uint64, ctypes.c_int]
     L.cm_timing_enable.argtypes = [ctypes.c_int]
     L.cm_timing_only.argtypes = [ctypes.c_char_p]
     L.cm_timing_get.argtypes = [ctypes.c_char_p, _f64p, _i64p]

@@ -34,7 +34,7 @@ HIP_FLAGS = [
     "-Wall", "-Wno-unused-command-line-argument",
 ]
 
-SOURCES = ["hash.hip", "balance.hip", "compress.hip", "compress_reg.hip", "compress_grp.hip", "compress_fft_reg.hip", "compress_fft_grp.hip", "decompress.hip",
+SOURCES = ["hash.hip", "balance.hip", "tables.hip", "noalloc.hip", "compress.hip", "compress_reg.hip", "compress_grp.hip", "compress_fft_reg.hip", "compress_fft_grp.hip", "decompress.hip",
            "plain.hip", "sparse.hip", "fft_mfma.hip", "api.cpp", "batch.cpp", "serve.cpp", "host_hash.cpp"]
 
 

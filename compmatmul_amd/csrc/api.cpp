@@ -7,6 +7,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <functional>
+#include <initializer_list>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -121,6 +123,9 @@ int init_locked(int device) {
   if (g.stream) CM_HIP(hipStreamDestroy(g.stream));
   CM_HIP(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
   g.device = device;
+  // the no-allocation entry's scratch address on this device
+  const int rc = cm::noalloc_setup(device);
+  if (rc) return fail(CM_EDEVICE, "no-allocation scratch: hipGetSymbolAddress failed");
   return CM_OK;
 }
 
@@ -195,7 +200,8 @@ int check_family(int family) {
   return CM_OK;
 }
 
-// W_b^j = exp(-2 pi i j / b) for j < b, built in FP64 from one octant
+// W_b^j = exp(-2 pi i j / b) for j < b (tables.hip), built once per (b,
+// element type) and kept for the process
 int twiddles(int lb, int f64, const void** out) {
   auto key = std::make_pair(lb, f64);
   auto it = g.twiddles.find(key);
@@ -203,52 +209,15 @@ int twiddles(int lb, int f64, const void** out) {
     *out = it->second;
     return CM_OK;
   }
-  const int64_t b = int64_t(1) << lb;
-  std::vector<double> re(b), im(b);
-  for (int64_t j = 0; j < b; ++j) {
-    // reduce j to the first octant with exact integer arithmetic
-    const int64_t q = (j * 8) / b;  // octant index 0..7
-    const int64_t r = j - q * b / 8;
-    double c, s;
-    auto cs = [&](int64_t num, double& cc, double& ss) {  // angle 2 pi num / b
-      const double ang = 2.0 * M_PI * double(num) / double(b);
-      cc = std::cos(ang);
-      ss = std::sin(ang);
-    };
-    if (b < 8) {
-      cs(j, c, s);
-    } else {
-      double c0, s0, c1, s1;
-      cs(r, c0, s0);                  // angle in [0, pi/4)
-      cs(b / 8 - r, c1, s1);          // complementary angle
-      switch (q) {                    // cos/sin of q*pi/4 + r-angle
-        case 0: c = c0; s = s0; break;
-        case 1: c = s1; s = c1; break;   // pi/2 - (b/8 - r)
-        case 2: c = -s0; s = c0; break;
-        case 3: c = -c1; s = s1; break;
-        case 4: c = -c0; s = -s0; break;
-        case 5: c = -s1; s = -c1; break;
-        case 6: c = s0; s = -c0; break;
-        default: c = c1; s = -s1; break;
-      }
-    }
-    re[j] = c;
-    im[j] = -s;  // e^{-i angle}
-  }
-  const size_t esz = f64 ? sizeof(double) : sizeof(float);
-  std::vector<unsigned char> host(size_t(b) * 2 * esz);
-  for (int64_t j = 0; j < b; ++j) {
-    if (f64) {
-      reinterpret_cast<double*>(host.data())[2 * j] = re[j];
-      reinterpret_cast<double*>(host.data())[2 * j + 1] = im[j];
-    } else {
-      reinterpret_cast<float*>(host.data())[2 * j] = float(re[j]);
-      reinterpret_cast<float*>(host.data())[2 * j + 1] = float(im[j]);
-    }
-  }
   void* dev = nullptr;
-  CM_HIP(hipMalloc(&dev, host.size()));
-  CM_HIP(hipMemcpy(dev, host.data(), host.size(), hipMemcpyHostToDevice));
+  CM_HIP(hipMalloc(&dev, cm::twiddle_bytes(lb, f64)));
+  hipError_t e = cm::launch_twiddles(lb, f64, dev, g.stream);
+  // batches and server waves read the table from streams of their own
+  if (e == hipSuccess) e = hipStreamSynchronize(g.stream);
+  if (e != hipSuccess) {
+    (void)hipFree(dev);
+    return fail(CM_EDEVICE, std::string("twiddle table: ") + hipGetErrorString(e));
+  }
   g.twiddles[key] = dev;
   *out = dev;
   return CM_OK;
@@ -257,27 +226,74 @@ int twiddles(int lb, int f64, const void** out) {
 // Table of the FFT register kernel: W_2048^(L * bitrev5(q)) at [q*64 + L].
 int fft_t2_table(const void** out) {
   if (!g.fft_t2) {
-    std::vector<float> host(2 * 2048);
-    for (int q = 0; q < 32; ++q) {
-      int br = 0;
-      for (int bit = 0; bit < 5; ++bit) br |= ((q >> bit) & 1) << (4 - bit);
-      for (int L = 0; L < 64; ++L) {
-        const int j = (L * br) % 2048;
-        const double ang = -2.0 * M_PI * double(j) / 2048.0;
-        host[2 * (q * 64 + L)] = float(std::cos(ang));
-        host[2 * (q * 64 + L) + 1] = float(std::sin(ang));
-      }
+    void* dev = nullptr;
+    CM_HIP(hipMalloc(&dev, 2048 * sizeof(float2)));
+    hipError_t e = cm::launch_t2_table(dev, g.stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(g.stream);
+    if (e != hipSuccess) {
+      (void)hipFree(dev);
+      return fail(CM_EDEVICE, std::string("DFT table: ") + hipGetErrorString(e));
     }
-    CM_HIP(hipMalloc(&g.fft_t2, host.size() * 4));
-    CM_HIP(hipMemcpy(g.fft_t2, host.data(), host.size() * 4, hipMemcpyHostToDevice));
+    g.fft_t2 = dev;
   }
   *out = g.fft_t2;
   return CM_OK;
 }
 
+// Device memory of one product: the library's growing buffers, or, for the
+// no-allocation entry, a caller-provided region carved front to back (the
+// twiddle tables are then written into it too).
+struct Workspace {
+  char* base = nullptr;  // nullptr: the library's buffers and cached tables
+  size_t cap = 0, off = 0;
+  bool carved() const { return base != nullptr; }
+  static size_t up(size_t x) { return (x + 255) & ~size_t(255); }
+  // bytes still free in the region (no limit for the library's buffers)
+  size_t room() const { return carved() ? (up(off) < cap ? cap - up(off) : 0) : SIZE_MAX; }
+  int take(DevBuf& buf, size_t bytes, void** out) {
+    if (!carved()) {
+      int rc = ensure_buf(buf, bytes);
+      if (rc) return rc;
+      *out = buf.p;
+      return CM_OK;
+    }
+    const size_t at = up(off);
+    if (at + bytes > cap)
+      return fail(CM_ENOMEM, "the product needs more than the " + std::to_string(cap >> 20) +
+                                 " MiB no-allocation scratch (at least " +
+                                 std::to_string((at + bytes + ((1 << 20) - 1)) >> 20) + " MiB)");
+    *out = base + at;
+    off = at + bytes;
+    return CM_OK;
+  }
+  int twiddles(int lb, int f64, const void** out) {
+    if (!carved()) return ::twiddles(lb, f64, out);
+    DevBuf none;
+    void* p = nullptr;
+    int rc = take(none, cm::twiddle_bytes(lb, f64), &p);
+    if (rc) return rc;
+    CM_HIP(cm::launch_twiddles(lb, f64, p, g.stream));
+    *out = p;
+    return CM_OK;
+  }
+  int t2_table(const void** out) {
+    if (!carved()) return ::fft_t2_table(out);
+    DevBuf none;
+    void* p = nullptr;
+    int rc = take(none, 2048 * sizeof(float2), &p);
+    if (rc) return rc;
+    CM_HIP(cm::launch_t2_table(p, g.stream));
+    *out = p;
+    return CM_OK;
+  }
+};
+
 // Draw h1, h2, s1, s2 and evaluate the packed arrays on the device.
+// With no_alloc, only parameters that travel inside the kernel arguments are
+// taken (multiply families with 4 d words <= 256), so nothing is allocated
+// or uploaded.
 int build_hashes(int family, int d, int lb, uint64_t seed, int m, int p, uint32_t* outA,
-                 uint32_t* outB) {
+                 uint32_t* outB, bool no_alloc = false) {
   cm::HashArgs ha{};
   ha.family = family;
   ha.d = d;
@@ -286,17 +302,23 @@ int build_hashes(int family, int d, int lb, uint64_t seed, int m, int p, uint32_
   ha.p = p;
   ha.outA = outA;
   ha.outB = outB;
-  if (family == cm::kMultiplyAddShift || family == cm::kMultiplyShift) {
-    const int w = family == cm::kMultiplyAddShift ? 2 : 1;
+  const bool mul = family == cm::kMultiplyAddShift || family == cm::kMultiplyShift;
+  const int64_t words = mul ? int64_t(4) * d * cm::param_words(family) : 0;
+  if (mul && words <= 256) {
+    ha.use_inline = 1;
+    for (int f = 0; f < 4; ++f)
+      cm::draw_multiply_into(family, d, seed + uint64_t(f),
+                             ha.inline_params + int64_t(f) * d * cm::param_words(family));
+  } else if (no_alloc) {
+    return fail(CM_ENOTSUP, "the no-allocation entry takes the multiply-add-shift family with "
+                            "d <= 32 or multiply-shift with d <= 64");
+  } else if (mul) {
     std::vector<uint64_t> all, one;
     for (int f = 0; f < 4; ++f) {
       cm::draw_multiply(family, d, seed + uint64_t(f), one);
       all.insert(all.end(), one.begin(), one.end());
     }
-    if (all.size() <= 256) {
-      ha.use_inline = 1;
-      std::copy(all.begin(), all.end(), ha.inline_params);
-    } else {
+    {
       int rc = ensure_buf(g.params, all.size() * 8);
       if (rc) return rc;
       rc = ensure_pinned(all.size() * 8);
@@ -306,7 +328,6 @@ int build_hashes(int family, int d, int lb, uint64_t seed, int m, int p, uint32_
       CM_HIP(hipMemcpyAsync(g.params.p, g.pinned, all.size() * 8, hipMemcpyHostToDevice, g.stream));
       ha.mul_params = static_cast<const uint64_t*>(g.params.p);
     }
-    (void)w;
   } else {
     const int chars = family == cm::kTab8 ? 4 : 2;
     const int64_t full = family == cm::kTab8 ? 256 : 65536;
@@ -332,6 +353,10 @@ int build_hashes(int family, int d, int lb, uint64_t seed, int m, int p, uint32_
     CM_HIP(hipMemcpyAsync(g.tables.p, g.pinned, bytes, hipMemcpyHostToDevice, g.stream));
     ha.tables = static_cast<const uint32_t*>(g.tables.p);
     ha.tab_len = int(tab_len);
+  }
+  if (no_alloc) {
+    CM_HIP(cm::launch_hash(ha, g.stream));
+    return CM_OK;
   }
   return timed("hash", [&] { return cm::launch_hash(ha, g.stream); });
 }
@@ -372,8 +397,23 @@ int stage_in(DevBuf& buf, const void* host, int rows, int cols, int64_t ld, size
   return CM_OK;
 }
 
+// Launch the named stages in order: timed for the library's buffers, plain
+// launches (no events) for a carved region.
+int run_stages(const Workspace& ws,
+               std::initializer_list<std::pair<const char*, std::function<hipError_t()>>> stages) {
+  for (const auto& st : stages) {
+    if (ws.carved()) {
+      const hipError_t e = st.second();
+      if (e != hipSuccess) return fail(CM_EDEVICE, std::string(st.first) + " launch failed: " + hipGetErrorString(e));
+    } else if (int rc = timed(st.first, st.second)) {
+      return rc;
+    }
+  }
+  return CM_OK;
+}
+
 int compress_device(cm_sketch* sk, const void* AT, int64_t ld_at, const void* B, int64_t ld_b,
-                    int n, unsigned flags) {
+                    int n, unsigned flags, Workspace& ws) {
   const int f64 = sk->f64, fwht = sk->fwht, d = sk->d, lb = sk->lb, m = sk->m, p = sk->p;
   const int64_t b = int64_t(1) << lb;
   const size_t esz = f64 ? 8 : 4;
@@ -385,11 +425,12 @@ int compress_device(cm_sketch* sk, const void* AT, int64_t ld_at, const void* B,
   cm::PairScaleArgs psa{f64, AT, ld_at, B, ld_b, m, n, p, nullptr};
   auto with_scales = [&]() { return fwht ? hipSuccess : cm::launch_pair_scale(psa, g.stream); };
   if (!fwht) {
-    int rc = twiddles(lb, f64, &tw);
+    int rc = ws.twiddles(lb, f64, &tw);
     if (rc) return rc;
-    if ((rc = ensure_buf(g.scale, std::max<size_t>(1, size_t(n) * 2 * esz)))) return rc;
-    psa.out = g.scale.p;
+    if ((rc = ws.take(g.scale, std::max<size_t>(1, size_t(n) * 2 * esz), &psa.out))) return rc;
   }
+  if (ws.carved() && (flags & (CM_FLAG_PLAIN | CM_FLAG_BF16_MFMA)))
+    return fail(CM_ENOTSUP, "the plain and BF16 paths allocate their own buffers");
   if (flags & CM_FLAG_PLAIN) {
     if (lb > max_lb2 + (fwht ? 1 : 0))
       return fail(CM_ENOTSUP, "b too large for the plain path (64 KB of LDS per transform)");
@@ -504,22 +545,30 @@ int compress_device(cm_sketch* sk, const void* AT, int64_t ld_at, const void* B,
     int64_t nchunks = std::max<int64_t>(1, slots * rounds / per_chunk);
     nchunks = std::min<int64_t>(nchunks, std::max(1, n));
     const size_t chunk_bytes = (size_t(d) << lbv) * cplx * esz;
-    nchunks = std::max<int64_t>(1, std::min<int64_t>(nchunks, int64_t((size_t(1) << 30) / chunk_bytes)));
-    const int kper = n > 0 ? int((n + nchunks - 1) / nchunks) : 0;
-    nchunks = n > 0 ? (n + kper - 1) / kper : 0;
     const void* twv = nullptr;  // twiddles of the transform length run
     const void* t2 = nullptr;
     int rc;
     if (!fwht) {
-      if ((rc = twiddles(lbv, 0, &twv)) || (rc = fft_t2_table(&t2))) return rc;
+      if (lbv == lb)
+        twv = tw;
+      else if ((rc = ws.twiddles(lbv, 0, &twv)))
+        return rc;
+      if ((rc = ws.t2_table(&t2))) return rc;
     }
-    if ((rc = ensure_buf(g.partial, std::max<size_t>(1, size_t(nchunks) * chunk_bytes)))) return rc;
-    if ((rc = ensure_buf(g.V, chunk_bytes))) return rc;
     const size_t lists_bytes = grp ? (fwht ? cm::fwht_grp_lists_bytes(d, lb, m, p)
                                            : cm::fft_grp_lists_bytes(d, lb, m, p))
                                : fwht ? cm::fwht_reg_lists_bytes(d, lb, m, p)
                                       : cm::fft_reg_lists_bytes(d, lb, m, p);
-    if ((rc = ensure_buf(g.entries, lists_bytes))) return rc;
+    // partial spectra under 1 GiB, and within what a carved region has left
+    const size_t fixed = Workspace::up(lists_bytes) + Workspace::up(chunk_bytes) + 256;
+    const size_t budget = std::min<size_t>(size_t(1) << 30, ws.room() > fixed ? ws.room() - fixed : 0);
+    nchunks = std::max<int64_t>(1, std::min<int64_t>(nchunks, int64_t(budget / chunk_bytes)));
+    const int kper = n > 0 ? int((n + nchunks - 1) / nchunks) : 0;
+    nchunks = n > 0 ? (n + kper - 1) / kper : 0;
+    void *partial = nullptr, *V = nullptr, *lists = nullptr;
+    if ((rc = ws.take(g.partial, std::max<size_t>(1, size_t(nchunks) * chunk_bytes), &partial)) ||
+        (rc = ws.take(g.V, chunk_bytes, &V)) || (rc = ws.take(g.entries, lists_bytes, &lists)))
+      return rc;
     cm::RegJob j{};
     j.AT = static_cast<const float*>(AT);
     j.ld_at = ld_at;
@@ -535,23 +584,22 @@ int compress_device(cm_sketch* sk, const void* AT, int64_t ld_at, const void* B,
     j.lbs = lbs;
     j.kper = kper;
     j.nchunks = int(nchunks);
-    j.lists = g.entries.p;
+    j.lists = lists;
     j.tw = twv;
     j.t2 = t2;
     j.scale = static_cast<const float2*>(psa.out);
-    j.partial = static_cast<float*>(g.partial.p);
-    if ((rc = timed("compress", [&] {
-           const hipError_t e = with_scales();
-           if (e != hipSuccess) return e;
-           if (grp) return fwht ? cm::launch_fwht_grp(j, g.stream) : cm::launch_fft_grp(j, g.stream);
-           return fwht ? cm::launch_fwht_reg(j, g.stream) : cm::launch_fft_reg(j, g.stream);
-         })))
-      return rc;
+    j.partial = static_cast<float*>(partial);
     const int f64_reduce = (flags & CM_FLAG_F64_REDUCE) || n > 16384;
-    cm::ReduceArgs ra{fwht, 0, f64_reduce, g.partial.p, int(nchunks), d, lbv, cm::kRegLb2, twv, g.V.p};
-    if ((rc = timed("reduce", [&] { return cm::launch_reduce_inverse(ra, g.stream); }))) return rc;
-    cm::CombineArgs ca{fwht, 0, g.V.p, d, lbv, cm::kRegLb2, twv, sk->P, lb};
-    return timed("combine", [&] { return cm::launch_combine(ca, g.stream); });
+    cm::ReduceArgs ra{fwht, 0, f64_reduce, partial, int(nchunks), d, lbv, cm::kRegLb2, twv, V};
+    cm::CombineArgs ca{fwht, 0, V, d, lbv, cm::kRegLb2, twv, sk->P, lb};
+    return run_stages(ws, {{"compress", [&] {
+                              const hipError_t e = with_scales();
+                              if (e != hipSuccess) return e;
+                              if (grp) return fwht ? cm::launch_fwht_grp(j, g.stream) : cm::launch_fft_grp(j, g.stream);
+                              return fwht ? cm::launch_fwht_reg(j, g.stream) : cm::launch_fft_reg(j, g.stream);
+                            }},
+                           {"reduce", [&] { return cm::launch_reduce_inverse(ra, g.stream); }},
+                           {"combine", [&] { return cm::launch_combine(ca, g.stream); }}});
   }
 
   const int lb2 = std::min(lb, max_lb2);
@@ -561,14 +609,17 @@ int compress_device(cm_sketch* sk, const void* AT, int64_t ld_at, const void* B,
   int64_t nchunks = std::max<int64_t>(1, (2048 + per_chunk - 1) / per_chunk);
   nchunks = std::min<int64_t>(nchunks, std::max(1, n));
   const size_t chunk_bytes = size_t(d) * b * cplx * esz;
-  const size_t cap = size_t(1) << 30;  // partial spectra stay under 1 GiB
+  // partial spectra under 1 GiB, and within what a carved region has left
+  const size_t fixed = Workspace::up(chunk_bytes) + 256;
+  const size_t cap = std::min<size_t>(size_t(1) << 30, ws.room() > fixed ? ws.room() - fixed : 0);
   nchunks = std::max<int64_t>(1, std::min<int64_t>(nchunks, int64_t(cap / chunk_bytes)));
   const int kper = n > 0 ? int((n + nchunks - 1) / nchunks) : 0;
   nchunks = n > 0 ? (n + kper - 1) / kper : 0;
-  int rc = ensure_buf(g.partial, std::max<size_t>(1, size_t(nchunks) * chunk_bytes));
-  if (rc) return rc;
-  rc = ensure_buf(g.V, chunk_bytes);
-  if (rc) return rc;
+  void *partial = nullptr, *V = nullptr;
+  int rc;
+  if ((rc = ws.take(g.partial, std::max<size_t>(1, size_t(nchunks) * chunk_bytes), &partial)) ||
+      (rc = ws.take(g.V, chunk_bytes, &V)))
+    return rc;
   cm::CompressArgs caa{};
   caa.fwht = fwht;
   caa.f64 = f64;
@@ -588,17 +639,16 @@ int compress_device(cm_sketch* sk, const void* AT, int64_t ld_at, const void* B,
   caa.kper = kper;
   caa.tw = tw;
   caa.scale = psa.out;
-  caa.partial = g.partial.p;
-  if ((rc = timed("compress", [&] {
-         const hipError_t e = with_scales();
-         return e != hipSuccess ? e : cm::launch_compress_lds(caa, g.stream);
-       })))
-    return rc;
+  caa.partial = partial;
   const int f64_reduce = (flags & CM_FLAG_F64_REDUCE) || n > 16384;
-  cm::ReduceArgs ra{fwht, f64, f64_reduce, g.partial.p, int(nchunks), d, lb, lb2, tw, g.V.p};
-  if ((rc = timed("reduce", [&] { return cm::launch_reduce_inverse(ra, g.stream); }))) return rc;
-  cm::CombineArgs ca{fwht, f64, g.V.p, d, lb, lb2, tw, sk->P, lb};
-  return timed("combine", [&] { return cm::launch_combine(ca, g.stream); });
+  cm::ReduceArgs ra{fwht, f64, f64_reduce, partial, int(nchunks), d, lb, lb2, tw, V};
+  cm::CombineArgs ca{fwht, f64, V, d, lb, lb2, tw, sk->P, lb};
+  return run_stages(ws, {{"compress", [&] {
+                            const hipError_t e = with_scales();
+                            return e != hipSuccess ? e : cm::launch_compress_lds(caa, g.stream);
+                          }},
+                         {"reduce", [&] { return cm::launch_reduce_inverse(ra, g.stream); }},
+                         {"combine", [&] { return cm::launch_combine(ca, g.stream); }}});
 }
 
 int compress_locked(int variant, int dtype, int m, int n, int p, const void* AT, int64_t ld_at,
@@ -639,8 +689,9 @@ int compress_locked(int variant, int dtype, int m, int n, int p, const void* AT,
       return rc;
     }
   }
+  Workspace lib_ws;  // the library's buffers
   if ((rc = build_hashes(family, d, sk->lb, seed, m, p, sk->hA, sk->hB)) ||
-      (rc = compress_device(sk, dAT, ld_at, dB, ld_b, n, flags))) {
+      (rc = compress_device(sk, dAT, ld_at, dB, ld_b, n, flags, lib_ws))) {
     if (fresh) cm_sketch_free(sk);
     return rc;
   }
@@ -695,6 +746,72 @@ int decompress_locked(const cm_sketch* sk, void* C, int64_t ldc, int mem, int re
   return CM_OK;
 }
 
+// cm_matmul_noalloc: compress and full decompression of device operands with
+// every intermediate (hash arrays, balance scales, twiddle tables, scatter
+// schedules, partial spectra, inverse, sketch) carved from the fixed device
+// scratch of noalloc.hip: no device or host allocation, no upload.
+int matmul_noalloc_locked(int variant, int dtype, int m, int n, int p, const void* AT,
+                          int64_t ld_at, const void* B, int64_t ld_b, void* C, int64_t ldc,
+                          int d, int b, uint64_t seed, int family) {
+  int rc = init_locked(-1);
+  if (rc) return rc;
+  if (variant != CM_FFT && variant != CM_FWHT) return fail(CM_EINVAL, "variant must be CM_FFT or CM_FWHT");
+  if (dtype != CM_F64 && dtype != CM_F32) return fail(CM_EINVAL, "dtype must be CM_F64 or CM_F32");
+  if ((rc = check_db(d, b)) || (rc = check_family(family))) return rc;
+  if (m < 0 || n < 0 || p < 0) return fail(CM_EINVAL, "matrix dimensions must be non-negative");
+  if (b > (1 << 30)) return fail(CM_ENOTSUP, "b above 2^30 is not supported");
+  if (ld_at < m || ld_b < p || ldc < p)
+    return fail(CM_EINVAL, "leading dimension smaller than the row length");
+  if ((int64_t(n) * m > 0 && !AT) || (int64_t(n) * p > 0 && !B) || (int64_t(m) * p > 0 && !C))
+    return fail(CM_EINVAL, "null operand pointer");
+  void* base = nullptr;
+  size_t cap = 0;
+  if (cm::noalloc_scratch(g.device, &base, &cap))
+    return fail(CM_EDEVICE, "no-allocation scratch is not set up on this device");
+  Workspace ws;
+  ws.base = static_cast<char*>(base);
+  ws.cap = cap;
+  const int fwht = variant == CM_FWHT, f64 = dtype == CM_F64;
+  const size_t esz = f64 ? 8 : 4;
+  cm_sketch sk{};
+  sk.fwht = fwht;
+  sk.f64 = f64;
+  sk.d = d;
+  sk.b = b;
+  sk.lb = ilog2(b);
+  sk.m = m;
+  sk.p = p;
+  sk.family = family;
+  sk.seed = seed;
+  DevBuf none;
+  void *hA = nullptr, *hB = nullptr, *P = nullptr;
+  if ((rc = ws.take(none, std::max<size_t>(4, size_t(d) * m * 4), &hA)) ||
+      (rc = ws.take(none, std::max<size_t>(4, size_t(d) * p * 4), &hB)) ||
+      (rc = ws.take(none, size_t(d) * b * esz, &P)))
+    return rc;
+  sk.hA = static_cast<uint32_t*>(hA);
+  sk.hB = static_cast<uint32_t*>(hB);
+  sk.P = P;
+  if ((rc = build_hashes(family, d, sk.lb, seed, m, p, sk.hA, sk.hB, true)) ||
+      (rc = compress_device(&sk, AT, ld_at, B, ld_b, n, 0, ws)))
+    return rc;
+  cm::DecompressArgs da{};
+  da.fwht = fwht;
+  da.f64 = f64;
+  da.P = sk.P;
+  da.hA = sk.hA;
+  da.hB = sk.hB;
+  da.m = m;
+  da.p = p;
+  da.d = d;
+  da.b = b;
+  da.C = C;
+  da.ldc = ldc;
+  CM_HIP(cm::launch_decompress(da, g.stream));
+  CM_HIP(hipStreamSynchronize(g.stream));
+  return CM_OK;
+}
+
 int matmul_dropin(int variant, int n_a, int n_inner, int n_b, double* C, const double* AT,
                   const double* B, int d, int b, uint64_t seed, int seed_given, int family) {
   std::lock_guard<std::recursive_mutex> lk(g_mu);
@@ -746,6 +863,14 @@ int cm_matmul_fft(int n_a, int n_inner, int n_b, double* C, const double* AT, co
 int cm_matmul_fwht(int n_a, int n_inner, int n_b, double* C, const double* AT, const double* B,
                    int d, int b, uint64_t seed, int seed_given, int family) {
   return matmul_dropin(CM_FWHT, n_a, n_inner, n_b, C, AT, B, d, b, seed, seed_given, family);
+}
+
+int cm_matmul_noalloc(int variant, int dtype, int n_a, int n_inner, int n_b, const void* AT,
+                      int64_t ld_at, const void* B, int64_t ld_b, void* C, int64_t ldc, int d,
+                      int b, uint64_t seed, int family) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  return matmul_noalloc_locked(variant, dtype, n_a, n_inner, n_b, AT, ld_at, B, ld_b, C, ldc, d, b,
+                               seed, family);
 }
 
 int cm_gemm(int n_a, int n_inner, int n_b, double* C, const double* AT, const double* B) {

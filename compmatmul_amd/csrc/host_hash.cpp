@@ -46,21 +46,25 @@ int64_t param_words(int family) {
   }
 }
 
-void draw_multiply(int family, int n, uint64_t seed, std::vector<uint64_t>& out) {
+void draw_multiply_into(int family, int n, uint64_t seed, uint64_t* out) {
   Xoshiro256ss r(seed);
-  out.clear();
   for (int f = 0; f < n; ++f) {
     if (family == kMultiplyShift) {
       uint64_t a;
       do { a = r(); } while ((a & 1) == 0);
-      out.push_back(a);
+      *out++ = a;
     } else {
       const uint64_t a = r();
       const uint64_t b = r();
-      out.push_back(a);
-      out.push_back(b);
+      *out++ = a;
+      *out++ = b;
     }
   }
+}
+
+void draw_multiply(int family, int n, uint64_t seed, std::vector<uint64_t>& out) {
+  out.resize(size_t(n) * size_t(param_words(family)));
+  draw_multiply_into(family, n, seed, out.data());
 }
 
 void draw_tables(int family, int n, uint64_t seed, std::vector<uint32_t>& out) {

@@ -35,6 +35,8 @@ int64_t param_words(int family);
 // Parameters of n functions drawn from one stream seeded with `seed`.
 // Tabulation tables are returned as u32 (n * words entries).
 void draw_multiply(int family, int n, uint64_t seed, std::vector<uint64_t>& out);
+// The same parameters into out[0 .. n * param_words(family)); no allocation.
+void draw_multiply_into(int family, int n, uint64_t seed, uint64_t* out);
 void draw_tables(int family, int n, uint64_t seed, std::vector<uint32_t>& out);
 
 }  // namespace cm

@@ -28,6 +28,14 @@ hipError_t launch_hash(const HashArgs& a, hipStream_t s);
 int hash_batch_blocks(const HashArgs& a);
 hipError_t launch_hash_batch(const HashArgs* jobs, const int2* blocks, int nblocks, hipStream_t s);
 
+// ------------------------------------------------------------- tables
+// W_b^j (j < b) as (re, im) pairs of the element type, and the 2048-point
+// register DFT's per-lane table W_2048^(L * bitrev5(q)) at [q * 64 + L]
+// (float2, 2048 entries); both computed in FP64 on the device (tables.hip).
+size_t twiddle_bytes(int lb, int f64);
+hipError_t launch_twiddles(int lb, int f64, void* out, hipStream_t s);
+hipError_t launch_t2_table(void* out, hipStream_t s);
+
 // ------------------------------------------------------------- FFT balance
 // Per inner index k, the power-of-two scales (2^e_k, 2^-e_k) applied to row
 // k of AT and row k of B before pair packing, which equalise the two halves
@@ -244,6 +252,13 @@ hipError_t launch_plain_mac(const PlainArgs& a, hipStream_t s);
 // CSR (rows x cols, indptr[rows + 1], indices, values) to dense row-major.
 hipError_t launch_csr_dense(const int32_t* indptr, const int32_t* indices, const float* values,
                             int rows, int cols, int64_t ld, float* out, hipStream_t s);
+
+// ------------------------------------------------------------- no-allocation scratch
+// noalloc.hip: the fixed device array of the no-allocation entry point.
+// noalloc_setup() looks up its address on `device` (the current device)
+// once; noalloc_scratch() returns that address and the array's size.
+int noalloc_setup(int device);
+int noalloc_scratch(int device, void** ptr, size_t* bytes);
 
 // ------------------------------------------------------------- gemm
 hipError_t launch_gemm_f64(int m, int n, int p, const double* AT, const double* B,

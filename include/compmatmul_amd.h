@@ -95,6 +95,32 @@ int cm_matmul_fwht(int n_a, int n_inner, int n_b, double* C, const double* AT,
 int cm_gemm(int n_a, int n_inner, int n_b, double* C, const double* AT,
             const double* B);
 
+/* ---------------------------------------------------------- no allocation
+ * Size of the fixed device scratch behind cm_matmul_noalloc (per GPU, held
+ * until the process ends). */
+#define CM_NOALLOC_SCRATCH_BYTES (64u << 20)
+
+/* C = the median-of-d estimate of AT^T B, as cm_matmul_fft / cm_matmul_fwht
+ * compute it, for device operands: AT (n x m, leading dimension ld_at), B
+ * (n x p, ld_b) and C (m x p, ldc) are device pointers of dtype CM_F32 or
+ * CM_F64.  Every intermediate (hash arrays, scatter schedules, partial
+ * spectra, sketch, twiddle tables) lives in a fixed device array of
+ * CM_NOALLOC_SCRATCH_BYTES, so the call allocates no device or host memory
+ * and uploads nothing; it returns CM_ENOMEM, naming the size needed, when a
+ * product's intermediates do not fit, and CM_ENOTSUP for the tabulation
+ * families or more hash parameters than travel in kernel arguments (d > 32
+ * for multiply-add-shift, d > 64 for multiply-shift).  variant CM_FFT or
+ * CM_FWHT; h1, h2, s1, s2 from seed + 0..3.  Synchronous: C is complete on
+ * return.  Every call on a GPU shares that GPU's scratch (it keeps the last
+ * call's contents); calls through the library are serialised by its lock,
+ * and calls that overlapped on one GPU would race on the scratch.
+ * No counterpart in the reference: it is for callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). */
+int cm_matmul_noalloc(int variant, int dtype, int n_a, int n_inner, int n_b, const void* AT,
+                      int64_t ld_at, const void* B, int64_t ld_b, void* C, int64_t ldc, int d,
+                      int b, uint64_t seed, int family);
+
 /* ---------------------------------------------------------------- hashing
  * Packed bucket/sign arrays of a sketch: outA[t*m+i] = h1_t(i) | signbit,
  * outB[t*p+j] = h2_t(j) | signbit, where signbit = 1u<<31 when s1_t(i)
