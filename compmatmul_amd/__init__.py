@@ -96,6 +96,7 @@ def lib():
                                    _i64p, ctypes.POINTER(ctypes.c_double)]
     L.cm_server_decompress.argtypes = [_vp, ctypes.c_uint64, _vp]
     L.cm_server_release.argtypes = [_vp, ctypes.c_uint64]
+    L.cm_server_values.argtypes = [_vp, ctypes.c_uint64, _vp]
     L.cm_server_stats_get.argtypes = [_vp, _vp]
     L.cm_server_destroy.argtypes = [_vp]
     L.cm_host_alloc.argtypes = [ctypes.c_size_t, ctypes.POINTER(_vp)]
@@ -535,7 +536,7 @@ class Server:
             r.C = C.ctypes.data
         t = ctypes.c_uint64()
         check(lib().cm_server_submit(self.handle, ctypes.byref(r), ctypes.byref(t)))
-        self._keep[t.value] = (keep, C, int(max_out), (m, p))
+        self._keep[t.value] = (keep, C, int(max_out), (m, p), (int(d), int(b)))
         return t.value
 
     def wait(self, ticket, timeout=None):
@@ -548,7 +549,7 @@ class Server:
 
     def result(self, ticket):
         """(rows, cols, values, latency_s, C or None) of a finished request."""
-        keep, C, cap, _ = self._keep[ticket]
+        keep, C, cap = self._keep[ticket][:3]
         oi = np.empty(max(1, cap), np.int32)
         oj = np.empty(max(1, cap), np.int32)
         ov = np.empty(max(1, cap), np.float32)
@@ -566,6 +567,12 @@ class Server:
         C = np.empty((m, p), np.float32)
         check(lib().cm_server_decompress(self.handle, ticket, C.ctypes.data))
         return C
+
+    def values(self, ticket):
+        """The request's device-resident sketch values (d x b, float32)."""
+        out = np.empty(self._keep[ticket][4], np.float32)
+        check(lib().cm_server_values(self.handle, ticket, out.ctypes.data))
+        return out
 
     def release(self, ticket):
         check(lib().cm_server_release(self.handle, ticket))

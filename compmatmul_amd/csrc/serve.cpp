@@ -640,6 +640,25 @@ int cm_server_decompress(cm_server* sv, uint64_t ticket, float* host_C) {
   return e == hipSuccess ? CM_OK : error(CM_EDEVICE, std::string("decompress: ") + hipGetErrorString(e));
 }
 
+int cm_server_values(cm_server* sv, uint64_t ticket, float* host_out) {
+  if (!sv || !host_out) return error(CM_EINVAL, "null argument");
+  std::lock_guard<std::recursive_mutex> lk(cm::rt::mutex());
+  const float* P = nullptr;
+  size_t bytes = 0;
+  {
+    std::lock_guard<std::mutex> lk2(sv->mu);
+    Request* q = find(sv, ticket);
+    if (!q) return error(CM_EINVAL, "unknown ticket");
+    if (q->state != kDone || !q->has_keep) return error(CM_EINVAL, "request not done or its sketch released");
+    P = q->keep.P;
+    bytes = size_t(q->r.d) * size_t(q->r.b) * sizeof(float);
+  }
+  hipStream_t s = cm::rt::stream();
+  hipError_t e = hipMemcpyAsync(host_out, P, bytes, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  return e == hipSuccess ? CM_OK : error(CM_EDEVICE, std::string("reading the sketch: ") + hipGetErrorString(e));
+}
+
 int cm_server_release(cm_server* sv, uint64_t ticket) {
   if (!sv) return error(CM_EINVAL, "null server");
   std::lock_guard<std::mutex> lk(sv->mu);

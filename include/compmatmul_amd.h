@@ -299,6 +299,10 @@ int cm_server_result(cm_server* server, uint64_t ticket, int64_t max_out, int32_
                      int32_t* out_j, float* out_val, int64_t* count, double* latency_s);
 /* Median-of-d estimate of all of C from the request's retained sketch. */
 int cm_server_decompress(cm_server* server, uint64_t ticket, float* host_C);
+/* The request's retained sketch values (d x b, FP32) copied to host memory.
+ * Replaces: reading matmul_sketch<Hash>::p (matmul_sketch.hpp:80) of the
+ * product the request describes. */
+int cm_server_values(cm_server* server, uint64_t ticket, float* host_out);
 /* Drop the request's retained sketch and its record. */
 int cm_server_release(cm_server* server, uint64_t ticket);
 typedef struct cm_server_stats {
