@@ -432,51 +432,79 @@ int compress_device(cm_sketch* sk, const void* AT, int64_t ld_at, const void* B,
   if (ws.carved() && (flags & (CM_FLAG_PLAIN | CM_FLAG_BF16_MFMA)))
     return fail(CM_ENOTSUP, "the plain and BF16 paths allocate their own buffers");
   if (flags & CM_FLAG_PLAIN) {
-    if (lb > max_lb2 + (fwht ? 1 : 0))
-      return fail(CM_ENOTSUP, "b too large for the plain path (64 KB of LDS per transform)");
-    int rc = ensure_buf(g.plainA, std::max<size_t>(1, size_t(n) * b * cplx * esz));
-    if (rc) return rc;
-    rc = ensure_buf(g.plainB, std::max<size_t>(1, size_t(n) * b * esz));
-    if (rc) return rc;
-    rc = ensure_buf(g.partial, size_t(d) * b * cplx * esz);
-    if (rc) return rc;
-    for (int t = 0; t < d; ++t) {
-      cm::PlainArgs pa{};
-      pa.fwht = fwht;
-      pa.f64 = f64;
-      pa.AT = AT;
-      pa.ld_at = ld_at;
-      pa.B = B;
-      pa.ld_b = ld_b;
-      pa.hA = sk->hA;
-      pa.hB = sk->hB;
-      pa.m = m;
-      pa.n = n;
-      pa.p = p;
-      pa.d = d;
-      pa.lb = lb;
-      pa.t = t;
-      pa.tw = tw;
-      pa.scale = psa.out;
-      pa.SA = g.plainA.p;
-      pa.SB = g.plainB.p;
-      pa.acc = static_cast<char*>(g.partial.p) + size_t(t) * b * cplx * esz;
-      if ((rc = timed("plain_sketch", [&] {
-             // the balance scales once, ahead of the first repetition's sketch
-             const hipError_t e = t == 0 ? with_scales() : hipSuccess;
-             return e != hipSuccess ? e : cm::launch_plain_sketch(pa, g.stream);
-           })))
-        return rc;
-      if ((rc = timed("plain_transform", [&] { return cm::launch_plain_transform(pa, g.stream); }))) return rc;
-      if ((rc = timed("plain_mac", [&] { return cm::launch_plain_mac(pa, g.stream); }))) return rc;
+    if (lb > cm::plain_max_lb(fwht, f64))
+      return fail(CM_ENOTSUP, "b too large for the plain path (2^" +
+                                  std::to_string(cm::plain_max_lb(fwht, f64)) + " at most)");
+    // Passes over (repetitions, inner indices): the sketches of one pass
+    // take at most 4 GiB; a pass holds whole repetitions when they fit, or a
+    // range of k of one repetition.
+    const size_t row_bytes = size_t(b) * 2 * esz;  // FFT: complex; FWHT: A and B rows
+    const int64_t rows_cap = std::max<int64_t>(1, int64_t((size_t(4) << 30) / row_bytes));
+    const int nk = n > 0 ? int(std::min<int64_t>(n, rows_cap)) : 0;
+    const int nt = nk == n && n > 0 ? int(std::max<int64_t>(1, std::min<int64_t>(d, rows_cap / n))) : 1;
+    // MAC chunks of about 32 rows of k: many short workgroups, so the last
+    // round of them is a small share of the stage
+    const int kchunks = std::max(1, (nk + 31) / 32);
+    const int lb2 = std::min(lb, max_lb2);
+    int rc;
+    if ((rc = ensure_buf(g.plainA, std::max<size_t>(1, size_t(nt) * nk * b * cplx * esz))) ||
+        (rc = ensure_buf(g.plainB, fwht ? std::max<size_t>(1, size_t(nt) * nk * b * esz) : 1)) ||
+        (rc = ensure_buf(g.entries, std::max<size_t>(1, size_t(kchunks) * nt * b * cplx * esz))) ||
+        (rc = ensure_buf(g.partial, size_t(d) * b * cplx * esz)) ||
+        (rc = ensure_buf(g.V, size_t(d) * b * cplx * esz)))
+      return rc;
+    if (n == 0) {  // no pass: the spectral sums are zero
+      const size_t words = size_t(d) * b * cplx * esz / 4;
+      for (size_t w0 = 0; w0 < words; w0 += size_t(1) << 30) {
+        const int cnt = int(std::min<size_t>(size_t(1) << 30, words - w0));
+        unsigned* at = static_cast<unsigned*>(g.partial.p) + w0;
+        if ((rc = timed("plain_mac", [&] { return cm::launch_zero_u32(at, cnt, g.stream); }))) return rc;
+      }
     }
-    cm::ReduceArgs ra{fwht, f64, int((flags & CM_FLAG_F64_REDUCE) != 0), g.partial.p, 1, d, lb, lb,
-                      tw, nullptr};
-    rc = ensure_buf(g.V, size_t(d) * b * cplx * esz);
-    if (rc) return rc;
-    ra.V = g.V.p;
+    for (int t0 = 0; n > 0 && t0 < d; t0 += nt) {
+      for (int k0 = 0; k0 < n; k0 += nk) {
+        cm::PlainArgs pa{};
+        pa.fwht = fwht;
+        pa.f64 = f64;
+        pa.AT = AT;
+        pa.ld_at = ld_at;
+        pa.B = B;
+        pa.ld_b = ld_b;
+        pa.hA = sk->hA;
+        pa.hB = sk->hB;
+        pa.m = m;
+        pa.n = n;
+        pa.p = p;
+        pa.d = d;
+        pa.lb = lb;
+        pa.t0 = t0;
+        pa.nt = std::min(nt, d - t0);
+        pa.k0 = k0;
+        pa.nk = std::min(nk, n - k0);
+        pa.tw = tw;
+        pa.scale = psa.out;
+        pa.SA = g.plainA.p;
+        pa.SB = g.plainB.p;
+        pa.kchunks = kchunks;
+        pa.part = g.entries.p;
+        pa.acc = g.partial.p;
+        pa.lb2 = lb2;
+        pa.first = k0 == 0;
+        const bool first_pass = t0 == 0 && k0 == 0;
+        if ((rc = timed("plain_sketch", [&] {
+               // the balance scales once, ahead of the first pass's sketch
+               const hipError_t e = first_pass ? with_scales() : hipSuccess;
+               return e != hipSuccess ? e : cm::launch_plain_sketch(pa, g.stream);
+             })))
+          return rc;
+        if ((rc = timed("plain_transform", [&] { return cm::launch_plain_transform(pa, g.stream); }))) return rc;
+        if ((rc = timed("plain_mac", [&] { return cm::launch_plain_mac(pa, g.stream); }))) return rc;
+      }
+    }
+    const int f64_reduce = (flags & CM_FLAG_F64_REDUCE) || n > 16384;
+    cm::ReduceArgs ra{fwht, f64, f64_reduce, g.partial.p, 1, d, lb, lb2, tw, g.V.p};
     if ((rc = timed("reduce", [&] { return cm::launch_reduce_inverse(ra, g.stream); }))) return rc;
-    cm::CombineArgs ca{fwht, f64, g.V.p, d, lb, lb, tw, sk->P, lb};
+    cm::CombineArgs ca{fwht, f64, g.V.p, d, lb, lb2, tw, sk->P, lb};
     return timed("combine", [&] { return cm::launch_combine(ca, g.stream); });
   }
 

@@ -231,19 +231,27 @@ hipError_t launch_decompress_batch(int fwht, int d, int staged, size_t lds,
                                    hipStream_t s);
 
 // ------------------------------------------------------------- plain path
-// Stage-by-stage pipeline that materialises per-k sketches and spectra.
+// Stage-by-stage pipeline that materialises per-k sketches and spectra in
+// HBM (plain.hip).  One pass covers repetitions [t0, t0 + nt) and inner
+// indices [k0, k0 + nk) (either nt = 1 or the whole of k): nt * nk rows of b
+// elements in SA (FFT: complex, pair-packed) and SB (FWHT: B's sketches).
 struct PlainArgs {
   int fwht, f64;
   const void* AT; int64_t ld_at;
   const void* B; int64_t ld_b;
   const uint32_t* hA; const uint32_t* hB;
   int m, n, p, d, lb;
-  int t;               // repetition handled by this call
-  const void* tw;
+  int t0, nt, k0, nk;
+  const void* tw;      // W_b^j, j < b, element type
   const void* scale;   // FFT: per k, (scale of row k of AT, of row k of B) (balance.hip)
-  void* SA; void* SB;  // n x b (x2 for complex) sketches / spectra
-  void* acc;           // b (x2 for complex) spectrum of repetition t
+  void* SA; void* SB;
+  int kchunks;         // chunks of k summed separately by the MAC
+  void* part;          // kchunks x nt x b partial sums (complex for FFT)
+  void* acc;           // d x b1 x b2 spectral sums in the reduce kernel's layout
+  int lb2;             // log2 of that layout's slice length
+  int first;           // this pass starts its repetitions' sums (else adds to them)
 };
+int plain_max_lb(int fwht, int f64);  // largest log2(b) of the plain path
 hipError_t launch_plain_sketch(const PlainArgs& a, hipStream_t s);
 hipError_t launch_plain_transform(const PlainArgs& a, hipStream_t s);
 hipError_t launch_plain_mac(const PlainArgs& a, hipStream_t s);

@@ -1,17 +1,30 @@
 // Unfused "plain" GPU pipeline and the FP64 GEMM.
 //
-// The plain path runs the reference's stages as separate launches per
-// repetition t (matmul_fft.hpp:70-107, matmul_fwht.hpp:64-95): count-sketch
-// every row k into HBM, transform every sketch in place, then reduce the
-// pointwise products over k.  It materialises n x b sketches and spectra in
-// HBM and is the per-product baseline the fused path is measured against.
-// It uses the same pair packing (z = S a + i S' b, accumulate z^2) and the
-// same reduce/combine kernels as the fused path, so the two differ only in
-// fusion.  It handles transforms that fit 64 KB of LDS: b <= 2^14 (FWHT, float),
-// 2^13 (FWHT double, FFT float), 2^12 (FFT double).
+// The plain path runs the reference's stages as separate passes over HBM
+// (matmul_fft.hpp:70-107, matmul_fwht.hpp:64-95): count-sketch every row k
+// into HBM, transform every sketch in place, then reduce the pointwise
+// products over k.  It materialises n x b sketches and spectra and is the
+// per-product baseline the fused path is measured against, so every stage
+// is written to stream HBM:
+//   sketch     one workgroup per (row, range of at most 16 KB of buckets):
+//              LDS scatter of the keys falling in the range, one coalesced
+//              write of the range;
+//   transform  in place per row, radix-16 (last pass narrower) register
+//              passes over an LDS copy of the row; a row larger than 128 KB
+//              runs as b = b1 * b2 in two passes, length-b1 columns (with
+//              the twiddles W_b^(n2 k1)) then length-b2 rows;
+//   MAC        tiles of (frequency range, chunk of k, repetition) summed in
+//              registers, then the chunks summed in a fixed order into the
+//              spectrum of each repetition (deterministic).
+// It uses the same pair packing and balance (z = 2^e S a + i 2^-e S' b,
+// accumulate z^2) and the same reduce/combine kernels as the fused path, so
+// the two differ only in fusion.  Any b up to the square of the 128 KB tile
+// length (2^28 for the FP32 FFT).
 //
 // cm::launch_gemm_f64 is the exact FP64 product behind compmatmul.gemm
 // (array.hpp:81-95: C = AT^T B, all row-major).
+#include <type_traits>
+
 #include "device_common.hpp"
 #include "kernels.hpp"
 
@@ -19,127 +32,535 @@ namespace cm {
 
 namespace {
 
-constexpr int NT = 256;
+constexpr int NT = 1024;       // threads of the transform kernels
+constexpr int ST = 256;        // threads of the sketch kernel
+constexpr int MT = 256;        // threads of the MAC kernels
 
 template <typename T, bool FWHT>
-__global__ void __launch_bounds__(NT) k_plain_sketch(PlainArgs a) {
+using Elem = std::conditional_t<FWHT, T, cpx<T>>;
+
+// log2 of the elements of one 128 KB transform tile (one workgroup per CU)
+template <typename T, bool FWHT>
+constexpr int lmax_log() {
+  return sizeof(Elem<T, FWHT>) == 4 ? 15 : sizeof(Elem<T, FWHT>) == 8 ? 14 : 13;
+}
+
+// log2 of the buckets of one 16 KB sketch range
+template <typename T, bool FWHT>
+constexpr int range_log() {
+  return sizeof(Elem<T, FWHT>) == 4 ? 12 : sizeof(Elem<T, FWHT>) == 8 ? 11 : 10;
+}
+
+// one padding slot every 16 elements keeps strided passes off one bank
+__device__ __forceinline__ int pad(int i) { return i + (i >> 4); }
+
+
+
+// cos(2 pi e / 16)
+__device__ __forceinline__ constexpr double cos16(int e) {
+  constexpr double c[16] = {1.0, 0.92387953251128674, 0.70710678118654752, 0.38268343236508978,
+                            0.0, -0.38268343236508978, -0.70710678118654752, -0.92387953251128674,
+                            -1.0, -0.92387953251128674, -0.70710678118654752, -0.38268343236508978,
+                            0.0, 0.38268343236508978, 0.70710678118654752, 0.92387953251128674};
+  return c[e & 15];
+}
+
+template <int R>
+constexpr int ilog2c() { return R <= 1 ? 0 : 1 + ilog2c<R / 2>(); }
+
+// R-point DFT (W = exp(-2 pi i / R)), natural order in and out: radix-2
+// decimation in frequency in registers, then the bit-reversal undone
+// reverse the low r bits of k (compile-time)
+__device__ __forceinline__ constexpr int brev_c(int k, int r) {
+  int br = 0;
+  for (int bit = 0; bit < r; ++bit) br |= ((k >> bit) & 1) << (r - 1 - bit);
+  return br;
+}
+
+// Every loop below has a constant trip count, so it unrolls completely and
+// the arrays stay in registers.
+template <int R, typename T>
+__device__ __forceinline__ void dft_reg(cpx<T> (&v)[R]) {
+  constexpr int r = ilog2c<R>();
+#pragma unroll
+  for (int st = 0; st < r; ++st) {
+    const int len = R >> (st + 1);
+#pragma unroll
+    for (int q = 0; q < R / 2; ++q) {  // butterfly q of the stage
+      const int pos = q % len, j = (q / len) * 2 * len + pos;
+      const cpx<T> x = v[j], y = v[j + len];
+      v[j] = cadd(x, y);
+      const cpx<T> dd = csub(x, y);
+      const int e = pos * 8 / len;  // W_R^(pos R / 2len) = W_16^e
+      if (e == 0) {
+        v[j + len] = dd;
+      } else if (e == 4) {  // -i
+        v[j + len] = cpx<T>{dd.y, -dd.x};
+      } else {
+        const T c = T(cos16(e)), sn = T(cos16(e + 12));  // sin = cos(angle - pi/2)
+        v[j + len] = cpx<T>{dd.x * c + dd.y * sn, dd.y * c - dd.x * sn};
+      }
+    }
+  }
+  cpx<T> o[R];
+#pragma unroll
+  for (int k = 0; k < R; ++k) o[k] = v[brev_c(k, r)];
+#pragma unroll
+  for (int k = 0; k < R; ++k) v[k] = o[k];
+}
+
+// R-point natural-order Walsh-Hadamard transform in registers
+template <int R, typename T>
+__device__ __forceinline__ void wht_reg(T (&v)[R]) {
+  constexpr int r = ilog2c<R>();
+#pragma unroll
+  for (int st = 0; st < r; ++st) {
+    const int len = R >> (st + 1);
+#pragma unroll
+    for (int q = 0; q < R / 2; ++q) {
+      const int j = (q / len) * 2 * len + q % len;
+      const T x = v[j], y = v[j + len];
+      v[j] = x + y;
+      v[j + len] = x - y;
+    }
+  }
+}
+
+// Twiddles of a length-2^l transform in LDS: W_L^x = hi[x >> 6] lo[x & 63]
+// (lo[i] = W_L^i, hi[i] = W_L^(64 i)), filled from the length-2^lb table.
+constexpr int TW_LO = 64;
+template <typename T>
+__device__ __forceinline__ void fill_twiddles(cpx<T>* lo, cpx<T>* hi, int l, const cpx<T>* __restrict__ tw,
+                                              int lb) {
+  const int L = 1 << l;
+  for (int i = threadIdx.x; i < TW_LO + (L >> 6) + 1; i += NT) {
+    if (i < TW_LO)
+      lo[i] = i < L ? tw[int64_t(i) << (lb - l)] : cpx<T>{T(1), T(0)};
+    else
+      hi[i - TW_LO] = tw[(int64_t(i - TW_LO) << 6 << (lb - l)) & ((int64_t(1) << lb) - 1)];
+  }
+}
+template <typename T>
+__device__ __forceinline__ cpx<T> twiddle(const cpx<T>* lo, const cpx<T>* hi, int x) {
+  return x < TW_LO ? lo[x] : cmul(hi[x >> 6], lo[x & (TW_LO - 1)]);
+}
+
+// One radix-R decimation-in-frequency pass at stride S = 2^s over `count`
+// sequences of length 2^l held in LDS (sequence c from element c * cs): the
+// R elements base + j S of each group are transformed, and for the FFT
+// output k is multiplied by W_(R S)^(pos k) = W_L^(pos k L / (R S)), from the
+// LDS twiddle tables (lo, hi).
+template <typename T, bool FWHT, int R>
+__device__ __forceinline__ void lds_pass(Elem<T, FWHT>* z, int l, int s, int count, int cs,
+                                         const cpx<T>* lo, const cpx<T>* hi) {
+  constexpr int r = ilog2c<R>();
+  const int S = 1 << s;
+  const int gl = l - r;  // groups per sequence, log2
+  const int total = count << gl;
+  for (int g = threadIdx.x; g < total; g += NT) {
+    const int c = g >> gl;
+    const int gg = g & ((1 << gl) - 1);
+    const int pos = gg & (S - 1);
+    const int base = c * cs + ((gg >> s) << (s + r)) + pos;
+    Elem<T, FWHT> v[R];
+#pragma unroll
+    for (int j = 0; j < R; ++j) v[j] = z[pad(base + j * S)];
+    if constexpr (FWHT) {
+      wht_reg<R>(v);
+    } else {
+      dft_reg<R>(v);
+      if (pos != 0) {
+        const int sh = l - s - r;
+#pragma unroll
+        for (int k = 1; k < R; ++k) v[k] = cmul(v[k], twiddle(lo, hi, (pos * k) << sh));
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < R; ++j) z[pad(base + j * S)] = v[j];
+  }
+}
+
+// Transform `count` LDS sequences of length 2^l: radix-16 passes from the
+// high bits down, the last pass over the l mod 4 lowest bits (if any).  FWHT
+// output is in natural order; FFT output k sits at dif_pos(k, l, rmax()).
+// widest pass, log2: radix 16, or 8 for the FP64 FFT (whose radix-16 pass
+// would spill at 1024 threads)
+template <typename T, bool FWHT>
+constexpr int rmax() { return !FWHT && sizeof(T) == 8 ? 3 : 4; }
+
+template <typename T, bool FWHT>
+__device__ __forceinline__ void lds_transform(Elem<T, FWHT>* z, int l, int count, int cs, const cpx<T>* lo,
+                              const cpx<T>* hi) {
+  int s = l;
+  while (s > 0) {
+    const int r = min(rmax<T, FWHT>(), s);
+    s -= r;
+    __syncthreads();
+    switch (r) {
+      case 4:
+        if constexpr (rmax<T, FWHT>() >= 4) lds_pass<T, FWHT, 16>(z, l, s, count, cs, lo, hi);
+        break;
+      case 3: lds_pass<T, FWHT, 8>(z, l, s, count, cs, lo, hi); break;
+      case 2: lds_pass<T, FWHT, 4>(z, l, s, count, cs, lo, hi); break;
+      default: lds_pass<T, FWHT, 2>(z, l, s, count, cs, lo, hi); break;
+    }
+  }
+  __syncthreads();
+}
+
+// LDS of a tile of 2^l elements (padded), rounded to 16 bytes, then (FFT)
+// the twiddle tables lo (64 entries) and hi (2^l / 64 + 1)
+template <typename T, bool FWHT>
+__host__ __device__ __forceinline__ size_t tile_bytes(int l) {
+  return ((size_t((1 << l) + (1 << l) / 16) * sizeof(Elem<T, FWHT>)) + 15) & ~size_t(15);
+}
+template <typename T, bool FWHT>
+__host__ __device__ __forceinline__ size_t transform_bytes(int l) {
+  return tile_bytes<T, FWHT>(l) + (FWHT ? 0 : (64 + (1 << l) / 64 + 1) * sizeof(cpx<T>));
+}
+template <typename T, bool FWHT>
+__device__ __forceinline__ cpx<T>* twiddle_lds(unsigned char* smem, int l) {
+  return reinterpret_cast<cpx<T>*>(smem + tile_bytes<T, FWHT>(l));
+}
+
+// 16-byte vectors of elements for the row loads and stores
+template <typename E>
+struct alignas(16) Vec16 {
+  E e[16 / sizeof(E)];
+};
+
+// Position of DFT output k after lds_transform with passes of rm bits: the
+// radix digits of k (rm bits each from the low end, the last one l mod rm
+// bits) in reverse order.
+__device__ __forceinline__ int dif_pos(int k, int l, int rm) {
+  int p = 0, off = l;
+  while (off > 0) {
+    const int r = min(rm, off);
+    off -= r;
+    p |= (k & ((1 << r) - 1)) << off;
+    k >>= r;
+  }
+  return p;
+}
+
+template <typename T, bool FWHT>
+__device__ __forceinline__ Elem<T, FWHT>* row_base(const PlainArgs& a, bool side_b, int64_t row) {
+  return static_cast<Elem<T, FWHT>*>(side_b ? a.SB : a.SA) + (row << a.lb);
+}
+
+// Sketch of row (t, k) for the buckets [q R, (q + 1) R) of one workgroup:
+// LDS scatter, then one coalesced write.  FWHT: blockIdx.y selects A (0) or
+// B (1); FFT: the balanced pair z = 2^e S a + i 2^-e S' b.
+template <typename T, bool FWHT>
+__global__ void __launch_bounds__(ST) k_plain_sketch(PlainArgs a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  const int b = 1 << a.lb;
-  const int k = blockIdx.x;
-  const int tid = threadIdx.x;
+  using E = Elem<T, FWHT>;
+  E* z = reinterpret_cast<E*>(smem_raw);
+  const int lr = min(a.lb, range_log<T, FWHT>());
+  const int R = 1 << lr;
+  const int q = blockIdx.x & ((1 << (a.lb - lr)) - 1);
+  const int row = blockIdx.x >> (a.lb - lr);
+  const int t = a.t0 + row / a.nk, k = a.k0 + row % a.nk;
+  const bool side_b = FWHT && blockIdx.y != 0;
+  constexpr int V = 16 / int(sizeof(E));
+  using Vc = Vec16<E>;
+  if (R >= V) {
+    for (int i = threadIdx.x; i < R / V; i += ST) reinterpret_cast<Vc*>(z)[i] = Vc{};
+  } else {
+    for (int e = threadIdx.x; e < R; e += ST) z[e] = E{};
+  }
+  __syncthreads();
   const T* __restrict__ ar = static_cast<const T*>(a.AT) + int64_t(k) * a.ld_at;
   const T* __restrict__ br = static_cast<const T*>(a.B) + int64_t(k) * a.ld_b;
-  const uint32_t* __restrict__ hA = a.hA + int64_t(a.t) * a.m;
-  const uint32_t* __restrict__ hB = a.hB + int64_t(a.t) * a.p;
+  const uint32_t* __restrict__ hA = a.hA + int64_t(t) * a.m;
+  const uint32_t* __restrict__ hB = a.hB + int64_t(t) * a.p;
+  const uint32_t qq = uint32_t(q);
   if constexpr (FWHT) {
-    // blockIdx.y selects the A side (0) or the B side (1)
-    T* s = reinterpret_cast<T*>(smem_raw);
-    const bool side_b = blockIdx.y != 0;
     const int keys = side_b ? a.p : a.m;
-    const T* __restrict__ row = side_b ? br : ar;
+    const T* __restrict__ v = side_b ? br : ar;
     const uint32_t* __restrict__ hh = side_b ? hB : hA;
-    for (int e = tid; e < b; e += NT) s[e] = T(0);
-    __syncthreads();
-    for (int i = tid; i < keys; i += NT) atomicAdd(&s[hbucket(hh[i])], apply_sign(row[i], hneg(hh[i])));
-    __syncthreads();
-    T* S = static_cast<T*>(side_b ? a.SB : a.SA) + int64_t(k) * b;
-    for (int e = tid; e < b; e += NT) S[e] = s[e];
+    for (int i = threadIdx.x; i < keys; i += ST) {
+      const uint32_t e = hh[i], h = hbucket(e);
+      if ((h >> lr) == qq) atomicAdd(&z[h & (R - 1)], apply_sign(v[i], hneg(e)));
+    }
   } else {
-    cpx<T>* z = reinterpret_cast<cpx<T>*>(smem_raw);
-    const cpx<T> bal = static_cast<const cpx<T>*>(a.scale)[k];  // balance.hip
-    for (int e = tid; e < b; e += NT) z[e] = {T(0), T(0)};
-    __syncthreads();
-    for (int i = tid; i < a.m; i += NT)
-      atomicAdd(&z[hbucket(hA[i])].x, apply_sign(ar[i] * bal.x, hneg(hA[i])));
-    for (int j = tid; j < a.p; j += NT)
-      atomicAdd(&z[hbucket(hB[j])].y, apply_sign(br[j] * bal.y, hneg(hB[j])));
-    __syncthreads();
-    cpx<T>* Z = static_cast<cpx<T>*>(a.SA) + int64_t(k) * b;
-    for (int e = tid; e < b; e += NT) Z[e] = z[e];
+    const cpx<T> bal = static_cast<const cpx<T>*>(a.scale)[k];
+    for (int i = threadIdx.x; i < a.m; i += ST) {
+      const uint32_t e = hA[i], h = hbucket(e);
+      if ((h >> lr) == qq) atomicAdd(&z[h & (R - 1)].x, apply_sign(ar[i] * bal.x, hneg(e)));
+    }
+    for (int j = threadIdx.x; j < a.p; j += ST) {
+      const uint32_t e = hB[j], h = hbucket(e);
+      if ((h >> lr) == qq) atomicAdd(&z[h & (R - 1)].y, apply_sign(br[j] * bal.y, hneg(e)));
+    }
+  }
+  __syncthreads();
+  E* out = row_base<T, FWHT>(a, side_b, row) + (int64_t(q) << lr);
+  if (R >= V) {
+    for (int i = threadIdx.x; i < R / V; i += ST) reinterpret_cast<Vc*>(out)[i] = reinterpret_cast<const Vc*>(z)[i];
+  } else {
+    for (int e = threadIdx.x; e < R; e += ST) out[e] = z[e];
   }
 }
 
+// Transform segments of length 2^l in place: segment `seg` of row `row` is
+// elements [seg 2^l, (seg + 1) 2^l).  A whole row when b fits a tile; the
+// second (row) pass of a two-pass transform otherwise.  Written back in
+// natural order within the segment.
 template <typename T, bool FWHT>
-__global__ void __launch_bounds__(NT) k_plain_transform(PlainArgs a) {
+__global__ void __launch_bounds__(NT) k_plain_rows(PlainArgs a, int l) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  const int b = 1 << a.lb;
-  const int k = blockIdx.x;
-  const int tid = threadIdx.x;
-  if constexpr (FWHT) {
-    T* s = reinterpret_cast<T*>(smem_raw);
-    T* V = static_cast<T*>(blockIdx.y ? a.SB : a.SA) + int64_t(k) * b;
-    for (int e = tid; e < b; e += NT) s[e] = V[e];
-    __syncthreads();
-    for (int len = 1; len < b; len <<= 1) {
-      for (int q = tid; q < (b >> 1); q += NT) {
-        const int j = ((q & ~(len - 1)) << 1) | (q & (len - 1));
-        const T x0 = s[j], x1 = s[j + len];
-        s[j] = x0 + x1;
-        s[j + len] = x0 - x1;
-      }
-      __syncthreads();
+  using E = Elem<T, FWHT>;
+  E* z = reinterpret_cast<E*>(smem_raw);
+  const int segs_log = a.lb - l;
+  const int64_t row = blockIdx.x >> segs_log;
+  const int seg = blockIdx.x & ((1 << segs_log) - 1);
+  E* g = row_base<T, FWHT>(a, FWHT && blockIdx.y != 0, row) + (int64_t(seg) << l);
+  const int L = 1 << l;
+  cpx<T>* lo = twiddle_lds<T, FWHT>(smem_raw, l);
+  cpx<T>* hi = lo + TW_LO;
+  if (!FWHT) fill_twiddles<T>(lo, hi, l, static_cast<const cpx<T>*>(a.tw), a.lb);
+  constexpr int V = 16 / int(sizeof(E));
+  constexpr int U = 4;  // vector loads in flight per thread
+  if (L >= V) {  // 16-byte vectors; the V elements of one share a padded run
+    const int nv = L / V;
+    const Vec16<E>* gv = reinterpret_cast<const Vec16<E>*>(g);
+    for (int i0 = threadIdx.x; i0 < nv; i0 += NT * U) {
+      Vec16<E> x[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (i0 + u * NT < nv) x[u] = gv[i0 + u * NT];
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (i0 + u * NT < nv)
+#pragma unroll
+          for (int v = 0; v < V; ++v) z[pad((i0 + u * NT) * V + v)] = x[u].e[v];
     }
-    for (int e = tid; e < b; e += NT) V[e] = s[e];
   } else {
-    cpx<T>* z = reinterpret_cast<cpx<T>*>(smem_raw);
-    const cpx<T>* __restrict__ tw = static_cast<const cpx<T>*>(a.tw);
-    cpx<T>* Z = static_cast<cpx<T>*>(a.SA) + int64_t(k) * b;
-    for (int e = tid; e < b; e += NT) z[e] = Z[e];
-    __syncthreads();
-    for (int len = b >> 1; len >= 1; len >>= 1) {
-      const int lstride = a.lb - __ffs(len);
-      for (int q = tid; q < (b >> 1); q += NT) {
-        const int pos = q & (len - 1);
-        const int j = ((q & ~(len - 1)) << 1) | pos;
-        const cpx<T> x0 = z[j], x1 = z[j + len];
-        z[j] = cadd(x0, x1);
-        z[j + len] = cmul(csub(x0, x1), tw[int64_t(pos) << lstride]);
-      }
-      __syncthreads();
+    for (int e = threadIdx.x; e < L; e += NT) z[pad(e)] = g[e];
+  }
+  lds_transform<T, FWHT>(z, l, 1, 0, lo, hi);
+  if (L >= V) {
+    const int nv = L / V;
+    Vec16<E>* gv = reinterpret_cast<Vec16<E>*>(g);
+    for (int i = threadIdx.x; i < nv; i += NT) {
+      Vec16<E> x;
+#pragma unroll
+      for (int v = 0; v < V; ++v) x.e[v] = z[pad(FWHT ? i * V + v : dif_pos(i * V + v, l, rmax<T, FWHT>()))];
+      gv[i] = x;
     }
-    for (int e = tid; e < b; e += NT) Z[e] = z[e];  // bit-reversed frequency order
+  } else {
+    for (int e = threadIdx.x; e < L; e += NT) g[e] = z[pad(FWHT ? e : dif_pos(e, l, rmax<T, FWHT>()))];
   }
 }
 
-// acc[f] = sum_k A_k[f] B_k[f] (FWHT) or sum_k Z_k[f]^2 (FFT, stored at the
-// natural frequency index); acc uses the [t][slice=0][b] partial layout.
+// First pass of a two-pass transform, b = b1 b2 (b1 = 2^l1): length-b1
+// transforms over n1 of the columns n2 in [c0, c0 + 2^lc) of one row,
+// element n1 b2 + n2; the FFT output k1 is multiplied by W_b^(n2 k1) and
+// stored at k1 b2 + n2.  The tile is column-major in LDS.
 template <typename T, bool FWHT>
-__global__ void __launch_bounds__(NT) k_plain_mac(PlainArgs a) {
-  const int b = 1 << a.lb;
-  const int f = blockIdx.x * NT + threadIdx.x;
-  if (f >= b) return;
-  if constexpr (FWHT) {
-    const T* SA = static_cast<const T*>(a.SA);
-    const T* SB = static_cast<const T*>(a.SB);
-    T s = T(0);
-    for (int k = 0; k < a.n; ++k) s += SA[int64_t(k) * b + f] * SB[int64_t(k) * b + f];
-    static_cast<T*>(a.acc)[f] = s;
+__global__ void __launch_bounds__(NT) k_plain_cols(PlainArgs a, int l1, int lc) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  using E = Elem<T, FWHT>;
+  E* z = reinterpret_cast<E*>(smem_raw);
+  const int l2 = a.lb - l1;
+  const int tiles_log = l2 - lc;
+  const int64_t row = blockIdx.x >> tiles_log;
+  const int c0 = (blockIdx.x & ((1 << tiles_log) - 1)) << lc;
+  E* g = row_base<T, FWHT>(a, FWHT && blockIdx.y != 0, row);
+  const int n = 1 << (l1 + lc), C = 1 << lc;
+  constexpr int V = 16 / int(sizeof(E));
+  constexpr int U = 4;  // vector loads in flight per thread
+  if (C >= V) {  // 16-byte vectors of V adjacent columns
+    const int nv = n / V;
+    for (int i0 = threadIdx.x; i0 < nv; i0 += NT * U) {
+      Vec16<E> x[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int e = (i0 + u * NT) * V;
+        if (i0 + u * NT < nv)
+          x[u] = *reinterpret_cast<const Vec16<E>*>(g + (int64_t(e >> lc) << l2) + c0 + (e & (C - 1)));
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int e = (i0 + u * NT) * V;
+        if (i0 + u * NT < nv)
+#pragma unroll
+          for (int v = 0; v < V; ++v) z[pad((((e & (C - 1)) + v) << l1) + (e >> lc))] = x[u].e[v];
+      }
+    }
   } else {
-    const cpx<T>* Z = static_cast<const cpx<T>*>(a.SA);
-    cpx<T> s = {T(0), T(0)};
-    for (int k = 0; k < a.n; ++k) s = cadd(s, csqr(Z[int64_t(k) * b + f]));
-    static_cast<cpx<T>*>(a.acc)[bitrev(f, a.lb)] = s;
+    for (int e = threadIdx.x; e < n; e += NT) {
+      const int n1 = e >> lc, cc = e & (C - 1);
+      z[pad((cc << l1) + n1)] = g[(int64_t(n1) << l2) + c0 + cc];
+    }
+  }
+  const cpx<T>* tw = static_cast<const cpx<T>*>(a.tw);
+  cpx<T>* lo = twiddle_lds<T, FWHT>(smem_raw, l1 + lc);
+  cpx<T>* hi = lo + TW_LO;
+  if (!FWHT) fill_twiddles<T>(lo, hi, l1, tw, a.lb);
+  lds_transform<T, FWHT>(z, l1, C, 1 << l1, lo, hi);
+  for (int e = threadIdx.x; e < n; e += NT) {
+    const int k1 = e >> lc, cc = e & (C - 1);
+    if constexpr (FWHT) {
+      g[(int64_t(k1) << l2) + c0 + cc] = z[pad((cc << l1) + k1)];
+    } else {
+      const cpx<T> v = z[pad((cc << l1) + dif_pos(k1, l1, rmax<T, FWHT>()))];
+      g[(int64_t(k1) << l2) + c0 + cc] = cmul(v, tw[int64_t(c0 + cc) * k1]);
+    }
   }
 }
 
+// elements per MAC thread: one 16-byte load per operand row
 template <typename T, bool FWHT>
-hipError_t plain_sketch_t(const PlainArgs& a, hipStream_t s) {
-  const size_t lds = (FWHT ? 1 : 2) * sizeof(T) * (size_t(1) << a.lb);
-  k_plain_sketch<T, FWHT><<<dim3(unsigned(a.n), FWHT ? 2 : 1), dim3(NT), lds, s>>>(a);
+constexpr int mac_v() { return 16 / int(sizeof(Elem<T, FWHT>)); }
+
+// Partial spectral sums over one chunk of k: blockIdx.x a range of MT * V
+// stored positions, blockIdx.y the chunk, blockIdx.z the repetition within
+// the pass; part[(chunk * nt + tz) b + P].
+template <typename T, bool FWHT>
+__global__ void __launch_bounds__(MT) k_plain_mac(PlainArgs a, int kper) {
+  using E = Elem<T, FWHT>;
+  constexpr int V = mac_v<T, FWHT>();
+  const int64_t b = int64_t(1) << a.lb;
+  const int64_t P0 = (int64_t(blockIdx.x) * MT + threadIdx.x) * V;
+  if (P0 >= b) return;
+  const int tz = blockIdx.z;
+  const int k0 = blockIdx.y * kper;
+  const int k1 = min(a.nk, k0 + kper);
+  const int nv = int(min<int64_t>(V, b - P0));
+  E s[V];
+#pragma unroll
+  for (int u = 0; u < V; ++u) s[u] = E{};
+  auto mac = [&](const E& x, const E& y, E& acc) {
+    if constexpr (FWHT)
+      acc += x * y;
+    else
+      acc = cadd(acc, csqr(x));
+  };
+  if (nv == V) {  // whole 16-byte vectors, the loads of KU rows in flight
+    using Vc = Vec16<E>;
+    constexpr int KU = 4;
+    int k = k0;
+    for (; k + KU <= k1; k += KU) {
+      Vc x[KU], y[KU];
+#pragma unroll
+      for (int u = 0; u < KU; ++u) {
+        const int64_t row = int64_t(tz) * a.nk + k + u;
+        x[u] = *reinterpret_cast<const Vc*>(row_base<T, FWHT>(a, false, row) + P0);
+        if constexpr (FWHT) y[u] = *reinterpret_cast<const Vc*>(row_base<T, FWHT>(a, true, row) + P0);
+      }
+#pragma unroll
+      for (int u = 0; u < KU; ++u)
+#pragma unroll
+        for (int v = 0; v < V; ++v) mac(x[u].e[v], FWHT ? y[u].e[v] : x[u].e[v], s[v]);
+    }
+    for (; k < k1; ++k) {
+      const int64_t row = int64_t(tz) * a.nk + k;
+      const Vc x = *reinterpret_cast<const Vc*>(row_base<T, FWHT>(a, false, row) + P0);
+      Vc y = x;
+      if constexpr (FWHT) y = *reinterpret_cast<const Vc*>(row_base<T, FWHT>(a, true, row) + P0);
+#pragma unroll
+      for (int v = 0; v < V; ++v) mac(x.e[v], y.e[v], s[v]);
+    }
+  } else {  // b below one vector
+    for (int k = k0; k < k1; ++k) {
+      const int64_t row = int64_t(tz) * a.nk + k;
+      const E* x = row_base<T, FWHT>(a, false, row) + P0;
+      const E* y = FWHT ? row_base<T, FWHT>(a, true, row) + P0 : x;
+#pragma unroll
+      for (int u = 0; u < V; ++u)
+        if (u < nv) mac(x[u], y[u], s[u]);
+    }
+  }
+  E* out = static_cast<E*>(a.part) + (int64_t(blockIdx.y) * a.nt + tz) * b + P0;
+#pragma unroll
+  for (int u = 0; u < V; ++u)
+    if (u < nv) out[u] = s[u];
+}
+
+// Chunks of one pass summed in order into the spectrum of each repetition,
+// at its slot of the sliced layout the reduce kernel reads ([t][s][f] with
+// X[s + b1 f] for the FFT, natural for the FWHT); added to what earlier
+// passes over the same repetition left there unless a.first.  l1 > 0: the
+// FFT rows came from the two-pass transform, so stored position P holds
+// X[(P >> l2) + ((P mod b2) << l1)].
+template <typename T, bool FWHT>
+__global__ void __launch_bounds__(MT) k_plain_mac_sum(PlainArgs a, int kchunks, int l1) {
+  using E = Elem<T, FWHT>;
+  const int64_t b = int64_t(1) << a.lb;
+  const int64_t P = int64_t(blockIdx.x) * MT + threadIdx.x;
+  if (P >= b) return;
+  const int tz = blockIdx.y;
+  int64_t F = P;
+  if (!FWHT && l1 > 0) {
+    const int l2 = a.lb - l1;
+    F = (P >> l2) + ((P & ((int64_t(1) << l2) - 1)) << l1);
+  }
+  int64_t slot = F;
+  if (!FWHT) {
+    const int lbs = a.lb - a.lb2;
+    slot = ((F & ((int64_t(1) << lbs) - 1)) << a.lb2) + (F >> lbs);
+  }
+  E* acc = static_cast<E*>(a.acc) + int64_t(a.t0 + tz) * b + slot;
+  const E* part = static_cast<const E*>(a.part) + int64_t(tz) * b + P;
+  E s = a.first ? E{} : *acc;
+  for (int c = 0; c < kchunks; ++c) {
+    if constexpr (FWHT)
+      s += part[int64_t(c) * a.nt * b];
+    else
+      s = cadd(s, part[int64_t(c) * a.nt * b]);
+  }
+  *acc = s;
+}
+
+template <typename K>
+bool lds_ok(K kernel, size_t bytes) {
+  return bytes <= 65536 ||
+         hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
+                             int(bytes)) == hipSuccess;
+}
+
+template <typename T, bool FWHT>
+hipError_t sketch_t(const PlainArgs& a, hipStream_t s) {
+  const int lr = std::min(a.lb, range_log<T, FWHT>());
+  const size_t lds = sizeof(Elem<T, FWHT>) << lr;
+  const int64_t blocks = (int64_t(a.nt) * a.nk) << (a.lb - lr);
+  k_plain_sketch<T, FWHT><<<dim3(unsigned(blocks), FWHT ? 2 : 1), dim3(ST), lds, s>>>(a);
   return hipGetLastError();
 }
 
 template <typename T, bool FWHT>
-hipError_t plain_transform_t(const PlainArgs& a, hipStream_t s) {
-  const size_t lds = (FWHT ? 1 : 2) * sizeof(T) * (size_t(1) << a.lb);
-  k_plain_transform<T, FWHT><<<dim3(unsigned(a.n), FWHT ? 2 : 1), dim3(NT), lds, s>>>(a);
+hipError_t transform_t(const PlainArgs& a, hipStream_t s) {
+  constexpr int LM = lmax_log<T, FWHT>();
+  // LDS sized to the segment (so rows of up to 64 KB keep two workgroups a CU)
+  if (!lds_ok(&k_plain_rows<T, FWHT>, transform_bytes<T, FWHT>(LM)) ||
+      !lds_ok(&k_plain_cols<T, FWHT>, transform_bytes<T, FWHT>(LM)))
+    return hipErrorInvalidValue;
+  const int64_t rows = int64_t(a.nt) * a.nk;
+  const int l2 = std::min(a.lb, LM);
+  const int l1 = a.lb - l2;
+  if (l1 > 0) {
+    const int lc = LM - l1;
+    k_plain_cols<T, FWHT><<<dim3(unsigned(rows << (l2 - lc)), FWHT ? 2 : 1), dim3(NT),
+                            transform_bytes<T, FWHT>(LM), s>>>(a, l1, lc);
+  }
+  k_plain_rows<T, FWHT><<<dim3(unsigned(rows << l1), FWHT ? 2 : 1), dim3(NT), transform_bytes<T, FWHT>(l2),
+                          s>>>(a, l2);
   return hipGetLastError();
 }
 
 template <typename T, bool FWHT>
-hipError_t plain_mac_t(const PlainArgs& a, hipStream_t s) {
-  const int b = 1 << a.lb;
-  k_plain_mac<T, FWHT><<<dim3(unsigned((b + NT - 1) / NT)), dim3(NT), 0, s>>>(a);
+hipError_t mac_t(const PlainArgs& a, hipStream_t s) {
+  constexpr int LM = lmax_log<T, FWHT>();
+  constexpr int V = mac_v<T, FWHT>();
+  const int64_t b = int64_t(1) << a.lb;
+  const int kper = (a.nk + a.kchunks - 1) / a.kchunks;
+  const int kchunks = (a.nk + kper - 1) / kper;
+  const dim3 grid(unsigned((b + MT * V - 1) / (MT * V)), unsigned(kchunks), unsigned(a.nt));
+  k_plain_mac<T, FWHT><<<grid, dim3(MT), 0, s>>>(a, kper);
+  const int l1 = a.lb > LM ? a.lb - LM : 0;
+  k_plain_mac_sum<T, FWHT><<<dim3(unsigned((b + MT - 1) / MT), unsigned(a.nt)), dim3(MT), 0, s>>>(a, kchunks, l1);
   return hipGetLastError();
 }
 
@@ -164,23 +585,31 @@ __global__ void __launch_bounds__(256) k_gemm_f64(int m, int n, int p,
   if (i < m && j < p) C[int64_t(i) * p + j] = acc;
 }
 
+
 }  // namespace
 
+int plain_max_lb(int fwht, int f64) {
+  const int lm = fwht ? (f64 ? lmax_log<double, true>() : lmax_log<float, true>())
+                      : (f64 ? lmax_log<double, false>() : lmax_log<float, false>());
+  return 2 * lm;
+}
+
 hipError_t launch_plain_sketch(const PlainArgs& a, hipStream_t s) {
-  if (a.n == 0) return hipSuccess;
-  if (a.f64) return a.fwht ? plain_sketch_t<double, true>(a, s) : plain_sketch_t<double, false>(a, s);
-  return a.fwht ? plain_sketch_t<float, true>(a, s) : plain_sketch_t<float, false>(a, s);
+  if (a.nk == 0 || a.nt == 0) return hipSuccess;
+  if (a.f64) return a.fwht ? sketch_t<double, true>(a, s) : sketch_t<double, false>(a, s);
+  return a.fwht ? sketch_t<float, true>(a, s) : sketch_t<float, false>(a, s);
 }
 
 hipError_t launch_plain_transform(const PlainArgs& a, hipStream_t s) {
-  if (a.n == 0) return hipSuccess;
-  if (a.f64) return a.fwht ? plain_transform_t<double, true>(a, s) : plain_transform_t<double, false>(a, s);
-  return a.fwht ? plain_transform_t<float, true>(a, s) : plain_transform_t<float, false>(a, s);
+  if (a.nk == 0 || a.nt == 0) return hipSuccess;
+  if (a.f64) return a.fwht ? transform_t<double, true>(a, s) : transform_t<double, false>(a, s);
+  return a.fwht ? transform_t<float, true>(a, s) : transform_t<float, false>(a, s);
 }
 
 hipError_t launch_plain_mac(const PlainArgs& a, hipStream_t s) {
-  if (a.f64) return a.fwht ? plain_mac_t<double, true>(a, s) : plain_mac_t<double, false>(a, s);
-  return a.fwht ? plain_mac_t<float, true>(a, s) : plain_mac_t<float, false>(a, s);
+  if (a.nk == 0 || a.nt == 0) return hipSuccess;
+  if (a.f64) return a.fwht ? mac_t<double, true>(a, s) : mac_t<double, false>(a, s);
+  return a.fwht ? mac_t<float, true>(a, s) : mac_t<float, false>(a, s);
 }
 
 hipError_t launch_gemm_f64(int m, int n, int p, const double* AT, const double* B, double* C,

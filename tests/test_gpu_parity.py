@@ -336,3 +336,39 @@ def test_decompress_host_ldc_keeps_padding(dtype):
     assert np.array_equal(wide[:, :p], dense)
     assert np.all(wide[:, p:] == dtype(-7.25))
     sk.free()
+
+
+# ---------------------------------------------------------------- plain path
+@pytest.mark.parametrize("variant", ["fft", "fwht"])
+@pytest.mark.parametrize("dtype,b", [(np.float32, 2), (np.float32, 64), (np.float32, 4096),
+                                     (np.float32, 1 << 14), (np.float32, 1 << 15),
+                                     (np.float32, 1 << 16), (np.float64, 1024),
+                                     (np.float64, 1 << 13), (np.float64, 1 << 15)])
+def test_plain_path_matches_oracle(variant, dtype, b):
+    # the unfused stages (single-pass rows up to 64 KB, column + row passes
+    # beyond) against the oracle's closed form with the same hashes
+    m, n, p, d = 230, 150, 170, 3
+    print(f"plain {variant} {np.dtype(dtype).name} b={b}", flush=True)
+    AT, B, _ = workloads.planted(m, n, p, 8, 5.0, 31 + b % 97, dtype)
+    sk = cma.compress(variant, AT, B, d, b, seed=606, dtype=dtype, flags=cma.FLAG_PLAIN)
+    hA, hB = sk.hashes()
+    want = orc.sketch_from_product(variant, AT.astype(np.float64).T @ B.astype(np.float64), hA, hB, b)
+    got = sk.values()
+    if dtype == np.float64:
+        assert np.abs(got - want).max() <= 1e-10 * max(np.abs(want).max(), 1.0)
+    else:
+        err = rel_l2(got, want)
+        print(f"  rel err {err:.2e}")
+        assert err <= 1e-5
+    C = sk.decompress()
+    assert np.array_equal(C.astype(np.float64), orc.decompress(variant, got, hA, hB))
+    sk.free()
+
+
+def test_plain_path_empty_and_multi_pass():
+    # n = 0 gives a zero sketch; rows of 2^15 reals (FWHT) split in passes
+    AT = np.zeros((0, 40), np.float32)
+    B = np.zeros((0, 30), np.float32)
+    sk = cma.compress("fft", AT, B, 3, 1 << 15, seed=1, dtype=np.float32, flags=cma.FLAG_PLAIN)
+    assert not sk.values().any()
+    sk.free()
