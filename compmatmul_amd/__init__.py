@@ -451,7 +451,7 @@ class _Request(ctypes.Structure):
                 ("seed", ctypes.c_uint64), ("family", ctypes.c_int), ("tau", ctypes.c_double),
                 ("max_out", ctypes.c_int64), ("format", ctypes.c_int), ("AT", _vp), ("B", _vp),
                 ("at_indptr", _vp), ("at_indices", _vp), ("at_values", _vp), ("b_indptr", _vp),
-                ("b_indices", _vp), ("b_values", _vp), ("C", _vp)]
+                ("b_indices", _vp), ("b_values", _vp), ("C", _vp), ("mem", ctypes.c_int)]
 
 
 class _ServerStats(ctypes.Structure):
@@ -496,44 +496,70 @@ class Server:
                                      int(grain), ctypes.byref(self.handle)))
 
     def submit(self, variant, AT, B, d, b, seed, hash=None, tau=None, max_out=4096,
-               want_C=False, shape=None):
+               want_C=False, shape=None, C=None):
         """AT (n x m) and B (n x p): float32 arrays / HostBuffers, or CSR
         operands (scipy.sparse CSR or (indptr, indices, values); then pass
-        shape=(m, n, p)).  Returns the ticket."""
+        shape=(m, n, p)).  DeviceArray operands (dense, or a CSR tuple of
+        DeviceArrays) stay resident in HBM and are read in place; such a
+        request writes C only into a float32 DeviceArray `C` (m x p).
+        Returns the ticket."""
         if hash not in FAMILIES:
             raise ValueError(f"Invalid hash `{hash}'")
         r = _Request()
         keep = []
         sparse = hasattr(AT, "indptr") or isinstance(AT, tuple)
+        dev = isinstance(AT, DeviceArray) or (isinstance(AT, tuple) and isinstance(AT[0], DeviceArray))
         if sparse:
             if shape is None:
                 if not hasattr(AT, "shape"):
                     raise ValueError("pass shape=(m, n, p) with tuple CSR operands")
                 shape = (AT.shape[1], AT.shape[0], B.shape[1])
             m, n, p = shape
-            a, bb = _csr_parts(AT), _csr_parts(B)
+            if dev:
+                a, bb = tuple(AT), tuple(B)
+                if not all(isinstance(x, DeviceArray) for x in a + bb):
+                    raise ValueError("device CSR operands are tuples of DeviceArrays")
+                ptrs_a, ptrs_b = [x.ptr for x in a], [x.ptr for x in bb]
+            else:
+                a, bb = _csr_parts(AT), _csr_parts(B)
+                ptrs_a, ptrs_b = [x.ctypes.data for x in a], [x.ctypes.data for x in bb]
             keep += [a, bb]
             r.format = CSR
-            r.at_indptr, r.at_indices, r.at_values = (x.ctypes.data for x in a)
-            r.b_indptr, r.b_indices, r.b_values = (x.ctypes.data for x in bb)
+            r.at_indptr, r.at_indices, r.at_values = ptrs_a
+            r.b_indptr, r.b_indices, r.b_values = ptrs_b
         else:
-            a, bb = _host_f32(AT), _host_f32(B)
+            if dev:
+                if not isinstance(B, DeviceArray) or AT.dtype != np.float32 or B.dtype != np.float32:
+                    raise ValueError("device operands are float32 DeviceArrays")
+                a, bb = AT, B
+            else:
+                a, bb = _host_f32(AT), _host_f32(B)
             n, m = a.shape
             if bb.shape[0] != n:
                 raise ValueError("Shape mismatch! Note that AT is assumed to be transposed")
             p = bb.shape[1]
             keep += [a, bb, AT, B]
             r.format = DENSE
-            r.AT, r.B = a.ctypes.data, bb.ctypes.data
+            r.AT, r.B = (a.ptr, bb.ptr) if dev else (a.ctypes.data, bb.ctypes.data)
         r.variant, r.m, r.n, r.p, r.d, r.b = VARIANTS[variant], m, n, p, int(d), int(b)
         r.seed, r.family = int(seed), FAMILIES[hash]
         r.tau = float("nan") if tau is None else float(tau)
         r.max_out = int(max_out)
-        C = None
-        if want_C:
+        r.mem = DEVICE if dev else HOST
+        if dev:
+            if want_C:
+                raise ValueError("a device-resident request writes C into a DeviceArray: pass C=")
+            if C is not None:
+                if not isinstance(C, DeviceArray) or C.shape != (m, p) or C.dtype != np.float32:
+                    raise ValueError("C must be a float32 m x p DeviceArray")
+                keep.append(C)
+                r.C = C.ptr
+        elif want_C:
             C = np.empty((m, p), np.float32)
             keep.append(C)
             r.C = C.ctypes.data
+        elif C is not None:
+            raise ValueError("C= is for device-resident requests; use want_C=True")
         t = ctypes.c_uint64()
         check(lib().cm_server_submit(self.handle, ctypes.byref(r), ctypes.byref(t)))
         self._keep[t.value] = (keep, C, int(max_out), (m, p), (int(d), int(b)))

@@ -6,8 +6,9 @@
 //      back, stamps its completion time, and returns the wave's memory;
 //   2. if a wave slot is free, admits queued requests in arrival order while
 //      their footprint fits the HBM pool (at most max_wave_products), uploads
-//      their operands on the slot's stream (CSR operands are expanded on the
-//      device), plans them as one batch in the pool and launches it.
+//      their host operands on the slot's stream (device-resident operands are
+//      read in place; CSR operands are expanded on the device), plans them as
+//      one batch in the pool and launches it.
 // Up to max_waves waves run at once on separate streams, so the persistent
 // compress kernels of a new wave fill the CUs that an older wave's tail
 // leaves idle.  Memory is one device allocation made at creation and carved
@@ -182,13 +183,16 @@ size_t keep_bytes(const cm_request& r) {
 size_t nnz(const int32_t* indptr, int n) { return indptr ? size_t(indptr[n]) : 0; }
 
 // operand staging + dense operands + C of one request, in its wave block
+// (device-resident dense operands and C are used in place)
 size_t operand_bytes(const cm_request& r) {
-  size_t s = align_up(size_t(r.n) * r.m * 4) + align_up(size_t(r.n) * r.p * 4);
-  if (r.format == CM_CSR) {
+  const bool dev = r.mem == CM_DEVICE;
+  size_t s = 0;
+  if (!dev || r.format == CM_CSR) s += align_up(size_t(r.n) * r.m * 4) + align_up(size_t(r.n) * r.p * 4);
+  if (r.format == CM_CSR && !dev) {
     s += 2 * align_up(size_t(r.n + 1) * 4);
     s += 2 * align_up(nnz(r.at_indptr, r.n) * 4) + 2 * align_up(nnz(r.b_indptr, r.n) * 4);
   }
-  if (r.C) s += align_up(size_t(r.m) * r.p * 4);
+  if (r.C && !dev) s += align_up(size_t(r.m) * r.p * 4);
   return s;
 }
 
@@ -361,11 +365,25 @@ bool launch_wave(cm_server* sv, Slot& w) {
   std::string err;
   for (size_t k = 0; k < take.size() && rc == CM_OK; ++k) {
     const cm_request& r = take[k]->r;
+    const bool dev = r.mem == CM_DEVICE;
     cm_product q = as_product(r);
-    float* dAT = reinterpret_cast<float*>(carve(size_t(r.n) * r.m * 4));
-    float* dB = reinterpret_cast<float*>(carve(size_t(r.n) * r.p * 4));
     hipError_t e = hipSuccess;
-    if (r.format == CM_DENSE) {
+    float* dAT = nullptr;
+    float* dB = nullptr;
+    if (dev && r.format == CM_DENSE) {  // resident operands, read in place
+      dAT = const_cast<float*>(r.AT);
+      dB = const_cast<float*>(r.B);
+    } else {
+      dAT = reinterpret_cast<float*>(carve(size_t(r.n) * r.m * 4));
+      dB = reinterpret_cast<float*>(carve(size_t(r.n) * r.p * 4));
+    }
+    if (dev) {
+      if (r.format == CM_CSR) {  // resident CSR arrays, expanded in place
+        e = cm::launch_csr_dense(r.at_indptr, r.at_indices, r.at_values, r.n, r.m, r.m, dAT, w.s);
+        if (e == hipSuccess)
+          e = cm::launch_csr_dense(r.b_indptr, r.b_indices, r.b_values, r.n, r.p, r.p, dB, w.s);
+      }
+    } else if (r.format == CM_DENSE) {
       if (size_t(r.n) * r.m) e = hipMemcpyAsync(dAT, r.AT, size_t(r.n) * r.m * 4, hipMemcpyHostToDevice, w.s);
       if (e == hipSuccess && size_t(r.n) * r.p)
         e = hipMemcpyAsync(dB, r.B, size_t(r.n) * r.p * 4, hipMemcpyHostToDevice, w.s);
@@ -395,7 +413,9 @@ bool launch_wave(cm_server* sv, Slot& w) {
     }
     q.AT = dAT;
     q.B = dB;
-    if (r.C) {
+    if (r.C && dev) {
+      q.C = r.C;  // written in place, nothing to read back
+    } else if (r.C) {
       w.dev_C[k] = reinterpret_cast<float*>(carve(size_t(r.m) * r.p * 4));
       q.C = w.dev_C[k];
     }
@@ -482,9 +502,11 @@ extern "C" {
 int cm_server_create(size_t hbm_bytes, int max_waves, int max_wave_products, int grain,
                      cm_server** out) {
   if (!out) return error(CM_EINVAL, "null output handle");
-  if (hbm_bytes < (size_t(1) << 20) || max_waves < 1 || max_waves > 8 || max_wave_products < 1 ||
+  // HIP gives a process four hardware queues by default: more wave streams
+  // would share them and not run concurrently
+  if (hbm_bytes < (size_t(1) << 20) || max_waves < 1 || max_waves > 4 || max_wave_products < 1 ||
       grain < 1)
-    return error(CM_EINVAL, "need hbm_bytes >= 1 MiB, 1 <= max_waves <= 8, max_wave_products >= 1, grain >= 1");
+    return error(CM_EINVAL, "need hbm_bytes >= 1 MiB, 1 <= max_waves <= 4, max_wave_products >= 1, grain >= 1");
   std::lock_guard<std::recursive_mutex> lk(cm::rt::mutex());
   int rc = cm::rt::init();
   if (rc) return rc;
@@ -530,8 +552,11 @@ int cm_server_submit(cm_server* sv, const cm_request* r, uint64_t* ticket) {
     return error(CM_ENOTSUP, "shape outside the register kernels' range");
   if (!std::isnan(r->tau) && r->max_out < 0) return error(CM_EINVAL, "max_out must be >= 0");
   int rc;
+  if (r->mem != CM_HOST && r->mem != CM_DEVICE) return error(CM_EINVAL, "mem must be CM_HOST or CM_DEVICE");
   if (r->format == CM_DENSE) {
     if (r->n > 0 && ((r->m > 0 && !r->AT) || (r->p > 0 && !r->B))) return error(CM_EINVAL, "missing operand");
+  } else if (r->format == CM_CSR && r->mem == CM_DEVICE) {
+    if (r->n > 0 && (!r->at_indptr || !r->b_indptr)) return error(CM_EINVAL, "missing CSR arrays");
   } else if (r->format == CM_CSR) {
     if ((rc = check_csr(r->at_indptr, r->at_indices, r->at_values, r->n, r->m, "AT")) ||
         (rc = check_csr(r->b_indptr, r->b_indices, r->b_values, r->n, r->p, "B")))

@@ -267,7 +267,7 @@ typedef struct cm_request {
   double tau;                   /* recover entries > tau (NaN: none) */
   int64_t max_out;              /* recovered entries kept */
   int format;                   /* CM_DENSE or CM_CSR, for both operands */
-  /* CM_DENSE: host FP32 row-major AT (n x m) and B (n x p) */
+  /* CM_DENSE: FP32 row-major AT (n x m) and B (n x p) */
   const float* AT;
   const float* B;
   /* CM_CSR: AT and B as n-row CSR matrices (indptr[n + 1], indices, values) */
@@ -277,13 +277,20 @@ typedef struct cm_request {
   const int32_t* b_indptr;
   const int32_t* b_indices;
   const float* b_values;
-  float* C;                     /* host m x p output, or NULL */
+  float* C;                     /* m x p output, or NULL */
+  int mem;                      /* where AT, B (or the CSR arrays) and C live:
+                                   CM_HOST (0): uploaded by the server, C read
+                                   back; CM_DEVICE: used in place (operands
+                                   resident in HBM), C written directly.
+                                   Device CSR arrays are not validated on the
+                                   host; columns out of range are dropped. */
 } cm_request;
 
 typedef struct cm_server cm_server;
 
 /* hbm_bytes: device memory the server may hold; max_waves: waves on the
- * device at once; max_wave_products: requests per wave; grain as for
+ * device at once (1 .. 4, one stream each: HIP gives a process four hardware
+ * queues by default); max_wave_products: requests per wave; grain as for
  * cm_batch_plan (must be > 0: results then depend on the request only). */
 int cm_server_create(size_t hbm_bytes, int max_waves, int max_wave_products, int grain,
                      cm_server** out);

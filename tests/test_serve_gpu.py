@@ -4,8 +4,10 @@ Bars: every request's result (recovered entries and C) bit-identical
 whether it runs alone or mixed with other requests in overlapping waves;
 identical to the batch path with the same grain (itself pinned to the
 oracle in test_batch_gpu.py); CSR operands give the dense result bit for
-bit; the retained sketch decompresses to the same C; admission respects the
-HBM budget.
+bit, and their sketch, C and recovered set match the oracle; device-resident
+requests give the host requests' results; the retained sketch decompresses
+to the same C; admission respects the HBM budget.  The Poisson serving form
+is checked against the oracle request by request in test_configs_gpu.py.
 """
 import numpy as np
 import pytest
@@ -13,6 +15,7 @@ import scipy.sparse as sp
 
 import compmatmul_amd as cma
 from compmatmul_amd import workloads
+from oracle import oracle as orc
 
 pytestmark = pytest.mark.gpu
 
@@ -91,9 +94,16 @@ def test_csr_operands_match_dense():
         r1, r2 = sv.result(t1), sv.result(t2)
         assert np.array_equal(r1[4], r2[4])
         assert np.array_equal(r1[0], r2[0]) and np.array_equal(r1[1], r2[1])
-        truth = set(zip(*np.nonzero(A.T.toarray().astype(np.float64) @ Bs.toarray() > 12.5)))
-        got = set(zip(r1[0].tolist(), r1[1].tolist()))
-        assert truth <= got or len(truth & got) >= 0.95 * len(truth)
+        # against the oracle: the sketch within 1e-5 of the closed form, the
+        # recovered set exactly the oracle estimate's entries above tau
+        hA, hB = orc.hash_arrays(0, 7, 1 << 12, 77, m, p)
+        C64 = A.T.toarray().astype(np.float64) @ Bs.toarray().astype(np.float64)
+        want = orc.sketch_from_product(v, C64, hA, hB, 1 << 12)
+        vals = sv.values(t1).astype(np.float64)
+        assert np.linalg.norm(vals - want) <= 1e-5 * np.linalg.norm(want)
+        est = orc.decompress(v, vals, hA, hB)
+        assert np.array_equal(r1[4].astype(np.float64), est)
+        assert set(zip(r1[0].tolist(), r1[1].tolist())) == set(zip(*np.nonzero(est > 12.5)))
         sv.release(t1)
         sv.release(t2)
     sv.close()
@@ -127,4 +137,37 @@ def test_bad_csr_rejected():
     with pytest.raises(ValueError):
         sv.submit("fft", (ip, ix, vals), (ip, np.array([0, 1], np.int32), vals), 3, 64, 1,
                   shape=(4, 2, 4))
+    sv.close()
+
+
+def test_device_resident_requests_match_host_requests():
+    # dense and CSR operands resident in HBM, read in place, C written into
+    # a device array: the same sketch, C and recovered list as the request
+    # with host operands
+    rng = np.random.default_rng(21)
+    m, n, p = 420, 650, 380
+    A = sp.random(n, m, density=0.01, format="csr", random_state=5, dtype=np.float32,
+                  data_rvs=lambda k: rng.standard_normal(k).astype(np.float32))
+    Bs = sp.random(n, p, density=0.01, format="csr", random_state=6, dtype=np.float32,
+                   data_rvs=lambda k: rng.standard_normal(k).astype(np.float32))
+    AT, B = data(77, m, n, p)
+    sv = cma.Server(hbm_bytes=1 << 30, max_waves=2, grain=GRAIN)
+    dev_csr = lambda x: tuple(cma.DeviceArray.from_numpy(np.ascontiguousarray(v, dt))
+                              for v, dt in ((x.indptr, np.int32), (x.indices, np.int32),
+                                            (x.data, np.float32)))
+    cases = [("dense", (AT, B), (cma.DeviceArray.from_numpy(AT), cma.DeviceArray.from_numpy(B)), None),
+             ("csr", (A, Bs), (dev_csr(A), dev_csr(Bs)), (m, n, p))]
+    for v in ("fft", "fwht"):
+        for name, host, dev, shape in cases:
+            print(f"{v} {name}", flush=True)
+            th = sv.submit(v, host[0], host[1], 7, 1 << 13, 31, tau=12.5, want_C=True)
+            dC = cma.DeviceArray((m, p), np.float32)
+            td = sv.submit(v, dev[0], dev[1], 7, 1 << 13, 31, tau=12.5, C=dC, shape=shape)
+            assert sv.wait(th, timeout=120) and sv.wait(td, timeout=120)
+            rh, rd = sv.result(th), sv.result(td)
+            assert np.array_equal(sv.values(th), sv.values(td))
+            assert np.array_equal(rh[4], dC.numpy())
+            assert all(np.array_equal(x, y) for x, y in zip(rh[:3], rd[:3]))
+            sv.release(th)
+            sv.release(td)
     sv.close()
