@@ -1,17 +1,27 @@
 """Benchmark of the compressed matrix product on one MI355X.
 
-Workload (the metric's single-GPU configuration): plain method, FWHT
-(XOR-convolution) variant, A 2048x2048 and B 2048x2048 fp32 with iid
-N(0, 1/2048) noise plus 128 planted entries of magnitude 6.0, sketch length
-b = 2^14, d = 15 repetitions, multiply-add-shift hashes with seed 13794.
+Default workload (the metric's configuration: products/s and p50/p99 latency
+at a fixed QPS, against the plain GPU path at equal recall): the serving
+form.  Requests arrive as a Poisson stream at a fixed QPS; each request's
+shape is drawn from the batch distribution (m, p log-uniform in [64, 4096],
+n in [128, 8192], b = 2^10 .. 2^16, d in {5, 7, 9, 11, 15}, FFT / FWHT
+50/50), 30% of them with CSR operands at 1% density (N(0, 1) values plus
+planted entries of 5.0), the rest dense N(0, 1/n) plus planted entries; the
+recovered entries (> tau = 12.5) are each request's result.  The operands
+of the 64 request templates are resident in HBM before the timed region
+(dense arrays, or CSR arrays the server expands on the device).
 
-One step = one full product from inputs already resident in HBM: hash
-functions drawn and evaluated, fused sketch -> transform -> spectral MAC,
-reduction and inverse, then the median-of-d decompression of all of C into
-HBM (the reference's matmul_fwht).  The recovered set (> tau = 18) is checked
-against the FP64 dense product after the timed region.
+One step = one request, from its arrival to its recovered list on the host.
+value = requests served / wall time of the timed requests.  The same
+arrivals are then replayed through one product per launch sequence on the
+fused single-product path and on the plain (unfused) path; their recall is
+compared on the first templates.  The roofline block is the dominant
+kernel class of the serving run (the persistent compress kernels), timed
+with HIP events on the wave streams it runs on.  The FP64 oracle on the
+host cores is the CPU baseline, on a bounded sample of the templates.
 
-Prints one JSON line (see the keys below).  --gpus N with N > 1 exits: this
+Other workloads (--workload): the named single-product configurations, the
+512-product batch, the precision form.  --gpus N with N > 1 exits: this
 build measures one GPU.
 """
 import argparse
@@ -28,14 +38,16 @@ sys.path.insert(0, HERE)
 import compmatmul_amd as cma  # noqa: E402
 from compmatmul_amd import workloads  # noqa: E402
 
-DEFAULT_WORKLOAD = "fwht2048_fp32"
+DEFAULT_WORKLOAD = "serve"
 HASH_SEED = 13794
 PEAK_FP32_TFLOPS = 157.3
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
-# largest sketch length of the unfused (plain) FP32 path: one LDS-resident
-# transform of at most 64 KB (FFT: complex)
-PLAIN_MAX_B = {"fft": 1 << 13, "fwht": 1 << 14}
+# The compress kernels run FP32 adds and multiplies on the vector ALUs (no
+# matrix cores); 157.3 TFLOP/s is MI355X's published FP32 peak (packed FMA).
+COMPUTE_BOUND = "valu"
+PEAK_NOTE = ("FP32 vector ALU: 157.3 TFLOP/s is the packed-FMA peak; with packed FP32 "
+             "disabled, add-only work peaks at 39.3 TFLOP/s")
 
 
 def compress_flops(variant, m, n, p, d, b):
@@ -48,32 +60,6 @@ def compress_flops(variant, m, n, p, d, b):
     if variant == "fwht":
         return 2 * d * n * (m + p) + 2 * d * n * b * lb + 2 * d * n * b
     return 2 * d * n * (m + p) + 2 * d * n * 2.5 * b * lb + 8 * d * n * (b // 2 + 1)
-
-
-def cpu_baseline(c, AT, B):
-    """The FP64 oracle on the host: compress over a bounded slice of the inner
-    dimension, scaled to the full product, plus the full decompression."""
-    sys.path.insert(0, HERE)
-    from oracle import oracle as orc
-    threads = min(16, os.cpu_count() or 1)
-    n = AT.shape[0]
-    ns = 128
-    AT64 = np.ascontiguousarray(AT[:ns], np.float64)
-    B64 = np.ascontiguousarray(B[:ns], np.float64)
-    orc.compress(c["variant"], AT64[:8], B64[:8], c["d"], c["b"], HASH_SEED, 0, threads)
-    t0 = time.perf_counter()
-    sk = orc.compress(c["variant"], AT64, B64, c["d"], c["b"], HASH_SEED, 0, threads)
-    t1 = time.perf_counter()
-    hA, hB = orc.hash_arrays(0, c["d"], c["b"], HASH_SEED, c["m"], c["p"])
-    t2 = time.perf_counter()
-    orc.decompress(c["variant"], sk, hA, hB, threads)
-    t3 = time.perf_counter()
-    per_product = (t1 - t0) * (n / ns) + (t3 - t2)
-    return {"value": 1.0 / per_product, "unit": "products/s", "cores": threads,
-            "kind": "port",
-            "sample": f"FP64 oracle (C++/OpenMP restatement, not FFTW/FXT): compress over "
-                      f"{ns} of {n} inner rows scaled x{n // ns}, plus the full "
-                      f"{c['m']}x{c['p']} median-of-{c['d']} decompression"}
 
 
 def copy_probe(nbytes):
@@ -166,7 +152,7 @@ def bench_batch(args):
         "config": {"workload": "batch of 512 products: m, p log-uniform [64, 4096], n [128, 8192], "
                                "b 2^10..2^16, d {5,7,9,11,15}, FFT/FWHT 50/50, tau 12.5",
                    "grain": bt.grain, "graph": True, "parallelism": "single GPU"},
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
+        "roofline": {"bound": COMPUTE_BOUND, "peak_note": PEAK_NOTE, "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
                      "kernel": "compress_fwht + compress_fft (persistent)",
                      "kernel_ms": comp_s * 1e3, "flops_per_launch": flops},
@@ -244,7 +230,7 @@ def bench_precision(args):
         "data": f"synthetic: N(0,1/{n}) noise + {c['plants']} planted entries of {c['magnitude']}",
         "config": {"workload": f"precision form: fft {m}x{n}x{p}, b={b}, d={d}, tau={tau}, "
                                "three arithmetic modes", "parallelism": "single GPU"},
-        "roofline": {"bound": "mfma", "achieved": flops / k_s / 1e12, "peak": PEAK_FP32_TFLOPS,
+        "roofline": {"bound": COMPUTE_BOUND, "peak_note": PEAK_NOTE, "achieved": flops / k_s / 1e12, "peak": PEAK_FP32_TFLOPS,
                      "unit": "TFLOP/s", "frac": flops / k_s / 1e12 / PEAK_FP32_TFLOPS,
                      "traffic": None, "kernel": "compress (fp32 mode)",
                      "kernel_ms": k_s * 1e3, "flops_per_launch": flops},
@@ -252,10 +238,17 @@ def bench_precision(args):
     }
 
 
-def serve_datasets(count, seed):
-    """`count` request templates from the batch shape distribution; 30% hold
-    CSR operands at 1% density (N(0,1) values plus the planted entries),
-    the rest dense operands in pinned host memory."""
+SERVE_TEMPLATES = 64
+SERVE_SEED = workloads.BATCH_SEED + 100
+SERVE_REQ_PER_STEP = 64  # one serving step: a window of 64 requests
+SERVE_QPS = 1000.0       # the fixed arrival rate of the serving workload
+
+
+def serve_templates(count, seed):
+    """`count` request templates from the batch shape distribution, operands
+    resident in HBM: 30% CSR at 1% density (N(0,1) values plus the planted
+    entries; device CSR arrays, and a dense device copy for the
+    single-product paths, which take dense operands), the rest dense."""
     import scipy.sparse as sparse
     specs = workloads.batch_specs(count, seed)
     rng = np.random.default_rng(seed + 1)
@@ -271,37 +264,153 @@ def serve_datasets(count, seed):
                 k = rng.integers(n)
                 A[k, rng.integers(m)] = s["magnitude"]
                 Bs[k, rng.integers(p)] = s["magnitude"]
-            out.append(dict(s, csr=True, AT=A.tocsr(), B=Bs.tocsr()))
+            A, Bs = A.tocsr(), Bs.tocsr()
+
+            def dev_csr(x):
+                return tuple(cma.DeviceArray.from_numpy(np.ascontiguousarray(v, dt))
+                             for v, dt in ((x.indptr, np.int32), (x.indices, np.int32),
+                                           (x.data, np.float32)))
+            AT, B = A.toarray(), Bs.toarray()
+            out.append(dict(s, csr=True, host=(AT, B), serve=(dev_csr(A), dev_csr(Bs)),
+                            dense=(cma.DeviceArray.from_numpy(AT), cma.DeviceArray.from_numpy(B))))
         else:
             AT, B, _ = workloads.planted(m, n, p, s["plants"], s["magnitude"], seed + 2 * i,
                                          np.float32, direct=True)
-            out.append(dict(s, csr=False, AT=cma.HostBuffer.from_numpy(AT),
-                            B=cma.HostBuffer.from_numpy(B)))
+            dev = (cma.DeviceArray.from_numpy(AT), cma.DeviceArray.from_numpy(B))
+            out.append(dict(s, csr=False, host=(AT, B), serve=dev, dense=dev))
     return out
 
 
-def bench_serve(args):
-    """Poisson arrivals at a fixed QPS into the continuous-batching server;
-    the same arrivals replayed through one product per launch sequence."""
-    templates = serve_datasets(64, workloads.BATCH_SEED + 100)
-    rng = np.random.default_rng(7)
-    nreq = args.requests
-    arrivals = np.cumsum(rng.exponential(1.0 / args.qps, nreq))
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_threads():
+    """Threads the oracle uses: OMP_NUM_THREADS when set, else the core count
+    (at most 16, the host share of one GPU)."""
+    env = os.environ.get("OMP_NUM_THREADS")
+    n = int(env) if env and env.isdigit() else (os.cpu_count() or 1)
+    return max(1, min(16, n))
+
+
+def time_oracle(products, threads, runs=5):
+    """1 warm-up and `runs` timed passes of the FP64 oracle (compress plus the
+    full median-of-d decompression) over `products`; the p50 pass time."""
+    from oracle import oracle as orc
+    def one():
+        for q in products:
+            sk = orc.compress(q["variant"], q["AT"], q["B"], q["d"], q["b"], HASH_SEED, 0, threads)
+            hA, hB = orc.hash_arrays(0, q["d"], q["b"], HASH_SEED, q["AT"].shape[1], q["B"].shape[1])
+            orc.decompress(q["variant"], sk, hA, hB, threads)
+    t0 = time.perf_counter()
+    one()
+    ts = [time.perf_counter() - t0] if runs == 0 else []  # runs = 0: the warm-up's time
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        one()
+        ts.append(time.perf_counter() - t0)
+    return float(np.median(ts)), ts
+
+
+def cpu_baseline_serving(templates, budget_gflop=60.0):
+    """The oracle on a bounded sample of the serving templates: templates in
+    order while their compress work stays within `budget_gflop`."""
+    threads = cpu_threads()
+    sample, flops = [], 0.0
+    for t in templates:
+        f = compress_flops(t["variant"], t["m"], t["n"], t["p"], t["d"], t["b"])
+        if sample and flops + f > budget_gflop * 1e9:
+            continue
+        sample.append(dict(variant=t["variant"], d=t["d"], b=t["b"],
+                           AT=np.ascontiguousarray(t["host"][0], np.float64),
+                           B=np.ascontiguousarray(t["host"][1], np.float64)))
+        flops += f
+        if flops > budget_gflop * 1e9 * 0.8:
+            break
+    p50, ts = time_oracle(sample, threads)
+    shapes = ", ".join(f"{q['variant']} {q['AT'].shape[1]}x{q['AT'].shape[0]}x{q['B'].shape[1]} "
+                       f"b={q['b']} d={q['d']}" for q in sample)
+    return {"value": len(sample) / p50, "unit": "products/s", "cores": threads, "kind": "port",
+            "cpu_model": cpu_model(),
+            "sample": f"FP64 oracle (C++/OpenMP restatement, not FFTW/MKL), compress plus full "
+                      f"decompression of {len(sample)} serving templates ({shapes}); 1 warm-up, "
+                      f"p50 of {len(ts)} runs",
+            "runs_s": ts}
+
+
+def cpu_full_configs(threads, names=("fft512_fp64", "fwht2048_fp32")):
+    """The CPU timing of the two smaller configurations in full: the FP64
+    oracle, 1 warm-up and 5 timed runs (one, when a run exceeds 15 s)."""
+    out = {}
+    for name in names:
+        c = workloads.CONFIGS[name]
+        AT, B, _ = workloads.make(name)
+        q = dict(variant=c["variant"], d=c["d"], b=c["b"], AT=np.ascontiguousarray(AT, np.float64),
+                 B=np.ascontiguousarray(B, np.float64))
+        first, _ = time_oracle([q], threads, runs=0)  # the warm-up alone
+        runs = 5 if first < 15.0 else 1
+        p50, ts = time_oracle([q], threads, runs=runs)
+        out[name] = {"products_per_s": 1.0 / p50, "p50_s": p50, "runs_s": ts, "cores": threads}
+        print(f"[bench] cpu {name}: p50 {p50:.3f} s over {len(ts)} runs", file=sys.stderr, flush=True)
+    return out
+
+
+def plain_hbm_bytes(variant, m, n, p, d, b, esz=4):
+    """HBM bytes the plain path's stages move: the sketches written, the
+    transform's read and write (twice for rows past one 128 KB tile), the
+    spectra read by the MAC."""
+    rows_bytes = d * n * b * esz * 2  # FFT: one complex row; FWHT: an A and a B row
+    tile = (1 << 15) if variant == "fwht" else (1 << 14)
+    passes = 1 if b <= tile else 2
+    return rows_bytes * (1 + 2 * passes + 1)
+
+
+def serve_arrivals(nreq, qps, templates, seed=7):
+    rng = np.random.default_rng(seed)
+    arrivals = np.cumsum(rng.exponential(1.0 / qps, nreq))
     picks = rng.integers(0, len(templates), nreq)
+    return arrivals, picks
+
+
+def bench_serve(args):
+    """The serving form: Poisson arrivals at a fixed QPS into the
+    continuous-batching server, operands resident in HBM; then the same
+    arrivals replayed one product per launch sequence (fused, plain)."""
+    t_setup = time.perf_counter()
+    templates = serve_templates(SERVE_TEMPLATES, SERVE_SEED)
+    print(f"[bench] serving: {len(templates)} templates resident in HBM "
+          f"({sum(t['csr'] for t in templates)} CSR) in {time.perf_counter() - t_setup:.1f} s",
+          file=sys.stderr, flush=True)
+    nreq, qps = args.steps * SERVE_REQ_PER_STEP, args.qps
+    arrivals, picks = serve_arrivals(nreq, qps, templates)
     seeds = HASH_SEED + np.arange(nreq)
-
-    def operands(t):
-        return t["AT"], t["B"]
-
+    grain = args.grain or 64
+    max_out = 1 << 16
     sv = cma.Server(hbm_bytes=args.hbm_gb << 30, max_waves=args.waves,
-                    max_wave_products=args.wave_products, grain=args.grain or 64)
-    # warm-up: one request of every template
-    for k, t in enumerate(templates):
-        a, b = operands(t)
-        tk = sv.submit(t["variant"], a, b, t["d"], t["b"], 1 + k, tau=t["tau"], max_out=1 << 16)
+                    max_wave_products=args.wave_products, grain=grain)
+
+    def submit(t, seed):
+        a, b = t["serve"]
+        shape = (t["m"], t["n"], t["p"]) if t["csr"] else None
+        return sv.submit(t["variant"], a, b, t["d"], t["b"], int(seed), tau=t["tau"],
+                         max_out=max_out, shape=shape)
+
+    # warm-up: `warmup` windows of requests over every template, then drained
+    warm = [submit(templates[k % len(templates)], 1 + k)
+            for k in range(args.warmup * SERVE_REQ_PER_STEP)]
+    for tk in warm:
         sv.wait(tk)
         sv.release(tk)
     import threading
+    cma.timing_enable(True, only="compress*")
+    cma.timing_reset()
     tickets = [None] * nreq
     t0 = time.perf_counter() + 0.05
 
@@ -310,123 +419,244 @@ def bench_serve(args):
             dt = t0 + arrivals[r] - time.perf_counter()
             if dt > 0:
                 time.sleep(dt)
-            t = templates[picks[r]]
-            a, b = operands(t)
-            tickets[r] = sv.submit(t["variant"], a, b, t["d"], t["b"], int(seeds[r]),
-                                   tau=t["tau"], max_out=1 << 16)
+            tickets[r] = submit(templates[picks[r]], seeds[r])
 
     th = threading.Thread(target=feeder)
     th.start()
     lat = np.zeros(nreq)
-    results = {}
+    served = {}
     for r in range(nreq):
         while tickets[r] is None:
             time.sleep(1e-4)
         sv.wait(tickets[r])
         ii, jj, _, lat[r], _ = sv.result(tickets[r])
-        if picks[r] < 16 and picks[r] not in results:
-            results[picks[r]] = set(zip(ii.tolist(), jj.tolist()))
+        if picks[r] < 16 and picks[r] not in served:
+            served[int(picks[r])] = set(zip(ii.tolist(), jj.tolist()))
         sv.release(tickets[r])
     th.join()
     t_end = time.perf_counter()
+    kern = {k: cma.timing_get(k) for k in ("compress_fft", "compress_fwht")}
+    cma.timing_enable(False)
     st = sv.stats()
     sv.close()
     served_s = t_end - t0
+    print(f"[bench] served {nreq} requests in {served_s:.2f} s", file=sys.stderr, flush=True)
 
-    # one product per launch sequence: the same arrivals, served in order,
-    # by the fused single-product path and by the unfused (plain) path where
-    # its LDS limit allows the sketch length (else fused)
-    dev = {}
-    for k, t in enumerate(templates):  # operands resident on the device, as for the server
-        if t["csr"]:
-            a, b = t["AT"].toarray(), t["B"].toarray()
-        else:
-            a, b = t["AT"].array, t["B"].array
-        dev[k] = (cma.DeviceArray.from_numpy(a), cma.DeviceArray.from_numpy(b))
+    # roofline of the dominant kernel class over the timed region: the
+    # algorithmic FLOPs of the requests' compress work / the summed launch
+    # durations (events on each wave's stream)
+    flops = {"compress_fft": 0.0, "compress_fwht": 0.0}
+    for r in range(nreq):
+        t = templates[picks[r]]
+        flops["compress_" + t["variant"]] += compress_flops(t["variant"], t["m"], t["n"], t["p"],
+                                                           t["d"], t["b"])
+    dom = max(kern, key=lambda k: kern[k][0])
+    dom_s = kern[dom][0] * 1e-3
+    achieved = flops[dom] / dom_s / 1e12 if dom_s > 0 else 0.0
 
-    def one_at_a_time(plain):
-        lat_ = np.zeros(nreq)
+    def one_at_a_time(plain, count):
+        """The same arrivals, one product per launch sequence, in order."""
+        lat_ = np.zeros(count)
+        rec = {}
         t1 = time.perf_counter() + 0.05
-        for r in range(nreq):
+        for r in range(count):
             dt = t1 + arrivals[r] - time.perf_counter()
             if dt > 0:
                 time.sleep(dt)
             t = templates[picks[r]]
-            a, b = dev[picks[r]]
-            fl = cma.FLAG_PLAIN if plain and t["b"] <= PLAIN_MAX_B[t["variant"]] else 0
+            a, b = t["dense"]
             sk = cma.compress(t["variant"], a, b, t["d"], t["b"], seed=int(seeds[r]),
-                              dtype=np.float32, flags=fl)
-            sk.recover(t["tau"])
+                              dtype=np.float32, flags=cma.FLAG_PLAIN if plain else 0)
+            ii, jj, _ = sk.recover(t["tau"])
             sk.free()
             lat_[r] = time.perf_counter() - (t1 + arrivals[r])
-        return lat_, time.perf_counter() - t1
+            if picks[r] < 16 and picks[r] not in rec:
+                rec[int(picks[r])] = set(zip(ii.tolist(), jj.tolist()))
+        return lat_, time.perf_counter() - t1, rec
 
-    base_lat, base_s = one_at_a_time(False)
-    plain_lat, plain_s = one_at_a_time(True) if args.plain_baseline else (None, None)
-
-    # recall of the served results on the first templates
-    hits = truth_n = got_n = 0
-    for k, got in results.items():
-        t = templates[k]
-        a = t["AT"].toarray() if t["csr"] else t["AT"].array
-        b = t["B"].toarray() if t["csr"] else t["B"].array
-        truth = set(zip(*np.nonzero(a.astype(np.float64).T @ b.astype(np.float64) > t["tau"])))
-        hits += len(got & truth)
-        truth_n += len(truth)
-        got_n += len(got)
     q = lambda x, f: float(np.quantile(x, f))
+    fused_lat, fused_s, fused_rec = one_at_a_time(False, nreq)
+    print(f"[bench] fused one at a time: {nreq / fused_s:.0f} products/s", file=sys.stderr, flush=True)
+    nplain = min(nreq, args.plain_requests)
+    plain_lat, plain_s, plain_rec = one_at_a_time(True, nplain)
+    print(f"[bench] plain one at a time: {nplain / plain_s:.0f} products/s", file=sys.stderr, flush=True)
+
+    # recall of each path against {C_true > tau} on the first templates seen
+    def recall(rec):
+        hits = truth_n = got_n = 0
+        for k, got in rec.items():
+            t = templates[k]
+            AT, B = t["host"]
+            truth = set(zip(*np.nonzero(AT.astype(np.float64).T @ B.astype(np.float64) > t["tau"])))
+            hits += len(got & truth)
+            truth_n += len(truth)
+            got_n += len(got)
+        return {"recall": hits / max(1, truth_n), "precision": hits / max(1, got_n),
+                "templates": len(rec)}
+    common = sorted(set(served) & set(fused_rec) & set(plain_rec))
+    rc = {name: recall({k: d[k] for k in common})
+          for name, d in (("server", served), ("fused_one_at_a_time", fused_rec),
+                          ("plain_one_at_a_time", plain_rec))}
+    value = nreq / served_s
+    plain_rate = nplain / plain_s
     return {
         "metric": "products/s",
-        "value": nreq / served_s,
+        "value": value,
         "unit": "products/s",
         "n_gpus": 1,
-        "steps": nreq,
-        "warmup": len(templates),
-        "ms_per_step": 1e3 * served_s / nreq,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * served_s / args.steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic: batch shape distribution, 30% CSR at 1% density, planted entries (seeded)",
-        "config": {"workload": f"serving: Poisson arrivals at {args.qps} QPS, {nreq} requests over "
-                               f"{len(templates)} request templates",
-                   "qps": args.qps, "max_waves": args.waves,
-                   "max_wave_products": args.wave_products, "grain": args.grain or 64,
+        "data": "synthetic: batch shape distribution, 30% CSR at 1% density, planted entries of "
+                "5.0 (seeded); operands resident in HBM",
+        "config": {"workload": f"serving: Poisson arrivals at {qps:g} QPS, {nreq} requests "
+                               f"({args.steps} steps of {SERVE_REQ_PER_STEP}) over "
+                               f"{len(templates)} templates (m, p in [64, 4096], n in [128, 8192], "
+                               "b 2^10..2^16, d {5,7,9,11,15}, FFT/FWHT 50/50, 30% CSR, tau 12.5)",
+                   "qps": qps, "max_waves": args.waves, "max_wave_products": args.wave_products,
+                   "grain": grain, "parallelism": "single GPU"},
+        "roofline": {"bound": COMPUTE_BOUND, "peak_note": PEAK_NOTE, "achieved": achieved,
+                     "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
+                     "traffic": None, "kernel": f"{dom} (persistent batch compress kernels)",
+                     "kernel_ms": 1e3 * dom_s / max(1, kern[dom][1]),
+                     "launches": kern[dom][1], "flops_timed_region": flops[dom]},
+        "detail": {
+            "latency_ms": {"p50": 1e3 * q(lat, 0.5), "p99": 1e3 * q(lat, 0.99),
+                           "max": 1e3 * float(lat.max())},
+            "vs_plain_one_product_per_launch": value / plain_rate,
+            "vs_fused_one_product_per_launch": value / (nreq / fused_s),
+            "fused_one_product_per_launch": {"products_per_s": nreq / fused_s,
+                                             "latency_ms": {"p50": 1e3 * q(fused_lat, 0.5),
+                                                            "p99": 1e3 * q(fused_lat, 0.99)}},
+            "plain_one_product_per_launch": {"products_per_s": plain_rate, "requests": nplain,
+                                             "latency_ms": {"p50": 1e3 * q(plain_lat, 0.5),
+                                                            "p99": 1e3 * q(plain_lat, 0.99)}},
+            "recall_equal": rc, "server": st,
+            "compress_ms_total": {k: v[0] for k, v in kern.items()},
+            "compress_launches": {k: v[1] for k, v in kern.items()},
+            "device_tflops_timed_region": (flops["compress_fft"] + flops["compress_fwht"]) / served_s / 1e12,
+        },
+    }, templates
+
+
+def bench_single(args, name, plain=False, quiet=False):
+    """One named configuration: `steps` products after `warmup`, the compress
+    class timed with HIP events; recovered set checked against the dense
+    FP64 product after the timed region."""
+    c = workloads.CONFIGS[name]
+    m, n, p, d, b = c["m"], c["n"], c["p"], c["d"], c["b"]
+    dtype = np.dtype(c["dtype"])
+    AT, B, _ = workloads.make(name)
+    print(f"[bench] workload {name} m={m} n={n} p={p} d={d} b={b}{' plain' if plain else ''}",
+          file=sys.stderr, flush=True)
+    dAT = cma.DeviceArray.from_numpy(AT)
+    dB = cma.DeviceArray.from_numpy(B)
+    dC = cma.DeviceArray((m, p), dtype)
+    flags = cma.FLAG_PLAIN if plain else 0
+    sk = None
+
+    def step():
+        nonlocal sk
+        sk = cma.compress(c["variant"], dAT, dB, d, b, seed=HASH_SEED, dtype=dtype,
+                          flags=flags, sketch=sk)
+        sk.decompress(dC)
+
+    for _ in range(args.warmup):
+        step()
+    cma.synchronize()
+    kname = "plain*" if plain else "compress"
+    cma.timing_enable(True, only=None if args.kernel_times else kname)
+    cma.timing_reset()
+    cma.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    cma.synchronize()
+    elapsed = time.perf_counter() - t0
+    totals = {k: cma.timing_get(k) for k in ("hash", "compress", "reduce", "combine",
+                                            "decompress", "plain_sketch", "plain_transform",
+                                            "plain_mac")}
+    cma.timing_enable(False)
+    print("[bench] per-kernel ms/launch: " + ", ".join(
+        f"{k}={v[0] / v[1]:.4f}" for k, v in totals.items() if v[1]), file=sys.stderr, flush=True)
+    C_est = dC.numpy()
+    C_true = AT.astype(np.float64).T @ B.astype(np.float64)
+    truth = set(zip(*np.nonzero(C_true > c["tau"])))
+    ii, jj, _ = sk.recover(c["tau"])
+    got = set(zip(ii.tolist(), jj.tolist()))
+    est_set = set(zip(*np.nonzero(C_est > c["tau"])))
+    recall = len(got & truth) / max(1, len(truth))
+    precision = len(got & truth) / max(1, len(got))
+    flops = compress_flops(c["variant"], m, n, p, d, b)
+    peak = PEAK_FP32_TFLOPS if dtype == np.float32 else PEAK_FP64_TFLOPS
+    if plain:  # the stages' time per product
+        k_avg_s = sum(totals[k][0] for k in ("plain_sketch", "plain_transform", "plain_mac")) * 1e-3 / args.steps
+        nbytes = plain_hbm_bytes(c["variant"], m, n, p, d, b, dtype.itemsize)
+        roof = {"bound": "hbm", "achieved": nbytes / k_avg_s / 1e9, "peak": PEAK_HBM_GBS,
+                "unit": "GB/s", "frac": nbytes / k_avg_s / 1e9 / PEAK_HBM_GBS, "traffic": None,
+                "kernel": "plain stages (sketch + transform + MAC)", "kernel_ms": k_avg_s * 1e3,
+                "bytes_per_product": nbytes}
+    else:
+        k_ms, k_n = totals["compress"]
+        k_avg_s = (k_ms / max(1, k_n)) * 1e-3
+        achieved = flops / k_avg_s / 1e12 if k_avg_s > 0 else 0.0
+        roof = {"bound": COMPUTE_BOUND, "peak_note": PEAK_NOTE, "achieved": achieved, "peak": peak,
+                "unit": "TFLOP/s", "frac": achieved / peak,
+                "traffic": traffic_from_profile(name),
+                "traffic_source": "profile constant: rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE "
+                                  f"(profiles/{name}_compress_traffic.md)",
+                "kernel": kname, "kernel_ms": k_avg_s * 1e3, "flops_per_launch": flops}
+    return {
+        "metric": "products/s",
+        "value": args.steps / elapsed,
+        "unit": "products/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * elapsed / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if dtype == np.float32 else "f64",
+        "data": f"synthetic: N(0,1/{n}) noise + {c['plants']} planted entries of "
+                f"{c['magnitude']} (seeded)",
+        "config": {"workload": f"{c['variant']} {m}x{n}x{p} {dtype.name}, b={b}, d={d}, "
+                               f"{c['plants']} plants, tau={c['tau']}",
+                   "m": m, "n": n, "p": p, "b": b, "d": d, "variant": c["variant"],
+                   "hash": "multiply add shift", "path": "plain" if plain else "fused",
                    "parallelism": "single GPU"},
-        "detail": {"latency_ms": {"p50": 1e3 * q(lat, 0.5), "p99": 1e3 * q(lat, 0.99),
-                                  "max": 1e3 * float(lat.max())},
-                   "one_product_per_launch": {"products_per_s": nreq / base_s,
-                                              "latency_ms": {"p50": 1e3 * q(base_lat, 0.5),
-                                                             "p99": 1e3 * q(base_lat, 0.99)}},
-                   "plain_one_product_per_launch": None if plain_lat is None else {
-                       "products_per_s": nreq / plain_s,
-                       "latency_ms": {"p50": 1e3 * q(plain_lat, 0.5),
-                                      "p99": 1e3 * q(plain_lat, 0.99)},
-                       "note": "unfused path for b within its LDS limit, fused otherwise"},
-                   "server": st, "recall": hits / max(1, truth_n),
-                   "precision": hits / max(1, got_n)},
+        "roofline": roof,
+        "detail": {"recall": recall, "precision": precision, "recovered": len(got),
+                   "truth": len(truth), "C_threshold_set_equal": est_set == got,
+                   "kernel_ms": {k: (v[0] / v[1] if v[1] else None) for k, v in totals.items()}},
     }
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=100)
-    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=20,
+                    help="timed steps (serve: windows of 64 requests; else products)")
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--plain", action="store_true", help="time the unfused path instead")
     ap.add_argument("--workload", default=DEFAULT_WORKLOAD,
                     choices=sorted(workloads.CONFIGS) + ["batch512", "serve", "precision"],
-                    help="named workload (default: the metric's configuration)")
+                    help="named workload (default: the metric's configuration, serving)")
     ap.add_argument("--grain", type=int, default=0,
                     help="batch512: work-item grain (0 = autotuned); serve: 0 = 64")
-    ap.add_argument("--qps", type=float, default=300.0, help="serve: arrival rate")
-    ap.add_argument("--requests", type=int, default=600, help="serve: requests")
+    ap.add_argument("--qps", type=float, default=SERVE_QPS, help="serve: arrival rate")
     ap.add_argument("--waves", type=int, default=3, help="serve: waves on the device at once")
     ap.add_argument("--wave-products", type=int, default=64, help="serve: requests per wave")
     ap.add_argument("--hbm-gb", type=int, default=64, help="serve: server HBM budget (GiB)")
-    ap.add_argument("--plain-baseline", action="store_true",
-                    help="serve: also replay the arrivals through the unfused path")
+    ap.add_argument("--plain-requests", type=int, default=512,
+                    help="serve: arrivals replayed through the plain path")
+    ap.add_argument("--no-detail-legs", action="store_true",
+                    help="serve: skip the single-product fwht 2048^3 legs in the detail")
     ap.add_argument("--kernel-times", action="store_true",
                     help="time every kernel class with HIP events (default: only the "
                          "dominant one, since each event pair slows the stream)")
@@ -447,96 +677,37 @@ def main():
     if name == "batch512":
         print(json.dumps(bench_batch(args)), flush=True)
         return
-    if name == "serve":
-        print(json.dumps(bench_serve(args)), flush=True)
-        return
     if name == "precision":
         print(json.dumps(bench_precision(args)), flush=True)
         return
-    c = workloads.CONFIGS[name]
-    m, n, p, d, b = c["m"], c["n"], c["p"], c["d"], c["b"]
-    dtype = np.dtype(c["dtype"])
-    AT, B, _ = workloads.make(name)
-    print(f"[bench] workload {name} m={m} n={n} p={p} d={d} b={b}", file=sys.stderr,
-          flush=True)
-    dAT = cma.DeviceArray.from_numpy(AT)
-    dB = cma.DeviceArray.from_numpy(B)
-    dC = cma.DeviceArray((m, p), dtype)
-    flags = cma.FLAG_PLAIN if args.plain else 0
-    sk = None
+    if name != "serve":
+        out = bench_single(args, name, plain=args.plain)
+        if not args.no_cpu_baseline and name in ("fft512_fp64", "fwht2048_fp32"):
+            full = cpu_full_configs(cpu_threads(), [name])[name]
+            out["cpu_baseline"] = {"value": full["products_per_s"], "unit": "products/s",
+                                   "cores": full["cores"], "kind": "port", "cpu_model": cpu_model(),
+                                   "sample": f"FP64 oracle (C++/OpenMP restatement, not FFTW/MKL): "
+                                             f"the full product, 1 warm-up, p50 of "
+                                             f"{len(full['runs_s'])} runs"}
+        print(json.dumps(out), flush=True)
+        return
 
-    def step():
-        nonlocal sk
-        sk = cma.compress(c["variant"], dAT, dB, d, b, seed=HASH_SEED, dtype=dtype,
-                          flags=flags, sketch=sk)
-        sk.decompress(dC)
-
-    for _ in range(args.warmup):
-        step()
-    cma.synchronize()
-    kname = "plain_transform" if args.plain else "compress"
-    cma.timing_enable(True, only=None if args.kernel_times else kname)
-    cma.timing_reset()
-    cma.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-    cma.synchronize()
-    elapsed = time.perf_counter() - t0
-    k_ms, k_n = cma.timing_get(kname)
-    totals = {k: cma.timing_get(k) for k in ("hash", "compress", "reduce", "combine",
-                                            "decompress", "plain_sketch", "plain_transform",
-                                            "plain_mac")}
-    cma.timing_enable(False)
-    print("[bench] per-kernel ms/launch: " + ", ".join(
-        f"{k}={v[0] / v[1]:.4f}" for k, v in totals.items() if v[1]), file=sys.stderr, flush=True)
-
-    # parity of the last product: recovered set vs the dense FP64 product
-    C_est = dC.numpy()
-    C_true = AT.astype(np.float64).T @ B.astype(np.float64)
-    truth = set(zip(*np.nonzero(C_true > c["tau"])))
-    ii, jj, _ = sk.recover(c["tau"])
-    got = set(zip(ii.tolist(), jj.tolist()))
-    est_set = set(zip(*np.nonzero(C_est > c["tau"])))
-    recall = len(got & truth) / max(1, len(truth))
-    precision = len(got & truth) / max(1, len(got))
-
-    ms_per_step = 1e3 * elapsed / args.steps
-    flops = compress_flops(c["variant"], m, n, p, d, b)
-    peak = PEAK_FP32_TFLOPS if dtype == np.float32 else PEAK_FP64_TFLOPS
-    k_avg_s = (k_ms / max(1, k_n)) * 1e-3
-    achieved = flops / k_avg_s / 1e12 if k_avg_s > 0 else 0.0
-    out = {
-        "metric": "products/s",
-        "value": args.steps / elapsed,
-        "unit": "products/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32" if dtype == np.float32 else "f64",
-        "data": f"synthetic: N(0,1/{n}) noise + {c['plants']} planted entries of "
-                f"{c['magnitude']} (seeded)",
-        "config": {"workload": f"{c['variant']} {m}x{n}x{p} {dtype.name}, b={b}, d={d}, "
-                               f"{c['plants']} plants, tau={c['tau']}",
-                   "m": m, "n": n, "p": p, "b": b, "d": d, "variant": c["variant"],
-                   "hash": "multiply add shift", "path": "plain" if args.plain else "fused",
-                   "parallelism": "single GPU"},
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak,
-                     "unit": "TFLOP/s", "frac": achieved / peak,
-                     "traffic": None if args.plain else traffic_from_profile(name),
-                     "kernel": kname,
-                     "kernel_ms": k_avg_s * 1e3, "flops_per_launch": flops},
-        "detail": {"recall": recall, "precision": precision, "recovered": len(got),
-                   "truth": len(truth), "C_threshold_set_equal": est_set == got,
-                   "kernel_ms": {k: (v[0] / v[1] if v[1] else None) for k, v in totals.items()}},
-    }
-    if not args.no_cpu_baseline and name == DEFAULT_WORKLOAD:
+    out, templates = bench_serve(args)
+    if not args.no_detail_legs:
+        # the fwht 2048^3 single-product workload, fused and plain, in the detail
+        import types
+        leg = types.SimpleNamespace(steps=100, warmup=10, kernel_times=False)
+        f = bench_single(leg, "fwht2048_fp32")
+        pl = bench_single(types.SimpleNamespace(steps=5, warmup=1, kernel_times=False),
+                          "fwht2048_fp32", plain=True)
+        out["detail"]["fwht2048_fp32"] = {
+            "products_per_s": f["value"], "roofline": f["roofline"], "recall": f["detail"]["recall"],
+            "plain_products_per_s": pl["value"], "plain_roofline": pl["roofline"],
+            "fused_vs_plain": f["value"] / pl["value"]}
+    if not args.no_cpu_baseline:
         print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
-        out["cpu_baseline"] = cpu_baseline(c, AT, B)
+        out["cpu_baseline"] = cpu_baseline_serving(templates)
+        out["detail"]["cpu_full_configs"] = cpu_full_configs(cpu_threads())
     print(json.dumps(out), flush=True)
 
 

@@ -140,15 +140,19 @@ hipEvent_t get_event() {
   return e;
 }
 
-// launch wrapper: brackets the launch with events when timing is on
+// launch wrapper: brackets the launch with events on `stream` (the stream
+// the launch runs on) when timing is on
 template <class F>
-int timed(const char* name, F&& launch) {
+int timed_on(hipStream_t stream, const char* name, F&& launch) {
   hipEvent_t a = nullptr, b = nullptr;
-  bool on = g.timing && (g.timing_only.empty() || g.timing_only == name);
+  const std::string& only = g.timing_only;
+  const bool prefix = !only.empty() && only.back() == '*';  // "compress*": every compress class
+  bool on = g.timing && (only.empty() || only == name ||
+                         (prefix && std::strncmp(name, only.c_str(), only.size() - 1) == 0));
   if (on) {
     a = get_event();
     b = get_event();
-    on = a && b && hipEventRecord(a, g.stream) == hipSuccess;
+    on = a && b && hipEventRecord(a, stream) == hipSuccess;
     if (!on) {
       if (a) g.event_pool.push_back(a);
       if (b) g.event_pool.push_back(b);
@@ -156,7 +160,7 @@ int timed(const char* name, F&& launch) {
   }
   const hipError_t e = launch();
   if (on) {
-    if (hipEventRecord(b, g.stream) == hipSuccess) {
+    if (hipEventRecord(b, stream) == hipSuccess) {
       g.pending.push_back({name, a, b});
     } else {
       g.event_pool.push_back(a);
@@ -167,11 +171,17 @@ int timed(const char* name, F&& launch) {
   return CM_OK;
 }
 
+template <class F>
+int timed(const char* name, F&& launch) {
+  return timed_on(g.stream, name, launch);
+}
+
+// each pair's own end event is waited for: pairs may sit on other streams
 void resolve_timing() {
   if (g.pending.empty()) return;
-  (void)hipStreamSynchronize(g.stream);
   for (auto& r : g.pending) {
     float ms = 0.f;
+    (void)hipEventSynchronize(r.b);
     (void)hipEventElapsedTime(&ms, r.a, r.b);
     auto& t = g.totals[r.name];
     t.first += ms;
@@ -868,6 +878,9 @@ int twiddles(int lb, int f64, const void** out) { return ::twiddles(lb, f64, out
 int fft_t2_table(const void** out) { return ::fft_t2_table(out); }
 int timed(const char* name, const std::function<hipError_t()>& launch) {
   return ::timed(name, launch);
+}
+int timed_on(hipStream_t stream, const char* name, const std::function<hipError_t()>& launch) {
+  return ::timed_on(stream, name, launch);
 }
 }  // namespace rt
 }  // namespace cm

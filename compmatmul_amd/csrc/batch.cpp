@@ -606,7 +606,7 @@ int enqueue(cm_batch* bt, bool untimed) {
       return e == hipSuccess ? CM_OK
                              : error(CM_EDEVICE, std::string(name) + " launch failed: " + hipGetErrorString(e));
     }
-    return cm::rt::timed(name, f);
+    return cm::rt::timed_on(bt->stream, name, f);
   };
   int rc;
   if ((rc = run("hash", [&] { return cm::launch_hash_batch(bt->hash, bt->hash_blocks, bt->n_hash_blocks, s); })))
@@ -734,7 +734,7 @@ int batch_plan_env(int count, const cm_product* ps, unsigned flags, int grain, c
   return build_plan(count, ps, flags & ~CM_BATCH_GRAPH, grain, env, out);
 }
 
-int batch_enqueue(cm_batch* bt) { return enqueue(bt, true); }
+int batch_enqueue(cm_batch* bt) { return enqueue(bt, false); }
 
 ProductView batch_product(const cm_batch* bt, int i) {
   const auto& P = bt->pp[i];

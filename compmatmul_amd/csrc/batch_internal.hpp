@@ -36,7 +36,7 @@ struct ProductView {
 
 int batch_plan_env(int count, const cm_product* ps, unsigned flags, int grain, const PlanEnv& env,
                    cm_batch** out);
-int batch_enqueue(cm_batch* bt);  // untimed, on the plan's stream
+int batch_enqueue(cm_batch* bt);  // on the plan's stream; timed there when timing is on
 ProductView batch_product(const cm_batch* bt, int i);
 void batch_destroy(cm_batch* bt);  // host state; the arena too when the plan allocated it
 

@@ -19,6 +19,8 @@ int twiddles(int lb, int f64, const void** out);
 int fft_t2_table(const void** out);
 // runs `launch`, bracketed by events when timing is on (not while capturing)
 int timed(const char* name, const std::function<hipError_t()>& launch);
+// the same, with the events on `stream`, the stream the launch runs on
+int timed_on(hipStream_t stream, const char* name, const std::function<hipError_t()>& launch);
 
 }  // namespace rt
 }  // namespace cm

@@ -180,11 +180,15 @@ int cm_synchronize(void);
 
 /* ---------------------------------------------------------------- timing
  * When enabled, the library brackets every kernel launch with HIP events on
- * its own stream; cm_timing_get returns the summed device time (ms) and the
- * launch count of the kernel class `name` ("compress", "reduce", "combine",
- * "decompress", "hash", "plain_sketch", "plain_transform", "plain_mac"). */
+ * the stream it runs on (its own stream, a batch's, or a server wave's);
+ * cm_timing_get returns the summed device time (ms) and the launch count of
+ * the kernel class `name` ("compress", "reduce", "combine", "decompress",
+ * "hash", "plain_sketch", "plain_transform", "plain_mac"; batches and server
+ * waves: "hash", "balance", "lists", "compress_fft", "compress_fwht",
+ * "finish", "decompress"). */
 int cm_timing_enable(int on);
-/* Restricts the timing to the kernel class `name` (NULL or "" restores every
+/* Restricts the timing to the kernel class `name`, or to every class that
+ * starts with its prefix when it ends in '*' (NULL or "" restores every
  * class).  Each event pair stalls the stream briefly, so a throughput
  * measurement times only its dominant class.  No reference counterpart: it
  * serves the measurement harness, as cm_timing_enable does. */
