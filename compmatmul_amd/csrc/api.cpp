@@ -642,9 +642,13 @@ int compress_device(cm_sketch* sk, const void* AT, int64_t ld_at, const void* B,
 
   const int lb2 = std::min(lb, max_lb2);
   const int b1 = 1 << (lb - lb2);
-  // aim at ~8 workgroups per CU (256 CUs) over all (t, slice, chunk) units
+  // whole rounds of the resident workgroups over all (t, slice, chunk)
+  // units (a partial last round costs a whole unit's time), about 16 inner
+  // indices per unit
   const int64_t per_chunk = int64_t(d) * b1;
-  int64_t nchunks = std::max<int64_t>(1, (2048 + per_chunk - 1) / per_chunk);
+  const int64_t slots = cm::compress_lds_slots(fwht, f64, lb2);
+  const int64_t rounds = std::max<int64_t>(1, (int64_t(n) * per_chunk + slots * 16 - 1) / (slots * 16));
+  int64_t nchunks = std::max<int64_t>(1, slots * rounds / per_chunk);
   nchunks = std::min<int64_t>(nchunks, std::max(1, n));
   const size_t chunk_bytes = size_t(d) * b * cplx * esz;
   // partial spectra under 1 GiB, and within what a carved region has left

@@ -29,6 +29,7 @@
 //   (reduce kernel), replacing the reference's per-t omp locks.
 #include "device_common.hpp"
 #include "kernels.hpp"
+#include "lds_transform.hpp"
 
 namespace cm {
 
@@ -95,114 +96,6 @@ __device__ void lds_dif(cpx<T>* z, int b2, int lb, const cpx<T>* __restrict__ tw
       z[j + len] = CONJ ? cmulc(csub(x0, x1), w) : cmul(csub(x0, x1), w);
     }
     __syncthreads();
-  }
-}
-
-template <typename T, bool FWHT>
-__global__ void __launch_bounds__(NT) k_compress_lds(CompressArgs a) {
-  constexpr int MAXR = LdsCfg<T>::MAXR;
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  const int b2 = 1 << a.lb2;
-  const int lbs = a.lb - a.lb2;
-  const int b1 = 1 << lbs;
-  int g = blockIdx.x;
-  const int slice = g & (b1 - 1);
-  g >>= lbs;
-  const int t = g % a.d;
-  const int chunk = g / a.d;
-  const int k0 = chunk * a.kper;
-  const int k1 = min(a.n, k0 + a.kper);
-  const int tid = threadIdx.x;
-  const T* __restrict__ AT = static_cast<const T*>(a.AT);
-  const T* __restrict__ B = static_cast<const T*>(a.B);
-  const uint32_t* __restrict__ hA = a.hA + int64_t(t) * a.m;
-  const uint32_t* __restrict__ hB = a.hB + int64_t(t) * a.p;
-  const int64_t out_off = ((int64_t(chunk) * a.d + t) * b1 + slice) * b2;
-
-  if constexpr (FWHT) {
-    T* sa = reinterpret_cast<T*>(smem_raw);
-    T* sb = sa + b2;
-    T acc[MAXR];
-#pragma unroll
-    for (int r = 0; r < MAXR; ++r) acc[r] = T(0);
-    for (int k = k0; k < k1; ++k) {
-      for (int e = tid; e < b2; e += NT) {
-        sa[e] = T(0);
-        sb[e] = T(0);
-      }
-      __syncthreads();
-      const T* __restrict__ ar = AT + int64_t(k) * a.ld_at;
-      const T* __restrict__ br = B + int64_t(k) * a.ld_b;
-      for (int i = tid; i < a.m; i += NT) {
-        const uint32_t e = hA[i], h = hbucket(e);
-        const uint32_t neg = hneg(e) ^ (__popc((h >> a.lb2) & slice) & 1);
-        atomicAdd(&sa[h & (b2 - 1)], apply_sign(ar[i], neg));
-      }
-      for (int j = tid; j < a.p; j += NT) {
-        const uint32_t e = hB[j], h = hbucket(e);
-        const uint32_t neg = hneg(e) ^ (__popc((h >> a.lb2) & slice) & 1);
-        atomicAdd(&sb[h & (b2 - 1)], apply_sign(br[j], neg));
-      }
-      __syncthreads();
-      lds_wht2(sa, sb, b2, tid);
-#pragma unroll
-      for (int r = 0; r < MAXR; ++r) {
-        const int e = tid + r * NT;
-        if (e < b2) acc[r] += sa[e] * sb[e];
-      }
-      __syncthreads();
-    }
-    T* out = static_cast<T*>(a.partial) + out_off;
-#pragma unroll
-    for (int r = 0; r < MAXR; ++r) {
-      const int e = tid + r * NT;
-      if (e < b2) out[e] = acc[r];
-    }
-  } else {
-    cpx<T>* z = reinterpret_cast<cpx<T>*>(smem_raw);
-    const cpx<T>* __restrict__ tw = static_cast<const cpx<T>*>(a.tw);
-    const uint64_t bmask = (uint64_t(1) << a.lb) - 1;
-    cpx<T> acc[MAXR];
-#pragma unroll
-    for (int r = 0; r < MAXR; ++r) acc[r] = {T(0), T(0)};
-    const cpx<T>* __restrict__ bal = static_cast<const cpx<T>*>(a.scale);
-    for (int k = k0; k < k1; ++k) {
-      for (int e = tid; e < b2; e += NT) z[e] = {T(0), T(0)};
-      __syncthreads();
-      const T* __restrict__ ar = AT + int64_t(k) * a.ld_at;
-      const T* __restrict__ br = B + int64_t(k) * a.ld_b;
-      const cpx<T> bk = bal[k];  // power-of-two balance of row k (balance.hip)
-      for (int i = tid; i < a.m; i += NT) {
-        const uint32_t e = hA[i], h = hbucket(e);
-        const T v = apply_sign(ar[i], hneg(e)) * bk.x;
-        const cpx<T> w = tw[(uint64_t(h) * uint64_t(slice)) & bmask];
-        cpx<T>* dst = &z[h & (b2 - 1)];
-        atomicAdd(&dst->x, v * w.x);
-        atomicAdd(&dst->y, v * w.y);
-      }
-      for (int j = tid; j < a.p; j += NT) {
-        const uint32_t e = hB[j], h = hbucket(e);
-        const T v = apply_sign(br[j], hneg(e)) * bk.y;
-        const cpx<T> w = tw[(uint64_t(h) * uint64_t(slice)) & bmask];
-        cpx<T>* dst = &z[h & (b2 - 1)];
-        atomicAdd(&dst->x, -v * w.y);  // i * v * w
-        atomicAdd(&dst->y, v * w.x);
-      }
-      __syncthreads();
-      lds_dif<T, false>(z, b2, a.lb, tw, tid);
-#pragma unroll
-      for (int r = 0; r < MAXR; ++r) {
-        const int e = tid + r * NT;
-        if (e < b2) acc[r] = cadd(acc[r], csqr(z[e]));
-      }
-      __syncthreads();
-    }
-    cpx<T>* out = static_cast<cpx<T>*>(a.partial) + out_off;
-#pragma unroll
-    for (int r = 0; r < MAXR; ++r) {
-      const int e = tid + r * NT;
-      if (e < b2) out[bitrev(e, a.lb2)] = acc[r];
-    }
   }
 }
 
@@ -445,13 +338,193 @@ __global__ void __launch_bounds__(NT) k_combine_batch(const FinishJob* __restric
     combine_body<float, false>(c, idx);
 }
 
+// General fused compress (FP64, and FP32 sketches below the register
+// kernels' range): one workgroup per (chunk of k, t, slice) keeps its slice
+// of the spectral accumulator in registers for its chunk of k.  Each slice
+// transform runs as radix-4/8/16 register passes over a padded LDS tile
+// (lds_transform.hpp: a barrier per pass, twiddles from an LDS table).  The scatter adds
+// with LDS atomics; the accumulator stays in registers, at the transform's
+// output positions, and is written in natural frequency order.
+// 256 threads at two waves per SIMD: 256 VGPRs a lane hold the accumulator
+// (b2 / 256 values) beside radix-16 passes, so a 4096-point FP64 slice takes
+// three LDS passes; two workgroups (LDS tiles) share a CU
+constexpr int LT = 256;
+constexpr int KU = 8;  // scatter keys per thread with loads in flight together
+template <typename T>
+constexpr int fft_rm() { return 4; }
+
+template <typename T, bool FWHT>
+__host__ __device__ __forceinline__ size_t lds2_bytes(int lb2) {
+  return FWHT ? ldsx::tile_bytes<T, true>(lb2 + 1) : ldsx::transform_bytes<T, false>(lb2);
+}
+
+template <typename T, bool FWHT>
+__global__ void __launch_bounds__(LT) __attribute__((amdgpu_waves_per_eu(2, 2))) k_compress_lds2(CompressArgs a) {
+  using namespace ldsx;
+  using E = Elem<T, FWHT>;
+  constexpr int PER = (1 << LdsCfg<T>::LB2MAX) / LT;  // accumulator slots per thread
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int lb2 = a.lb2, b2 = 1 << lb2;
+  const int lbs = a.lb - lb2;
+  const int b1 = 1 << lbs;
+  int g = blockIdx.x;
+  const int slice = g & (b1 - 1);
+  g >>= lbs;
+  const int t = g % a.d;
+  const int chunk = g / a.d;
+  const int k0 = chunk * a.kper;
+  const int k1 = min(a.n, k0 + a.kper);
+  const int tid = threadIdx.x;
+  const T* __restrict__ AT = static_cast<const T*>(a.AT);
+  const T* __restrict__ B = static_cast<const T*>(a.B);
+  const uint32_t* __restrict__ hA = a.hA + int64_t(t) * a.m;
+  const uint32_t* __restrict__ hB = a.hB + int64_t(t) * a.p;
+  const int64_t out_off = ((int64_t(chunk) * a.d + t) * b1 + slice) * b2;
+  E acc[PER];
+#pragma unroll
+  for (int r = 0; r < PER; ++r) acc[r] = E{};
+  if constexpr (FWHT) {
+    // A's sketch at elements [0, b2), B's at [b2, 2 b2), one padded tile
+    T* sa = reinterpret_cast<T*>(smem_raw);
+    const int wwords = int(tile_bytes<T, true>(lb2 + 1) / 16);
+    for (int k = k0; k < k1; ++k) {
+      __syncthreads();  // the previous k's products are read
+      for (int w = tid; w < wwords; w += LT) reinterpret_cast<float4*>(smem_raw)[w] = make_float4(0.f, 0.f, 0.f, 0.f);
+      __syncthreads();
+      const T* __restrict__ ar = AT + int64_t(k) * a.ld_at;
+      const T* __restrict__ br = B + int64_t(k) * a.ld_b;
+      // keys [0, m) of A then [0, p) of B, KU per thread with all their
+      // loads in flight before the LDS adds
+      for (int base = 0; base < a.m + a.p; base += LT * KU) {
+        uint32_t e[KU];
+        T v[KU];
+#pragma unroll
+        for (int u = 0; u < KU; ++u) {
+          const int key = base + u * LT + tid;
+          e[u] = 0u;
+          v[u] = T(0);
+          if (key < a.m) {
+            e[u] = hA[key];
+            v[u] = ar[key];
+          } else if (key < a.m + a.p) {
+            e[u] = hB[key - a.m];
+            v[u] = br[key - a.m];
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < KU; ++u) {
+          const int key = base + u * LT + tid;
+          if (key < a.m + a.p) {
+            const uint32_t h = hbucket(e[u]);
+            const uint32_t neg = hneg(e[u]) ^ (__popc((h >> lb2) & slice) & 1);
+            atomicAdd(&sa[pad((key < a.m ? 0 : b2) + int(h & (b2 - 1)))], apply_sign(v[u], neg));
+          }
+        }
+      }
+      lds_transform<LT, T, true>(sa, lb2, 2, b2, nullptr, nullptr);  // both sketches
+#pragma unroll
+      for (int r = 0; r < PER; ++r) {
+        const int e = tid + r * LT;
+        if (e < b2) acc[r] += sa[pad(e)] * sa[pad(b2 + e)];
+      }
+    }
+    T* out = static_cast<T*>(a.partial) + out_off;
+#pragma unroll
+    for (int r = 0; r < PER; ++r) {
+      const int e = tid + r * LT;
+      if (e < b2) out[e] = acc[r];
+    }
+  } else {
+    constexpr int FRM = fft_rm<T>();
+    cpx<T>* z = reinterpret_cast<cpx<T>*>(smem_raw);
+    cpx<T>* lo = twiddle_lds<T, false>(smem_raw, lb2);
+    cpx<T>* hi = lo + TW_LO;
+    const cpx<T>* __restrict__ tw = static_cast<const cpx<T>*>(a.tw);
+    fill_twiddles<LT, T>(lo, hi, lb2, tw, a.lb);
+    const uint64_t bmask = (uint64_t(1) << a.lb) - 1;
+    const cpx<T>* __restrict__ bal = static_cast<const cpx<T>*>(a.scale);
+    for (int k = k0; k < k1; ++k) {
+      __syncthreads();  // the previous k's squares are read
+      for (int w = tid; w < int(tile_bytes<T, false>(lb2) / 16); w += LT)
+        reinterpret_cast<float4*>(smem_raw)[w] = make_float4(0.f, 0.f, 0.f, 0.f);
+      __syncthreads();
+      const T* __restrict__ ar = AT + int64_t(k) * a.ld_at;
+      const T* __restrict__ br = B + int64_t(k) * a.ld_b;
+      const cpx<T> bk = bal[k];  // power-of-two balance of row k (balance.hip)
+      // keys [0, m) of A then [0, p) of B, KU per thread with all their
+      // loads in flight before the LDS adds
+      for (int base = 0; base < a.m + a.p; base += LT * KU) {
+        uint32_t e[KU];
+        T v[KU];
+#pragma unroll
+        for (int u = 0; u < KU; ++u) {
+          const int key = base + u * LT + tid;
+          e[u] = 0u;
+          v[u] = T(0);
+          if (key < a.m) {
+            e[u] = hA[key];
+            v[u] = ar[key] * bk.x;
+          } else if (key < a.m + a.p) {
+            e[u] = hB[key - a.m];
+            v[u] = br[key - a.m] * bk.y;
+          }
+        }
+        if (slice == 0) {  // modulation W_b^0 = 1: A adds to Re, B to Im only
+#pragma unroll
+          for (int u = 0; u < KU; ++u) {
+            const int key = base + u * LT + tid;
+            if (key < a.m + a.p) {
+              cpx<T>* dst = &z[pad(hbucket(e[u]) & (b2 - 1))];
+              atomicAdd(key < a.m ? &dst->x : &dst->y, apply_sign(v[u], hneg(e[u])));
+            }
+          }
+        } else {
+          cpx<T> w[KU];
+#pragma unroll
+          for (int u = 0; u < KU; ++u) w[u] = tw[(uint64_t(hbucket(e[u])) * uint64_t(slice)) & bmask];
+#pragma unroll
+          for (int u = 0; u < KU; ++u) {
+            const int key = base + u * LT + tid;
+            if (key < a.m + a.p) {
+              const T x = apply_sign(v[u], hneg(e[u]));
+              cpx<T>* dst = &z[pad(hbucket(e[u]) & (b2 - 1))];
+              if (key < a.m) {
+                atomicAdd(&dst->x, x * w[u].x);
+                atomicAdd(&dst->y, x * w[u].y);
+              } else {  // i x w
+                atomicAdd(&dst->x, -x * w[u].y);
+                atomicAdd(&dst->y, x * w[u].x);
+              }
+            }
+          }
+        }
+      }
+      lds_transform<LT, T, false, FRM>(z, lb2, 1, 0, lo, hi);
+#pragma unroll
+      for (int r = 0; r < PER; ++r) {
+        const int e = tid + r * LT;
+        if (e < b2) acc[r] = cadd(acc[r], csqr(z[pad(e)]));
+      }
+    }
+    cpx<T>* out = static_cast<cpx<T>*>(a.partial) + out_off;
+#pragma unroll
+    for (int r = 0; r < PER; ++r) {
+      const int e = tid + r * LT;
+      if (e < b2) out[dif_nat(e, lb2, FRM)] = acc[r];
+    }
+  }
+}
+
 template <typename T, bool FWHT>
 hipError_t compress_lds_t(const CompressArgs& a, hipStream_t s) {
   const int b1 = 1 << (a.lb - a.lb2);
   const int64_t grid = int64_t(a.nchunks) * a.d * b1;
-  const size_t lds = FWHT ? 2 * sizeof(T) * (size_t(1) << a.lb2)
-                          : 2 * sizeof(T) * (size_t(1) << a.lb2);
-  k_compress_lds<T, FWHT><<<dim3(unsigned(grid)), dim3(NT), lds, s>>>(a);
+  const size_t lds = lds2_bytes<T, FWHT>(a.lb2);
+  if (lds > 65536 &&
+      hipFuncSetAttribute(reinterpret_cast<const void*>(&k_compress_lds2<T, FWHT>),
+                          hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)) != hipSuccess)
+    return hipErrorInvalidValue;
+  k_compress_lds2<T, FWHT><<<dim3(unsigned(grid)), dim3(LT), lds, s>>>(a);
   return hipGetLastError();
 }
 
@@ -487,6 +560,35 @@ hipError_t combine_t(const CombineArgs& a, hipStream_t s) {
 }
 
 }  // namespace
+
+namespace {
+template <typename T, bool FWHT>
+int lds2_slots(int lb2) {
+  int dev = 0, cus = 0, per_cu = 0;
+  const size_t lds = lds2_bytes<T, FWHT>(lb2);
+  if (lds > 65536)
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_compress_lds2<T, FWHT>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_compress_lds2<T, FWHT>, LT, lds) != hipSuccess ||
+      cus <= 0 || per_cu <= 0)
+    return 256;
+  return cus * per_cu;
+}
+}  // namespace
+
+int compress_lds_slots(int fwht, int f64, int lb2) {
+  static int cache[2][2][32] = {};  // one occupancy query per configuration
+  int& c = cache[fwht != 0][f64 != 0][lb2 & 31];
+  if (!c) {
+    if (f64)
+      c = fwht ? lds2_slots<double, true>(lb2) : lds2_slots<double, false>(lb2);
+    else
+      c = fwht ? lds2_slots<float, true>(lb2) : lds2_slots<float, false>(lb2);
+  }
+  return c;
+}
 
 int compress_lds_max_lb2(int fwht, int f64) {
   (void)fwht;

@@ -70,6 +70,7 @@ struct CompressArgs {
   void* partial;       // [nchunks][d][b1][b2] (x2 for complex)
 };
 int compress_lds_max_lb2(int fwht, int f64);
+int compress_lds_slots(int fwht, int f64, int lb2);  // resident workgroups of the general kernel
 hipError_t launch_compress_lds(const CompressArgs& a, hipStream_t s);
 
 // ------------------------------------------------- register-resident compress

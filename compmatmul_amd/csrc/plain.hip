@@ -27,6 +27,7 @@
 
 #include "device_common.hpp"
 #include "kernels.hpp"
+#include "lds_transform.hpp"
 
 namespace cm {
 
@@ -36,8 +37,7 @@ constexpr int NT = 1024;       // threads of the transform kernels
 constexpr int ST = 256;        // threads of the sketch kernel
 constexpr int MT = 256;        // threads of the MAC kernels
 
-template <typename T, bool FWHT>
-using Elem = std::conditional_t<FWHT, T, cpx<T>>;
+using namespace ldsx;
 
 // log2 of the elements of one 128 KB transform tile (one workgroup per CU)
 template <typename T, bool FWHT>
@@ -49,198 +49,6 @@ constexpr int lmax_log() {
 template <typename T, bool FWHT>
 constexpr int range_log() {
   return sizeof(Elem<T, FWHT>) == 4 ? 12 : sizeof(Elem<T, FWHT>) == 8 ? 11 : 10;
-}
-
-// one padding slot every 16 elements keeps strided passes off one bank
-__device__ __forceinline__ int pad(int i) { return i + (i >> 4); }
-
-
-
-// cos(2 pi e / 16)
-__device__ __forceinline__ constexpr double cos16(int e) {
-  constexpr double c[16] = {1.0, 0.92387953251128674, 0.70710678118654752, 0.38268343236508978,
-                            0.0, -0.38268343236508978, -0.70710678118654752, -0.92387953251128674,
-                            -1.0, -0.92387953251128674, -0.70710678118654752, -0.38268343236508978,
-                            0.0, 0.38268343236508978, 0.70710678118654752, 0.92387953251128674};
-  return c[e & 15];
-}
-
-template <int R>
-constexpr int ilog2c() { return R <= 1 ? 0 : 1 + ilog2c<R / 2>(); }
-
-// R-point DFT (W = exp(-2 pi i / R)), natural order in and out: radix-2
-// decimation in frequency in registers, then the bit-reversal undone
-// reverse the low r bits of k (compile-time)
-__device__ __forceinline__ constexpr int brev_c(int k, int r) {
-  int br = 0;
-  for (int bit = 0; bit < r; ++bit) br |= ((k >> bit) & 1) << (r - 1 - bit);
-  return br;
-}
-
-// Every loop below has a constant trip count, so it unrolls completely and
-// the arrays stay in registers.
-template <int R, typename T>
-__device__ __forceinline__ void dft_reg(cpx<T> (&v)[R]) {
-  constexpr int r = ilog2c<R>();
-#pragma unroll
-  for (int st = 0; st < r; ++st) {
-    const int len = R >> (st + 1);
-#pragma unroll
-    for (int q = 0; q < R / 2; ++q) {  // butterfly q of the stage
-      const int pos = q % len, j = (q / len) * 2 * len + pos;
-      const cpx<T> x = v[j], y = v[j + len];
-      v[j] = cadd(x, y);
-      const cpx<T> dd = csub(x, y);
-      const int e = pos * 8 / len;  // W_R^(pos R / 2len) = W_16^e
-      if (e == 0) {
-        v[j + len] = dd;
-      } else if (e == 4) {  // -i
-        v[j + len] = cpx<T>{dd.y, -dd.x};
-      } else {
-        const T c = T(cos16(e)), sn = T(cos16(e + 12));  // sin = cos(angle - pi/2)
-        v[j + len] = cpx<T>{dd.x * c + dd.y * sn, dd.y * c - dd.x * sn};
-      }
-    }
-  }
-  cpx<T> o[R];
-#pragma unroll
-  for (int k = 0; k < R; ++k) o[k] = v[brev_c(k, r)];
-#pragma unroll
-  for (int k = 0; k < R; ++k) v[k] = o[k];
-}
-
-// R-point natural-order Walsh-Hadamard transform in registers
-template <int R, typename T>
-__device__ __forceinline__ void wht_reg(T (&v)[R]) {
-  constexpr int r = ilog2c<R>();
-#pragma unroll
-  for (int st = 0; st < r; ++st) {
-    const int len = R >> (st + 1);
-#pragma unroll
-    for (int q = 0; q < R / 2; ++q) {
-      const int j = (q / len) * 2 * len + q % len;
-      const T x = v[j], y = v[j + len];
-      v[j] = x + y;
-      v[j + len] = x - y;
-    }
-  }
-}
-
-// Twiddles of a length-2^l transform in LDS: W_L^x = hi[x >> 6] lo[x & 63]
-// (lo[i] = W_L^i, hi[i] = W_L^(64 i)), filled from the length-2^lb table.
-constexpr int TW_LO = 64;
-template <typename T>
-__device__ __forceinline__ void fill_twiddles(cpx<T>* lo, cpx<T>* hi, int l, const cpx<T>* __restrict__ tw,
-                                              int lb) {
-  const int L = 1 << l;
-  for (int i = threadIdx.x; i < TW_LO + (L >> 6) + 1; i += NT) {
-    if (i < TW_LO)
-      lo[i] = i < L ? tw[int64_t(i) << (lb - l)] : cpx<T>{T(1), T(0)};
-    else
-      hi[i - TW_LO] = tw[(int64_t(i - TW_LO) << 6 << (lb - l)) & ((int64_t(1) << lb) - 1)];
-  }
-}
-template <typename T>
-__device__ __forceinline__ cpx<T> twiddle(const cpx<T>* lo, const cpx<T>* hi, int x) {
-  return x < TW_LO ? lo[x] : cmul(hi[x >> 6], lo[x & (TW_LO - 1)]);
-}
-
-// One radix-R decimation-in-frequency pass at stride S = 2^s over `count`
-// sequences of length 2^l held in LDS (sequence c from element c * cs): the
-// R elements base + j S of each group are transformed, and for the FFT
-// output k is multiplied by W_(R S)^(pos k) = W_L^(pos k L / (R S)), from the
-// LDS twiddle tables (lo, hi).
-template <typename T, bool FWHT, int R>
-__device__ __forceinline__ void lds_pass(Elem<T, FWHT>* z, int l, int s, int count, int cs,
-                                         const cpx<T>* lo, const cpx<T>* hi) {
-  constexpr int r = ilog2c<R>();
-  const int S = 1 << s;
-  const int gl = l - r;  // groups per sequence, log2
-  const int total = count << gl;
-  for (int g = threadIdx.x; g < total; g += NT) {
-    const int c = g >> gl;
-    const int gg = g & ((1 << gl) - 1);
-    const int pos = gg & (S - 1);
-    const int base = c * cs + ((gg >> s) << (s + r)) + pos;
-    Elem<T, FWHT> v[R];
-#pragma unroll
-    for (int j = 0; j < R; ++j) v[j] = z[pad(base + j * S)];
-    if constexpr (FWHT) {
-      wht_reg<R>(v);
-    } else {
-      dft_reg<R>(v);
-      if (pos != 0) {
-        const int sh = l - s - r;
-#pragma unroll
-        for (int k = 1; k < R; ++k) v[k] = cmul(v[k], twiddle(lo, hi, (pos * k) << sh));
-      }
-    }
-#pragma unroll
-    for (int j = 0; j < R; ++j) z[pad(base + j * S)] = v[j];
-  }
-}
-
-// Transform `count` LDS sequences of length 2^l: radix-16 passes from the
-// high bits down, the last pass over the l mod 4 lowest bits (if any).  FWHT
-// output is in natural order; FFT output k sits at dif_pos(k, l, rmax()).
-// widest pass, log2: radix 16, or 8 for the FP64 FFT (whose radix-16 pass
-// would spill at 1024 threads)
-template <typename T, bool FWHT>
-constexpr int rmax() { return !FWHT && sizeof(T) == 8 ? 3 : 4; }
-
-template <typename T, bool FWHT>
-__device__ __forceinline__ void lds_transform(Elem<T, FWHT>* z, int l, int count, int cs, const cpx<T>* lo,
-                              const cpx<T>* hi) {
-  int s = l;
-  while (s > 0) {
-    const int r = min(rmax<T, FWHT>(), s);
-    s -= r;
-    __syncthreads();
-    switch (r) {
-      case 4:
-        if constexpr (rmax<T, FWHT>() >= 4) lds_pass<T, FWHT, 16>(z, l, s, count, cs, lo, hi);
-        break;
-      case 3: lds_pass<T, FWHT, 8>(z, l, s, count, cs, lo, hi); break;
-      case 2: lds_pass<T, FWHT, 4>(z, l, s, count, cs, lo, hi); break;
-      default: lds_pass<T, FWHT, 2>(z, l, s, count, cs, lo, hi); break;
-    }
-  }
-  __syncthreads();
-}
-
-// LDS of a tile of 2^l elements (padded), rounded to 16 bytes, then (FFT)
-// the twiddle tables lo (64 entries) and hi (2^l / 64 + 1)
-template <typename T, bool FWHT>
-__host__ __device__ __forceinline__ size_t tile_bytes(int l) {
-  return ((size_t((1 << l) + (1 << l) / 16) * sizeof(Elem<T, FWHT>)) + 15) & ~size_t(15);
-}
-template <typename T, bool FWHT>
-__host__ __device__ __forceinline__ size_t transform_bytes(int l) {
-  return tile_bytes<T, FWHT>(l) + (FWHT ? 0 : (64 + (1 << l) / 64 + 1) * sizeof(cpx<T>));
-}
-template <typename T, bool FWHT>
-__device__ __forceinline__ cpx<T>* twiddle_lds(unsigned char* smem, int l) {
-  return reinterpret_cast<cpx<T>*>(smem + tile_bytes<T, FWHT>(l));
-}
-
-// 16-byte vectors of elements for the row loads and stores
-template <typename E>
-struct alignas(16) Vec16 {
-  E e[16 / sizeof(E)];
-};
-
-// Position of DFT output k after lds_transform with passes of rm bits: the
-// radix digits of k (rm bits each from the low end, the last one l mod rm
-// bits) in reverse order.
-__device__ __forceinline__ int dif_pos(int k, int l, int rm) {
-  int p = 0, off = l;
-  while (off > 0) {
-    const int r = min(rm, off);
-    off -= r;
-    p |= (k & ((1 << r) - 1)) << off;
-    k >>= r;
-  }
-  return p;
 }
 
 template <typename T, bool FWHT>
@@ -319,7 +127,7 @@ __global__ void __launch_bounds__(NT) k_plain_rows(PlainArgs a, int l) {
   const int L = 1 << l;
   cpx<T>* lo = twiddle_lds<T, FWHT>(smem_raw, l);
   cpx<T>* hi = lo + TW_LO;
-  if (!FWHT) fill_twiddles<T>(lo, hi, l, static_cast<const cpx<T>*>(a.tw), a.lb);
+  if (!FWHT) fill_twiddles<NT, T>(lo, hi, l, static_cast<const cpx<T>*>(a.tw), a.lb);
   constexpr int V = 16 / int(sizeof(E));
   constexpr int U = 4;  // vector loads in flight per thread
   if (L >= V) {  // 16-byte vectors; the V elements of one share a padded run
@@ -339,7 +147,7 @@ __global__ void __launch_bounds__(NT) k_plain_rows(PlainArgs a, int l) {
   } else {
     for (int e = threadIdx.x; e < L; e += NT) z[pad(e)] = g[e];
   }
-  lds_transform<T, FWHT>(z, l, 1, 0, lo, hi);
+  lds_transform<NT, T, FWHT>(z, l, 1, 0, lo, hi);
   if (L >= V) {
     const int nv = L / V;
     Vec16<E>* gv = reinterpret_cast<Vec16<E>*>(g);
@@ -398,8 +206,8 @@ __global__ void __launch_bounds__(NT) k_plain_cols(PlainArgs a, int l1, int lc) 
   const cpx<T>* tw = static_cast<const cpx<T>*>(a.tw);
   cpx<T>* lo = twiddle_lds<T, FWHT>(smem_raw, l1 + lc);
   cpx<T>* hi = lo + TW_LO;
-  if (!FWHT) fill_twiddles<T>(lo, hi, l1, tw, a.lb);
-  lds_transform<T, FWHT>(z, l1, C, 1 << l1, lo, hi);
+  if (!FWHT) fill_twiddles<NT, T>(lo, hi, l1, tw, a.lb);
+  lds_transform<NT, T, FWHT>(z, l1, C, 1 << l1, lo, hi);
   for (int e = threadIdx.x; e < n; e += NT) {
     const int k1 = e >> lc, cc = e & (C - 1);
     if constexpr (FWHT) {
