@@ -164,9 +164,10 @@ def bench_batch(args):
 
 
 def bench_precision(args):
-    """The precision form (FFT, 4096 x 16384 x 4096, b = 2^16, d = 11) in three
+    """The precision form (FFT, 4096 x 16384 x 4096, b = 2^16, d = 11) in its
     arithmetic modes: FP32 butterflies, FP64 cross-chunk spectral
-    accumulation, BF16 matrix-core DFT stages with hi/lo split correction.
+    accumulation, and matrix-core DFT stages with hi/lo split correction on
+    BF16 or (column-scaled) FP16 operands.
     Sketch errors against the device FP64 path with the same hashes (itself
     within 1e-10 of the oracle in the tests); estimate errors against the
     dense FP64 product; recovered set against {C > tau}."""
@@ -183,7 +184,8 @@ def bench_precision(args):
     C_ref = ref.decompress()
     ref.free()
     dAT, dB = cma.DeviceArray.from_numpy(AT), cma.DeviceArray.from_numpy(B)
-    modes = {"fp32": 0, "fp64_cross_chunk": cma.FLAG_F64_REDUCE, "bf16_mfma_split": cma.FLAG_BF16_MFMA}
+    modes = {"fp32": 0, "fp64_cross_chunk": cma.FLAG_F64_REDUCE, "bf16_mfma_split": cma.FLAG_BF16_MFMA,
+             "fp16_mfma_split": cma.FLAG_FP16_MFMA}
     out = {}
     for mode, flags in modes.items():
         sk = None

@@ -24,7 +24,7 @@ LIB_PATH = os.environ.get("COMPMATMUL_AMD_LIB") or os.path.join(HERE, "libcompma
 FFT, FWHT = 0, 1
 F64, F32 = 0, 1
 HOST, DEVICE = 0, 1
-FLAG_F64_REDUCE, FLAG_PLAIN, FLAG_LDS_KERNEL, FLAG_BF16_MFMA, FLAG_SINGLE_WAVE = 1, 2, 4, 8, 16
+FLAG_F64_REDUCE, FLAG_PLAIN, FLAG_LDS_KERNEL, FLAG_BF16_MFMA, FLAG_SINGLE_WAVE, FLAG_FP16_MFMA = 1, 2, 4, 8, 16, 32
 OK, EINVAL, ENOMEM, EDEVICE, ENOTSUP, EHASH = 0, -1, -2, -3, -4, -5
 FAMILIES = {None: 0, "multiply add shift": 0, "multiply shift": 1,
             "tabulation8": 2, "tabulation16": 3}

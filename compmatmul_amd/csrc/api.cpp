@@ -439,8 +439,8 @@ int compress_device(cm_sketch* sk, const void* AT, int64_t ld_at, const void* B,
     if (rc) return rc;
     if ((rc = ws.take(g.scale, std::max<size_t>(1, size_t(n) * 2 * esz), &psa.out))) return rc;
   }
-  if (ws.carved() && (flags & (CM_FLAG_PLAIN | CM_FLAG_BF16_MFMA)))
-    return fail(CM_ENOTSUP, "the plain and BF16 paths allocate their own buffers");
+  if (ws.carved() && (flags & (CM_FLAG_PLAIN | CM_FLAG_BF16_MFMA | CM_FLAG_FP16_MFMA)))
+    return fail(CM_ENOTSUP, "the plain and matrix-core paths allocate their own buffers");
   if (flags & CM_FLAG_PLAIN) {
     if (lb > cm::plain_max_lb(fwht, f64))
       return fail(CM_ENOTSUP, "b too large for the plain path (2^" +
@@ -518,9 +518,11 @@ int compress_device(cm_sketch* sk, const void* AT, int64_t ld_at, const void* B,
     return timed("combine", [&] { return cm::launch_combine(ca, g.stream); });
   }
 
-  if (flags & CM_FLAG_BF16_MFMA) {
+  if (flags & (CM_FLAG_BF16_MFMA | CM_FLAG_FP16_MFMA)) {
+    if ((flags & CM_FLAG_BF16_MFMA) && (flags & CM_FLAG_FP16_MFMA))
+      return fail(CM_EINVAL, "CM_FLAG_BF16_MFMA and CM_FLAG_FP16_MFMA exclude each other");
     if (fwht || f64 || !cm::fft_bf16_supported(lb))
-      return fail(CM_ENOTSUP, "the BF16 matrix-core mode takes the FFT variant, float32 and "
+      return fail(CM_ENOTSUP, "the matrix-core modes take the FFT variant, float32 and "
                               "b = 16^k with 2^12 <= b <= 2^28");
     // two row buffers of at most 512 MiB each
     const int rows = int(std::max<int64_t>(1, std::min<int64_t>(std::max(n, 1), (int64_t(1) << 29) / (b * 8))));
@@ -553,6 +555,7 @@ int compress_device(cm_sketch* sk, const void* AT, int64_t ld_at, const void* B,
     fa.Z0 = static_cast<float2*>(g.plainA.p);
     fa.Z1 = static_cast<float2*>(g.plainB.p);
     fa.partial = static_cast<float*>(g.partial.p);
+    fa.fp16 = (flags & CM_FLAG_FP16_MFMA) ? 1 : 0;
     if ((rc = timed("compress", [&] {
            const hipError_t e = with_scales();
            return e != hipSuccess ? e : cm::launch_fft_bf16_compress(fa, g.stream);

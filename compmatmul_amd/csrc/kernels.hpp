@@ -150,8 +150,8 @@ hipError_t launch_fft_reg_batch(const RegJob* jobs, const int2* items, const int
 inline int reg_lists_count(int fwht, int d, int lb) { return (fwht ? 2 : 1) * (d << reg_lbs(lb)); }
 hipError_t launch_zero_u32(unsigned* p, int n, hipStream_t s);
 
-// BF16 matrix-core FFT compress (fft_mfma.hip): radix-16 Stockham stages
-// as hi/lo-split BF16 MFMA products, b = 16^S with 2^12 <= b <= 2^28.
+// Matrix-core FFT compress (fft_mfma.hip): radix-16 Stockham stages as
+// hi/lo-split BF16 or FP16 MFMA products, b = 16^S with 2^12 <= b <= 2^28.
 // Writes the per-chunk spectra in the sliced layout the reduce kernel reads
 // (slices of 2^kRegLb2).
 struct Bf16FftArgs {
@@ -164,6 +164,7 @@ struct Bf16FftArgs {
   const float2* scale;     // per k, (scale of row k of AT, of row k of B) (balance.hip)
   float2* Z0; float2* Z1;  // rows x b complex each
   float* partial;          // [nchunks][d][b1][2048] complex
+  int fp16;                // 1: FP16 operands (column-scaled), 0: BF16
 };
 int fft_bf16_supported(int lb);
 hipError_t launch_fft_bf16_compress(const Bf16FftArgs& a, hipStream_t s);

@@ -59,6 +59,9 @@ extern "C" {
                                  transforms as radix-16 DFT stages on BF16
                                  matrix cores with hi/lo split correction and
                                  FP32 accumulation (precision study mode) */
+#define CM_FLAG_FP16_MFMA 32u /* as CM_FLAG_BF16_MFMA with FP16 operands:
+                                 each DFT-16 input column scaled by a power
+                                 of two into FP16's range before the split */
 
 typedef struct cm_sketch cm_sketch; /* device-resident compressed product */
 

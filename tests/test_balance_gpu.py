@@ -78,10 +78,11 @@ def test_fft_f32_imbalanced_large_rows(mode):
     assert err <= 1e-5
 
 
-def test_fft_bf16_mode_imbalanced():
+@pytest.mark.parametrize("flag", [cma.FLAG_BF16_MFMA, cma.FLAG_FP16_MFMA], ids=["bf16", "fp16"])
+def test_fft_matrix_core_modes_imbalanced(flag):
     AT, B = imbalanced(256, 128, 192, 1e3, "per_k", 51, np.float32)
-    err, _, _ = sketch_err("fft", AT, B, 3, 1 << 12, cma.FLAG_BF16_MFMA)
-    print(f"fft bf16 ratio 1e3 per_k: rel err {err:.2e}")
+    err, _, _ = sketch_err("fft", AT, B, 3, 1 << 12, flag)
+    print(f"fft matrix-core mode {flag} ratio 1e3 per_k: rel err {err:.2e}")
     assert err <= 1e-5
 
 

@@ -1,9 +1,9 @@
 """The three larger workloads, checked against the oracle at their real sizes.
 
 * Precision form: A 4096 x 16384, B 16384 x 4096 (AT 16384 x 4096), FFT,
-  b = 2^16, d = 11, 512 planted entries of 5.0, in the three arithmetic
-  modes (FP32 butterflies; FP64 accumulation of the per-chunk spectra; BF16
-  matrix-core radix-16 stages with hi/lo split).  Each sketch within 1e-5
+  b = 2^16, d = 11, 512 planted entries of 5.0, in its arithmetic modes
+  (FP32 butterflies; FP64 accumulation of the per-chunk spectra; BF16 and
+  FP16 matrix-core radix-16 stages with hi/lo split).  Each sketch within 1e-5
   relative L2 of the oracle's FP64 closed form with the same hashes
   (p_t[h] = sum over h1(i) + h2(j) = h of s1 s2 C[i, j]); the recovered set
   equal to {C_true > 12.5} of the dense FP64 product.
@@ -60,7 +60,7 @@ def oracle_check(name, variant, AT64, B64, d, b, seed, values, C, rec, tau, bar=
 
 
 # ------------------------------------------------------------ precision form
-def test_precision_form_three_modes():
+def test_precision_form_modes():
     c = workloads.CONFIGS["fft4096x16384_fp32"]
     m, n, p, d, b, tau = c["m"], c["n"], c["p"], c["d"], c["b"], c["tau"]
     print(f"precision form {m}x{n}x{p} b={b} d={d}", flush=True)
@@ -71,7 +71,8 @@ def test_precision_form_three_modes():
     hA, hB = orc.hash_arrays(0, d, b, SEED, m, p)
     want = orc.sketch_from_product("fft", C64, hA, hB, b)
     dA, dB = cma.DeviceArray.from_numpy(AT), cma.DeviceArray.from_numpy(B)
-    modes = {"fp32": 0, "fp64_reduce": cma.FLAG_F64_REDUCE, "bf16_mfma": cma.FLAG_BF16_MFMA}
+    modes = {"fp32": 0, "fp64_reduce": cma.FLAG_F64_REDUCE, "bf16_mfma": cma.FLAG_BF16_MFMA,
+             "fp16_mfma": cma.FLAG_FP16_MFMA}
     for name, flags in modes.items():
         print(f"  mode {name}", flush=True)
         sk = cma.compress("fft", dA, dB, d, b, seed=SEED, dtype=np.float32, flags=flags)
