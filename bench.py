@@ -231,7 +231,7 @@ def bench_precision(args):
         "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": f"synthetic: N(0,1/{n}) noise + {c['plants']} planted entries of {c['magnitude']}",
         "config": {"workload": f"precision form: fft {m}x{n}x{p}, b={b}, d={d}, tau={tau}, "
-                               "three arithmetic modes", "parallelism": "single GPU"},
+                               "four arithmetic modes (FP32, FP64 cross-chunk, BF16 and FP16 matrix-core splits)", "parallelism": "single GPU"},
         "roofline": {"bound": COMPUTE_BOUND, "peak_note": PEAK_NOTE, "achieved": flops / k_s / 1e12, "peak": PEAK_FP32_TFLOPS,
                      "unit": "TFLOP/s", "frac": flops / k_s / 1e12 / PEAK_FP32_TFLOPS,
                      "traffic": None, "kernel": "compress (fp32 mode)",

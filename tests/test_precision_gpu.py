@@ -1,4 +1,4 @@
-"""The three arithmetic modes of the FP32 FFT path against the oracle.
+"""The arithmetic modes of the FP32 FFT path against the oracle.
 
 Modes: FP32 butterflies (the fused register kernels), FP64 accumulation of
 the per-chunk spectra (CM_FLAG_F64_REDUCE), and matrix-core radix-16 DFT
