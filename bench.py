@@ -25,6 +25,7 @@ Other workloads (--workload): the named single-product configurations, the
 build measures one GPU.
 """
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -63,12 +64,24 @@ def compress_flops(variant, m, n, p, d, b):
 
 
 def copy_probe(nbytes):
+    """Kernels of known traffic for the FETCH_SIZE / WRITE_SIZE scales: 5
+    device-to-device copies of `nbytes` (read and write nbytes each), then 5
+    fills of `nbytes` by the HIP runtime's hipMemset (write only).  Sizes
+    above the 256 MB MALL keep the counters on HBM traffic."""
     src = cma.DeviceArray((nbytes // 4,), np.float32)
     dst = cma.DeviceArray((nbytes // 4,), np.float32)
     for _ in range(5):
         cma.check(cma.lib().cm_memcpy(dst.ptr, src.ptr, nbytes, cma.DEVICE, cma.DEVICE))
     cma.synchronize()
-    print(json.dumps({"copy_probe_bytes": nbytes, "copies": 5}), flush=True)
+    hip = ctypes.CDLL("libamdhip64.so")  # already loaded by the library
+    hip.hipMemset.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t]
+    hip.hipDeviceSynchronize.argtypes = []
+    fills = 0
+    for _ in range(5):
+        fills += hip.hipMemset(dst.ptr, 0, nbytes) == 0
+    ok = hip.hipDeviceSynchronize() == 0
+    print(json.dumps({"copy_probe_bytes": nbytes, "copies": 5, "fills": fills, "fills_ok": ok}),
+          flush=True)
 
 
 def traffic_from_profile(name):
@@ -555,14 +568,20 @@ def bench_single(args, name, plain=False, quiet=False):
     AT, B, _ = workloads.make(name)
     print(f"[bench] workload {name} m={m} n={n} p={p} d={d} b={b}{' plain' if plain else ''}",
           file=sys.stderr, flush=True)
-    dAT = cma.DeviceArray.from_numpy(AT)
-    dB = cma.DeviceArray.from_numpy(B)
+    # `rotate` copies of the operands, used in turn: with more bytes than the
+    # 256 MB MALL, each product reads its inputs from HBM (traffic counters)
+    rot = max(1, getattr(args, "rotate", 1))
+    dATs = [cma.DeviceArray.from_numpy(AT) for _ in range(rot)]
+    dBs = [cma.DeviceArray.from_numpy(B) for _ in range(rot)]
     dC = cma.DeviceArray((m, p), dtype)
     flags = cma.FLAG_PLAIN if plain else 0
     sk = None
+    calls = 0
 
     def step():
-        nonlocal sk
+        nonlocal sk, calls
+        dAT, dB = dATs[calls % rot], dBs[calls % rot]
+        calls += 1
         sk = cma.compress(c["variant"], dAT, dB, d, b, seed=HASH_SEED, dtype=dtype,
                           flags=flags, sketch=sk)
         sk.decompress(dC)
@@ -662,9 +681,13 @@ def main():
     ap.add_argument("--kernel-times", action="store_true",
                     help="time every kernel class with HIP events (default: only the "
                          "dominant one, since each event pair slows the stream)")
+    ap.add_argument("--rotate", type=int, default=1, metavar="N",
+                    help="single-product workloads: cycle through N device copies of the "
+                         "operands (N x their bytes above the 256 MB MALL: inputs come from HBM)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
-                    help="only run 5 device-to-device copies of MIB MiB (the known-size "
-                         "kernel that scales FETCH_SIZE / WRITE_SIZE counter readings)")
+                    help="only run the kernels of known traffic that scale FETCH_SIZE / "
+                         "WRITE_SIZE counter readings: 5 device-to-device copies, and with "
+                         "torch 5 sums and 5 fills, of MIB MiB each")
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe(args.copy_probe << 20)
