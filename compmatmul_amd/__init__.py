@@ -101,6 +101,12 @@ def lib():
     L.cm_server_destroy.argtypes = [_vp]
     L.cm_host_alloc.argtypes = [ctypes.c_size_t, ctypes.POINTER(_vp)]
     L.cm_host_free.argtypes = [_vp]
+    L.cm_context_create.argtypes = [ctypes.c_int, _vp, ctypes.c_size_t, ctypes.POINTER(_vp)]
+    L.cm_context_destroy.argtypes = [_vp]
+    L.cm_ctx_compress.argtypes = [_vp] + L.cm_compress_ex.argtypes
+    L.cm_ctx_decompress.argtypes = [_vp] + L.cm_decompress.argtypes
+    L.cm_ctx_recover.argtypes = [_vp] + L.cm_recover.argtypes
+    L.cm_ctx_synchronize.argtypes = [_vp]
     _LIB = L
     return L
 
@@ -174,11 +180,18 @@ def _operand(x, dtype):
 
 
 class Sketch:
-    """Device-resident compressed product (d x b values plus hash arrays)."""
+    """Device-resident compressed product (d x b values plus hash arrays).
+    A sketch made through a Context decompresses and recovers through it."""
 
     def __init__(self, variant, dtype, d, b, m, p):
         self.handle = _vp()
+        self.context = None
         self.variant, self.dtype, self.d, self.b, self.m, self.p = variant, dtype, d, b, m, p
+
+    def _decompress_call(self, ptr, mem):
+        if self.context is not None:
+            return lib().cm_ctx_decompress(self.context.handle, self.handle, ptr, self.p, mem)
+        return lib().cm_decompress(self.handle, ptr, self.p, mem)
 
     def _check_out(self, out):
         """ValueError unless `out` can take the m x p estimate in this dtype."""
@@ -199,10 +212,10 @@ class Sketch:
         if out is not None:
             self._check_out(out)
         if isinstance(out, DeviceArray):
-            check(lib().cm_decompress(self.handle, out.ptr, self.p, DEVICE))
+            check(self._decompress_call(out.ptr, DEVICE))
             return out
         C = np.empty((self.m, self.p), self.dtype) if out is None else out
-        check(lib().cm_decompress(self.handle, C.ctypes.data, self.p, HOST))
+        check(self._decompress_call(C.ctypes.data, HOST))
         return C
 
     def recover(self, tau, max_out=1 << 16):
@@ -212,9 +225,12 @@ class Sketch:
             oj = np.empty(max_out, np.int32)
             ov = np.empty(max_out, np.float64)
             cnt = ctypes.c_int64()
-            check(lib().cm_recover(self.handle, float(tau), max_out,
-                                   oi.ctypes.data_as(_i32p), oj.ctypes.data_as(_i32p),
-                                   ov.ctypes.data_as(_f64p), ctypes.byref(cnt)))
+            args = (self.handle, float(tau), max_out, oi.ctypes.data_as(_i32p),
+                    oj.ctypes.data_as(_i32p), ov.ctypes.data_as(_f64p), ctypes.byref(cnt))
+            if self.context is not None:
+                check(lib().cm_ctx_recover(self.context.handle, *args))
+            else:
+                check(lib().cm_recover(*args))
             if cnt.value <= max_out:
                 k = cnt.value
                 return oi[:k].copy(), oj[:k].copy(), ov[:k].copy()
@@ -244,12 +260,14 @@ class Sketch:
             pass
 
 
-def compress(variant, AT, B, d, b, seed=None, hash=None, dtype=None, flags=0, sketch=None):
+def compress(variant, AT, B, d, b, seed=None, hash=None, dtype=None, flags=0, sketch=None,
+             context=None):
     """Compressed product of AT (n x m) and B (n x p).
 
     `AT`/`B` are numpy arrays or DeviceArrays; dtype float64 or float32
     (default: the operands' dtype).  Pass `sketch` to reuse its device
-    buffers (no allocation).  Returns a Sketch.
+    buffers (no allocation), `context` to run on a Context.  Returns a
+    Sketch.
     """
     v = VARIANTS[variant]
     if dtype is None:
@@ -271,11 +289,48 @@ def compress(variant, AT, B, d, b, seed=None, hash=None, dtype=None, flags=0, sk
     if seed is None:
         seed = default_seed()
     sk = sketch if sketch is not None else Sketch(v, dtype, d, b, m, p)
-    check(lib().cm_compress_ex(v, code, m, n, p, pa, lda, pb, ldb, d, b, seed,
-                               FAMILIES[hash], ma, flags, ctypes.byref(sk.handle)))
+    args = (v, code, m, n, p, pa, lda, pb, ldb, d, b, seed, FAMILIES[hash], ma, flags,
+            ctypes.byref(sk.handle))
+    if context is not None:
+        check(lib().cm_ctx_compress(context.handle, *args))
+    else:
+        check(lib().cm_compress_ex(*args))
     sk.seed = seed
+    sk.context = context
     del ka, kb
     return sk
+
+
+class Context:
+    """A device, a stream and working memory of its own (cm_context_create).
+    Sketches compressed through a context decompress and recover through it;
+    calls on different contexts run concurrently from different threads.
+    With workspace_bytes > 0 every per-call intermediate is carved from one
+    device region of that size, made here: a product that needs more raises
+    MemoryError, and the plain and matrix-core paths raise CMError."""
+
+    def __init__(self, device=0, workspace_bytes=0):
+        self.handle = _vp()
+        check(lib().cm_context_create(int(device), None, int(workspace_bytes),
+                                      ctypes.byref(self.handle)))
+
+    def compress(self, variant, AT, B, d, b, seed=None, hash=None, dtype=None, flags=0,
+                 sketch=None):
+        return compress(variant, AT, B, d, b, seed, hash, dtype, flags, sketch, context=self)
+
+    def synchronize(self):
+        check(lib().cm_ctx_synchronize(self.handle))
+
+    def destroy(self):
+        if self.handle:
+            check(lib().cm_context_destroy(self.handle))
+            self.handle = _vp()
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
 
 
 def matmul(variant, AT, B, d, b, seed=None, hash=None, dtype=None):

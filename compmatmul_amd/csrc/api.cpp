@@ -12,6 +12,7 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <numeric>
 #include <string>
@@ -27,6 +28,8 @@ struct cm_sketch {
   uint32_t* hA;  // device, d x m packed bucket/sign
   uint32_t* hB;  // device, d x p
   void* P;       // device, d x b sketch values (element type per f64)
+  bool on_default = true;  // made on the library's default context (its stream)
+  int device = 0;          // where hA, hB and P live
 };
 
 namespace {
@@ -73,13 +76,22 @@ struct State {
   std::vector<Rec> pending;
   std::vector<hipEvent_t> event_pool;
   std::map<std::string, std::pair<double, int64_t>> totals;
+  // a context's workspace: every per-call device intermediate is carved
+  // from [ws_base, ws_base + ws_cap) (nullptr: buffers grow on demand)
+  char* ws_base = nullptr;
+  size_t ws_cap = 0;
 };
-State g;
+// The library's default context (device 0 on first use), which every entry
+// point without a context argument uses, and the context of this thread's
+// current call.
+State g_default;
+thread_local State* g_cur = &g_default;
+inline State& cur() { return *g_cur; }
 
 int ensure_buf(DevBuf& b, size_t bytes) {
   if (b.n >= bytes && b.p) return CM_OK;
   if (b.p) {
-    CM_HIP(hipStreamSynchronize(g.stream));
+    CM_HIP(hipStreamSynchronize(cur().stream));
     CM_HIP(hipFree(b.p));
     b.p = nullptr;
     b.n = 0;
@@ -95,20 +107,20 @@ int ensure_buf(DevBuf& b, size_t bytes) {
 }
 
 int ensure_pinned(size_t bytes) {
-  if (g.pinned_n >= bytes) return CM_OK;
-  if (g.pinned) {
-    CM_HIP(hipStreamSynchronize(g.stream));
-    CM_HIP(hipHostFree(g.pinned));
-    g.pinned = nullptr;
-    g.pinned_n = 0;
+  if (cur().pinned_n >= bytes) return CM_OK;
+  if (cur().pinned) {
+    CM_HIP(hipStreamSynchronize(cur().stream));
+    CM_HIP(hipHostFree(cur().pinned));
+    cur().pinned = nullptr;
+    cur().pinned_n = 0;
   }
-  CM_HIP(hipHostMalloc(&g.pinned, bytes, hipHostMallocDefault));
-  g.pinned_n = bytes;
+  CM_HIP(hipHostMalloc(&cur().pinned, bytes, hipHostMallocDefault));
+  cur().pinned_n = bytes;
   return CM_OK;
 }
 
 int init_locked(int device) {
-  if (g.stream && (device < 0 || device == g.device)) return CM_OK;
+  if (cur().stream && (device < 0 || device == cur().device)) return CM_OK;
   if (device < 0) device = 0;
   int count = 0;
   if (hipGetDeviceCount(&count) != hipSuccess || count == 0)
@@ -120,9 +132,9 @@ int init_locked(int device) {
     return fail(CM_EDEVICE, std::string("compmatmul_amd is built for gfx950, found ") +
                                 prop.gcnArchName);
   CM_HIP(hipSetDevice(device));
-  if (g.stream) CM_HIP(hipStreamDestroy(g.stream));
-  CM_HIP(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
-  g.device = device;
+  if (cur().stream) CM_HIP(hipStreamDestroy(cur().stream));
+  CM_HIP(hipStreamCreateWithFlags(&cur().stream, hipStreamNonBlocking));
+  cur().device = device;
   // the no-allocation entry's scratch address on this device
   const int rc = cm::noalloc_setup(device);
   if (rc) return fail(CM_EDEVICE, "no-allocation scratch: hipGetSymbolAddress failed");
@@ -130,9 +142,9 @@ int init_locked(int device) {
 }
 
 hipEvent_t get_event() {
-  if (!g.event_pool.empty()) {
-    hipEvent_t e = g.event_pool.back();
-    g.event_pool.pop_back();
+  if (!cur().event_pool.empty()) {
+    hipEvent_t e = cur().event_pool.back();
+    cur().event_pool.pop_back();
     return e;
   }
   hipEvent_t e = nullptr;
@@ -145,26 +157,26 @@ hipEvent_t get_event() {
 template <class F>
 int timed_on(hipStream_t stream, const char* name, F&& launch) {
   hipEvent_t a = nullptr, b = nullptr;
-  const std::string& only = g.timing_only;
+  const std::string& only = cur().timing_only;
   const bool prefix = !only.empty() && only.back() == '*';  // "compress*": every compress class
-  bool on = g.timing && (only.empty() || only == name ||
+  bool on = cur().timing && (only.empty() || only == name ||
                          (prefix && std::strncmp(name, only.c_str(), only.size() - 1) == 0));
   if (on) {
     a = get_event();
     b = get_event();
     on = a && b && hipEventRecord(a, stream) == hipSuccess;
     if (!on) {
-      if (a) g.event_pool.push_back(a);
-      if (b) g.event_pool.push_back(b);
+      if (a) cur().event_pool.push_back(a);
+      if (b) cur().event_pool.push_back(b);
     }
   }
   const hipError_t e = launch();
   if (on) {
     if (hipEventRecord(b, stream) == hipSuccess) {
-      g.pending.push_back({name, a, b});
+      cur().pending.push_back({name, a, b});
     } else {
-      g.event_pool.push_back(a);
-      g.event_pool.push_back(b);
+      cur().event_pool.push_back(a);
+      cur().event_pool.push_back(b);
     }
   }
   if (e != hipSuccess) return fail(CM_EDEVICE, std::string(name) + " launch failed: " + hipGetErrorString(e));
@@ -173,23 +185,23 @@ int timed_on(hipStream_t stream, const char* name, F&& launch) {
 
 template <class F>
 int timed(const char* name, F&& launch) {
-  return timed_on(g.stream, name, launch);
+  return timed_on(cur().stream, name, launch);
 }
 
 // each pair's own end event is waited for: pairs may sit on other streams
 void resolve_timing() {
-  if (g.pending.empty()) return;
-  for (auto& r : g.pending) {
+  if (cur().pending.empty()) return;
+  for (auto& r : cur().pending) {
     float ms = 0.f;
     (void)hipEventSynchronize(r.b);
     (void)hipEventElapsedTime(&ms, r.a, r.b);
-    auto& t = g.totals[r.name];
+    auto& t = cur().totals[r.name];
     t.first += ms;
     t.second += 1;
-    g.event_pool.push_back(r.a);
-    g.event_pool.push_back(r.b);
+    cur().event_pool.push_back(r.a);
+    cur().event_pool.push_back(r.b);
   }
-  g.pending.clear();
+  cur().pending.clear();
 }
 
 int ilog2(int64_t x) {
@@ -214,39 +226,39 @@ int check_family(int family) {
 // element type) and kept for the process
 int twiddles(int lb, int f64, const void** out) {
   auto key = std::make_pair(lb, f64);
-  auto it = g.twiddles.find(key);
-  if (it != g.twiddles.end()) {
+  auto it = cur().twiddles.find(key);
+  if (it != cur().twiddles.end()) {
     *out = it->second;
     return CM_OK;
   }
   void* dev = nullptr;
   CM_HIP(hipMalloc(&dev, cm::twiddle_bytes(lb, f64)));
-  hipError_t e = cm::launch_twiddles(lb, f64, dev, g.stream);
+  hipError_t e = cm::launch_twiddles(lb, f64, dev, cur().stream);
   // batches and server waves read the table from streams of their own
-  if (e == hipSuccess) e = hipStreamSynchronize(g.stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(cur().stream);
   if (e != hipSuccess) {
     (void)hipFree(dev);
     return fail(CM_EDEVICE, std::string("twiddle table: ") + hipGetErrorString(e));
   }
-  g.twiddles[key] = dev;
+  cur().twiddles[key] = dev;
   *out = dev;
   return CM_OK;
 }
 
 // Table of the FFT register kernel: W_2048^(L * bitrev5(q)) at [q*64 + L].
 int fft_t2_table(const void** out) {
-  if (!g.fft_t2) {
+  if (!cur().fft_t2) {
     void* dev = nullptr;
     CM_HIP(hipMalloc(&dev, 2048 * sizeof(float2)));
-    hipError_t e = cm::launch_t2_table(dev, g.stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(g.stream);
+    hipError_t e = cm::launch_t2_table(dev, cur().stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(cur().stream);
     if (e != hipSuccess) {
       (void)hipFree(dev);
       return fail(CM_EDEVICE, std::string("DFT table: ") + hipGetErrorString(e));
     }
-    g.fft_t2 = dev;
+    cur().fft_t2 = dev;
   }
-  *out = g.fft_t2;
+  *out = cur().fft_t2;
   return CM_OK;
 }
 
@@ -256,6 +268,7 @@ int fft_t2_table(const void** out) {
 struct Workspace {
   char* base = nullptr;  // nullptr: the library's buffers and cached tables
   size_t cap = 0, off = 0;
+  const char* what = "no-allocation scratch";
   bool carved() const { return base != nullptr; }
   static size_t up(size_t x) { return (x + 255) & ~size_t(255); }
   // bytes still free in the region (no limit for the library's buffers)
@@ -269,8 +282,8 @@ struct Workspace {
     }
     const size_t at = up(off);
     if (at + bytes > cap)
-      return fail(CM_ENOMEM, "the product needs more than the " + std::to_string(cap >> 20) +
-                                 " MiB no-allocation scratch (at least " +
+      return fail(CM_ENOMEM, "the product needs more than the " + std::to_string(cap >> 20) + " MiB " +
+                                 what + " (at least " +
                                  std::to_string((at + bytes + ((1 << 20) - 1)) >> 20) + " MiB)");
     *out = base + at;
     off = at + bytes;
@@ -282,7 +295,7 @@ struct Workspace {
     void* p = nullptr;
     int rc = take(none, cm::twiddle_bytes(lb, f64), &p);
     if (rc) return rc;
-    CM_HIP(cm::launch_twiddles(lb, f64, p, g.stream));
+    CM_HIP(cm::launch_twiddles(lb, f64, p, cur().stream));
     *out = p;
     return CM_OK;
   }
@@ -292,18 +305,33 @@ struct Workspace {
     void* p = nullptr;
     int rc = take(none, 2048 * sizeof(float2), &p);
     if (rc) return rc;
-    CM_HIP(cm::launch_t2_table(p, g.stream));
+    CM_HIP(cm::launch_t2_table(p, cur().stream));
     *out = p;
     return CM_OK;
   }
 };
 
+// The workspace of one call on the current context: its budget region,
+// carved from the start, or the library's buffers.
+Workspace call_workspace() {
+  Workspace ws;
+  if (cur().ws_base) {
+    ws.base = cur().ws_base;
+    ws.cap = cur().ws_cap;
+    ws.what = "context workspace";
+  }
+  return ws;
+}
+
 // Draw h1, h2, s1, s2 and evaluate the packed arrays on the device.
 // With no_alloc, only parameters that travel inside the kernel arguments are
 // taken (multiply families with 4 d words <= 256), so nothing is allocated
 // or uploaded.
+// Parameters and tables that do not fit the kernel arguments are uploaded
+// into buffers taken from `ws` (a carved workspace: from its region, through
+// a synchronous copy instead of the pinned staging buffer).
 int build_hashes(int family, int d, int lb, uint64_t seed, int m, int p, uint32_t* outA,
-                 uint32_t* outB, bool no_alloc = false) {
+                 uint32_t* outB, Workspace& ws, bool no_alloc = false) {
   cm::HashArgs ha{};
   ha.family = family;
   ha.d = d;
@@ -329,14 +357,20 @@ int build_hashes(int family, int d, int lb, uint64_t seed, int m, int p, uint32_
       all.insert(all.end(), one.begin(), one.end());
     }
     {
-      int rc = ensure_buf(g.params, all.size() * 8);
+      void* dst = nullptr;
+      int rc = ws.take(cur().params, all.size() * 8, &dst);
       if (rc) return rc;
-      rc = ensure_pinned(all.size() * 8);
-      if (rc) return rc;
-      CM_HIP(hipStreamSynchronize(g.stream));
-      std::memcpy(g.pinned, all.data(), all.size() * 8);
-      CM_HIP(hipMemcpyAsync(g.params.p, g.pinned, all.size() * 8, hipMemcpyHostToDevice, g.stream));
-      ha.mul_params = static_cast<const uint64_t*>(g.params.p);
+      if (ws.carved()) {
+        CM_HIP(hipMemcpyAsync(dst, all.data(), all.size() * 8, hipMemcpyHostToDevice, cur().stream));
+        CM_HIP(hipStreamSynchronize(cur().stream));  // `all` is released on return
+      } else {
+        rc = ensure_pinned(all.size() * 8);
+        if (rc) return rc;
+        CM_HIP(hipStreamSynchronize(cur().stream));
+        std::memcpy(cur().pinned, all.data(), all.size() * 8);
+        CM_HIP(hipMemcpyAsync(dst, cur().pinned, all.size() * 8, hipMemcpyHostToDevice, cur().stream));
+      }
+      ha.mul_params = static_cast<const uint64_t*>(dst);
     }
   } else {
     const int chars = family == cm::kTab8 ? 4 : 2;
@@ -344,12 +378,19 @@ int build_hashes(int family, int d, int lb, uint64_t seed, int m, int p, uint32_
     const int64_t maxkey = std::max(1, std::max(m, p)) - 1;
     const int64_t tab_len = family == cm::kTab8 ? 256 : std::min<int64_t>(full, maxkey + 1);
     const size_t bytes = size_t(4) * d * chars * tab_len * 4;
-    int rc = ensure_buf(g.tables, bytes);
+    void* dev = nullptr;
+    int rc = ws.take(cur().tables, bytes, &dev);
     if (rc) return rc;
-    rc = ensure_pinned(bytes);
-    if (rc) return rc;
-    CM_HIP(hipStreamSynchronize(g.stream));
-    uint32_t* dst = static_cast<uint32_t*>(g.pinned);
+    std::vector<uint32_t> staged;  // a carved workspace: host staging, copied synchronously
+    if (ws.carved()) {
+      staged.resize(bytes / 4);
+    } else {
+      rc = ensure_pinned(bytes);
+      if (rc) return rc;
+      CM_HIP(hipStreamSynchronize(cur().stream));
+    }
+    uint32_t* const host = ws.carved() ? staged.data() : static_cast<uint32_t*>(cur().pinned);
+    uint32_t* dst = host;
     std::vector<uint32_t> tabs;
     for (int f = 0; f < 4; ++f) {
       cm::draw_tables(family, d, seed + uint64_t(f), tabs);
@@ -360,15 +401,16 @@ int build_hashes(int family, int d, int lb, uint64_t seed, int m, int p, uint32_
           dst += tab_len;
         }
     }
-    CM_HIP(hipMemcpyAsync(g.tables.p, g.pinned, bytes, hipMemcpyHostToDevice, g.stream));
-    ha.tables = static_cast<const uint32_t*>(g.tables.p);
+    CM_HIP(hipMemcpyAsync(dev, host, bytes, hipMemcpyHostToDevice, cur().stream));
+    if (ws.carved()) CM_HIP(hipStreamSynchronize(cur().stream));
+    ha.tables = static_cast<const uint32_t*>(dev);
     ha.tab_len = int(tab_len);
   }
   if (no_alloc) {
-    CM_HIP(cm::launch_hash(ha, g.stream));
+    CM_HIP(cm::launch_hash(ha, cur().stream));
     return CM_OK;
   }
-  return timed("hash", [&] { return cm::launch_hash(ha, g.stream); });
+  return timed("hash", [&] { return cm::launch_hash(ha, cur().stream); });
 }
 
 int alloc_sketch(int fwht, int f64, int d, int b, int m, int p, cm_sketch** out) {
@@ -381,6 +423,7 @@ int alloc_sketch(int fwht, int f64, int d, int b, int m, int p, cm_sketch** out)
   sk->m = m;
   sk->p = p;
   const size_t esz = f64 ? 8 : 4;
+  if (hipGetDevice(&sk->device) != hipSuccess) sk->device = 0;
   hipError_t e1 = hipMalloc(&sk->hA, std::max<size_t>(4, size_t(d) * m * 4));
   hipError_t e2 = hipMalloc(&sk->hB, std::max<size_t>(4, size_t(d) * p * 4));
   hipError_t e3 = hipMalloc(&sk->P, size_t(d) * b * esz);
@@ -393,17 +436,18 @@ int alloc_sketch(int fwht, int f64, int d, int b, int m, int p, cm_sketch** out)
 }
 
 // stage a host matrix (rows x cols, leading dimension ld) into a device buffer
-int stage_in(DevBuf& buf, const void* host, int rows, int cols, int64_t ld, size_t esz,
+int stage_in(Workspace& ws, DevBuf& buf, const void* host, int rows, int cols, int64_t ld, size_t esz,
              const void** dev) {
   if (rows == 0 || cols == 0) {
     *dev = nullptr;
     return CM_OK;
   }
   const size_t elems = size_t(rows - 1) * size_t(ld) + size_t(cols);
-  int rc = ensure_buf(buf, elems * esz);
+  void* d = nullptr;
+  int rc = ws.take(buf, elems * esz, &d);
   if (rc) return rc;
-  CM_HIP(hipMemcpyAsync(buf.p, host, elems * esz, hipMemcpyHostToDevice, g.stream));
-  *dev = buf.p;
+  CM_HIP(hipMemcpyAsync(d, host, elems * esz, hipMemcpyHostToDevice, cur().stream));
+  *dev = d;
   return CM_OK;
 }
 
@@ -433,11 +477,11 @@ int compress_device(cm_sketch* sk, const void* AT, int64_t ld_at, const void* B,
   // FFT: per-k power-of-two balance of the pair-packed rows, computed by
   // the first launch of the compress stage (balance.hip)
   cm::PairScaleArgs psa{f64, AT, ld_at, B, ld_b, m, n, p, nullptr};
-  auto with_scales = [&]() { return fwht ? hipSuccess : cm::launch_pair_scale(psa, g.stream); };
+  auto with_scales = [&]() { return fwht ? hipSuccess : cm::launch_pair_scale(psa, cur().stream); };
   if (!fwht) {
     int rc = ws.twiddles(lb, f64, &tw);
     if (rc) return rc;
-    if ((rc = ws.take(g.scale, std::max<size_t>(1, size_t(n) * 2 * esz), &psa.out))) return rc;
+    if ((rc = ws.take(cur().scale, std::max<size_t>(1, size_t(n) * 2 * esz), &psa.out))) return rc;
   }
   if (ws.carved() && (flags & (CM_FLAG_PLAIN | CM_FLAG_BF16_MFMA | CM_FLAG_FP16_MFMA)))
     return fail(CM_ENOTSUP, "the plain and matrix-core paths allocate their own buffers");
@@ -457,18 +501,18 @@ int compress_device(cm_sketch* sk, const void* AT, int64_t ld_at, const void* B,
     const int kchunks = std::max(1, (nk + 31) / 32);
     const int lb2 = std::min(lb, max_lb2);
     int rc;
-    if ((rc = ensure_buf(g.plainA, std::max<size_t>(1, size_t(nt) * nk * b * cplx * esz))) ||
-        (rc = ensure_buf(g.plainB, fwht ? std::max<size_t>(1, size_t(nt) * nk * b * esz) : 1)) ||
-        (rc = ensure_buf(g.entries, std::max<size_t>(1, size_t(kchunks) * nt * b * cplx * esz))) ||
-        (rc = ensure_buf(g.partial, size_t(d) * b * cplx * esz)) ||
-        (rc = ensure_buf(g.V, size_t(d) * b * cplx * esz)))
+    if ((rc = ensure_buf(cur().plainA, std::max<size_t>(1, size_t(nt) * nk * b * cplx * esz))) ||
+        (rc = ensure_buf(cur().plainB, fwht ? std::max<size_t>(1, size_t(nt) * nk * b * esz) : 1)) ||
+        (rc = ensure_buf(cur().entries, std::max<size_t>(1, size_t(kchunks) * nt * b * cplx * esz))) ||
+        (rc = ensure_buf(cur().partial, size_t(d) * b * cplx * esz)) ||
+        (rc = ensure_buf(cur().V, size_t(d) * b * cplx * esz)))
       return rc;
     if (n == 0) {  // no pass: the spectral sums are zero
       const size_t words = size_t(d) * b * cplx * esz / 4;
       for (size_t w0 = 0; w0 < words; w0 += size_t(1) << 30) {
         const int cnt = int(std::min<size_t>(size_t(1) << 30, words - w0));
-        unsigned* at = static_cast<unsigned*>(g.partial.p) + w0;
-        if ((rc = timed("plain_mac", [&] { return cm::launch_zero_u32(at, cnt, g.stream); }))) return rc;
+        unsigned* at = static_cast<unsigned*>(cur().partial.p) + w0;
+        if ((rc = timed("plain_mac", [&] { return cm::launch_zero_u32(at, cnt, cur().stream); }))) return rc;
       }
     }
     for (int t0 = 0; n > 0 && t0 < d; t0 += nt) {
@@ -493,29 +537,29 @@ int compress_device(cm_sketch* sk, const void* AT, int64_t ld_at, const void* B,
         pa.nk = std::min(nk, n - k0);
         pa.tw = tw;
         pa.scale = psa.out;
-        pa.SA = g.plainA.p;
-        pa.SB = g.plainB.p;
+        pa.SA = cur().plainA.p;
+        pa.SB = cur().plainB.p;
         pa.kchunks = kchunks;
-        pa.part = g.entries.p;
-        pa.acc = g.partial.p;
+        pa.part = cur().entries.p;
+        pa.acc = cur().partial.p;
         pa.lb2 = lb2;
         pa.first = k0 == 0;
         const bool first_pass = t0 == 0 && k0 == 0;
         if ((rc = timed("plain_sketch", [&] {
                // the balance scales once, ahead of the first pass's sketch
                const hipError_t e = first_pass ? with_scales() : hipSuccess;
-               return e != hipSuccess ? e : cm::launch_plain_sketch(pa, g.stream);
+               return e != hipSuccess ? e : cm::launch_plain_sketch(pa, cur().stream);
              })))
           return rc;
-        if ((rc = timed("plain_transform", [&] { return cm::launch_plain_transform(pa, g.stream); }))) return rc;
-        if ((rc = timed("plain_mac", [&] { return cm::launch_plain_mac(pa, g.stream); }))) return rc;
+        if ((rc = timed("plain_transform", [&] { return cm::launch_plain_transform(pa, cur().stream); }))) return rc;
+        if ((rc = timed("plain_mac", [&] { return cm::launch_plain_mac(pa, cur().stream); }))) return rc;
       }
     }
     const int f64_reduce = (flags & CM_FLAG_F64_REDUCE) || n > 16384;
-    cm::ReduceArgs ra{fwht, f64, f64_reduce, g.partial.p, 1, d, lb, lb2, tw, g.V.p};
-    if ((rc = timed("reduce", [&] { return cm::launch_reduce_inverse(ra, g.stream); }))) return rc;
-    cm::CombineArgs ca{fwht, f64, g.V.p, d, lb, lb2, tw, sk->P, lb};
-    return timed("combine", [&] { return cm::launch_combine(ca, g.stream); });
+    cm::ReduceArgs ra{fwht, f64, f64_reduce, cur().partial.p, 1, d, lb, lb2, tw, cur().V.p};
+    if ((rc = timed("reduce", [&] { return cm::launch_reduce_inverse(ra, cur().stream); }))) return rc;
+    cm::CombineArgs ca{fwht, f64, cur().V.p, d, lb, lb2, tw, sk->P, lb};
+    return timed("combine", [&] { return cm::launch_combine(ca, cur().stream); });
   }
 
   if (flags & (CM_FLAG_BF16_MFMA | CM_FLAG_FP16_MFMA)) {
@@ -532,9 +576,9 @@ int compress_device(cm_sketch* sk, const void* AT, int64_t ld_at, const void* B,
     if ((rc = twiddles(lb, 0, &twb))) return rc;
     const size_t row_bytes = size_t(rows) * b * 8;
     const size_t chunk_bytes = size_t(d) * b * 8;
-    if ((rc = ensure_buf(g.plainA, row_bytes)) || (rc = ensure_buf(g.plainB, row_bytes)) ||
-        (rc = ensure_buf(g.partial, std::max<size_t>(1, size_t(nchunks) * chunk_bytes))) ||
-        (rc = ensure_buf(g.V, chunk_bytes)))
+    if ((rc = ensure_buf(cur().plainA, row_bytes)) || (rc = ensure_buf(cur().plainB, row_bytes)) ||
+        (rc = ensure_buf(cur().partial, std::max<size_t>(1, size_t(nchunks) * chunk_bytes))) ||
+        (rc = ensure_buf(cur().V, chunk_bytes)))
       return rc;
     cm::Bf16FftArgs fa{};
     fa.AT = static_cast<const float*>(AT);
@@ -552,20 +596,20 @@ int compress_device(cm_sketch* sk, const void* AT, int64_t ld_at, const void* B,
     fa.nchunks = nchunks;
     fa.tw = twb;
     fa.scale = static_cast<const float2*>(psa.out);
-    fa.Z0 = static_cast<float2*>(g.plainA.p);
-    fa.Z1 = static_cast<float2*>(g.plainB.p);
-    fa.partial = static_cast<float*>(g.partial.p);
+    fa.Z0 = static_cast<float2*>(cur().plainA.p);
+    fa.Z1 = static_cast<float2*>(cur().plainB.p);
+    fa.partial = static_cast<float*>(cur().partial.p);
     fa.fp16 = (flags & CM_FLAG_FP16_MFMA) ? 1 : 0;
     if ((rc = timed("compress", [&] {
            const hipError_t e = with_scales();
-           return e != hipSuccess ? e : cm::launch_fft_bf16_compress(fa, g.stream);
+           return e != hipSuccess ? e : cm::launch_fft_bf16_compress(fa, cur().stream);
          })))
       return rc;
     const int f64_reduce = (flags & CM_FLAG_F64_REDUCE) || n > 16384;
-    cm::ReduceArgs ra{0, 0, f64_reduce, g.partial.p, nchunks, d, lb, cm::kRegLb2, twb, g.V.p};
-    if ((rc = timed("reduce", [&] { return cm::launch_reduce_inverse(ra, g.stream); }))) return rc;
-    cm::CombineArgs ca{0, 0, g.V.p, d, lb, cm::kRegLb2, twb, sk->P, lb};
-    return timed("combine", [&] { return cm::launch_combine(ca, g.stream); });
+    cm::ReduceArgs ra{0, 0, f64_reduce, cur().partial.p, nchunks, d, lb, cm::kRegLb2, twb, cur().V.p};
+    if ((rc = timed("reduce", [&] { return cm::launch_reduce_inverse(ra, cur().stream); }))) return rc;
+    cm::CombineArgs ca{0, 0, cur().V.p, d, lb, cm::kRegLb2, twb, sk->P, lb};
+    return timed("combine", [&] { return cm::launch_combine(ca, cur().stream); });
   }
 
   // fast path: FP32 with b >= 1024 runs the register-resident kernels
@@ -607,8 +651,8 @@ int compress_device(cm_sketch* sk, const void* AT, int64_t ld_at, const void* B,
     const int kper = n > 0 ? int((n + nchunks - 1) / nchunks) : 0;
     nchunks = n > 0 ? (n + kper - 1) / kper : 0;
     void *partial = nullptr, *V = nullptr, *lists = nullptr;
-    if ((rc = ws.take(g.partial, std::max<size_t>(1, size_t(nchunks) * chunk_bytes), &partial)) ||
-        (rc = ws.take(g.V, chunk_bytes, &V)) || (rc = ws.take(g.entries, lists_bytes, &lists)))
+    if ((rc = ws.take(cur().partial, std::max<size_t>(1, size_t(nchunks) * chunk_bytes), &partial)) ||
+        (rc = ws.take(cur().V, chunk_bytes, &V)) || (rc = ws.take(cur().entries, lists_bytes, &lists)))
       return rc;
     cm::RegJob j{};
     j.AT = static_cast<const float*>(AT);
@@ -636,11 +680,11 @@ int compress_device(cm_sketch* sk, const void* AT, int64_t ld_at, const void* B,
     return run_stages(ws, {{"compress", [&] {
                               const hipError_t e = with_scales();
                               if (e != hipSuccess) return e;
-                              if (grp) return fwht ? cm::launch_fwht_grp(j, g.stream) : cm::launch_fft_grp(j, g.stream);
-                              return fwht ? cm::launch_fwht_reg(j, g.stream) : cm::launch_fft_reg(j, g.stream);
+                              if (grp) return fwht ? cm::launch_fwht_grp(j, cur().stream) : cm::launch_fft_grp(j, cur().stream);
+                              return fwht ? cm::launch_fwht_reg(j, cur().stream) : cm::launch_fft_reg(j, cur().stream);
                             }},
-                           {"reduce", [&] { return cm::launch_reduce_inverse(ra, g.stream); }},
-                           {"combine", [&] { return cm::launch_combine(ca, g.stream); }}});
+                           {"reduce", [&] { return cm::launch_reduce_inverse(ra, cur().stream); }},
+                           {"combine", [&] { return cm::launch_combine(ca, cur().stream); }}});
   }
 
   const int lb2 = std::min(lb, max_lb2);
@@ -662,8 +706,8 @@ int compress_device(cm_sketch* sk, const void* AT, int64_t ld_at, const void* B,
   nchunks = n > 0 ? (n + kper - 1) / kper : 0;
   void *partial = nullptr, *V = nullptr;
   int rc;
-  if ((rc = ws.take(g.partial, std::max<size_t>(1, size_t(nchunks) * chunk_bytes), &partial)) ||
-      (rc = ws.take(g.V, chunk_bytes, &V)))
+  if ((rc = ws.take(cur().partial, std::max<size_t>(1, size_t(nchunks) * chunk_bytes), &partial)) ||
+      (rc = ws.take(cur().V, chunk_bytes, &V)))
     return rc;
   cm::CompressArgs caa{};
   caa.fwht = fwht;
@@ -690,10 +734,10 @@ int compress_device(cm_sketch* sk, const void* AT, int64_t ld_at, const void* B,
   cm::CombineArgs ca{fwht, f64, V, d, lb, lb2, tw, sk->P, lb};
   return run_stages(ws, {{"compress", [&] {
                             const hipError_t e = with_scales();
-                            return e != hipSuccess ? e : cm::launch_compress_lds(caa, g.stream);
+                            return e != hipSuccess ? e : cm::launch_compress_lds(caa, cur().stream);
                           }},
-                         {"reduce", [&] { return cm::launch_reduce_inverse(ra, g.stream); }},
-                         {"combine", [&] { return cm::launch_combine(ca, g.stream); }}});
+                         {"reduce", [&] { return cm::launch_reduce_inverse(ra, cur().stream); }},
+                         {"combine", [&] { return cm::launch_combine(ca, cur().stream); }}});
 }
 
 int compress_locked(int variant, int dtype, int m, int n, int p, const void* AT, int64_t ld_at,
@@ -727,16 +771,16 @@ int compress_locked(int variant, int dtype, int m, int n, int p, const void* AT,
   const size_t esz = f64 ? 8 : 4;
   const void* dAT = AT;
   const void* dB = B;
+  Workspace ws = call_workspace();  // the context's region, or the library's buffers
   if (mem == CM_HOST) {
-    if ((rc = stage_in(g.AT, AT, n, m, ld_at, esz, &dAT)) ||
-        (rc = stage_in(g.B, B, n, p, ld_b, esz, &dB))) {
+    if ((rc = stage_in(ws, cur().AT, AT, n, m, ld_at, esz, &dAT)) ||
+        (rc = stage_in(ws, cur().B, B, n, p, ld_b, esz, &dB))) {
       if (fresh) cm_sketch_free(sk);
       return rc;
     }
   }
-  Workspace lib_ws;  // the library's buffers
-  if ((rc = build_hashes(family, d, sk->lb, seed, m, p, sk->hA, sk->hB)) ||
-      (rc = compress_device(sk, dAT, ld_at, dB, ld_b, n, flags, lib_ws))) {
+  if ((rc = build_hashes(family, d, sk->lb, seed, m, p, sk->hA, sk->hB, ws)) ||
+      (rc = compress_device(sk, dAT, ld_at, dB, ld_b, n, flags, ws))) {
     if (fresh) cm_sketch_free(sk);
     return rc;
   }
@@ -744,8 +788,18 @@ int compress_locked(int variant, int dtype, int m, int n, int p, const void* AT,
   return CM_OK;
 }
 
+// Device buffers of a recovery pass (in the context's workspace or the
+// library's buffers).
+struct RecoverBufs {
+  unsigned long long* counter = nullptr;
+  int32_t* oi = nullptr;
+  int32_t* oj = nullptr;
+  void* ov = nullptr;
+};
+
 int decompress_locked(const cm_sketch* sk, void* C, int64_t ldc, int mem, int recover, double tau,
-                      int64_t max_out) {
+                      int64_t max_out, RecoverBufs* rec = nullptr) {
+  Workspace ws = call_workspace();
   cm::DecompressArgs da{};
   da.fwht = sk->fwht;
   da.f64 = sk->f64;
@@ -759,34 +813,38 @@ int decompress_locked(const cm_sketch* sk, void* C, int64_t ldc, int mem, int re
   da.ldc = ldc;
   const size_t esz = sk->f64 ? 8 : 4;
   int rc;
+  void* dC = nullptr;
   if (C && mem == CM_HOST) {
     // the device copy is dense (pitch p); the copy back writes only the m x p
     // window of the caller's matrix, so columns p .. ldc-1 stay untouched
-    if ((rc = ensure_buf(g.C, std::max<size_t>(1, size_t(sk->m) * sk->p * esz)))) return rc;
-    da.C = g.C.p;
+    if ((rc = ws.take(cur().C, std::max<size_t>(1, size_t(sk->m) * sk->p * esz), &dC))) return rc;
+    da.C = dC;
     da.ldc = sk->p;
   } else {
     da.C = C;
   }
   if (recover) {
-    if ((rc = ensure_buf(g.counter, 8)) || (rc = ensure_buf(g.oi, std::max<int64_t>(1, max_out) * 4)) ||
-        (rc = ensure_buf(g.oj, std::max<int64_t>(1, max_out) * 4)) ||
-        (rc = ensure_buf(g.ov, std::max<int64_t>(1, max_out) * esz)))
+    void *cnt = nullptr, *oi = nullptr, *oj = nullptr, *ov = nullptr;
+    if ((rc = ws.take(cur().counter, 8, &cnt)) ||
+        (rc = ws.take(cur().oi, std::max<int64_t>(1, max_out) * 4, &oi)) ||
+        (rc = ws.take(cur().oj, std::max<int64_t>(1, max_out) * 4, &oj)) ||
+        (rc = ws.take(cur().ov, std::max<int64_t>(1, max_out) * esz, &ov)))
       return rc;
-    CM_HIP(hipMemsetAsync(g.counter.p, 0, 8, g.stream));
+    CM_HIP(hipMemsetAsync(cnt, 0, 8, cur().stream));
     da.recover = 1;
     da.tau = tau;
-    da.counter = static_cast<unsigned long long*>(g.counter.p);
+    da.counter = static_cast<unsigned long long*>(cnt);
     da.max_out = max_out;
-    da.oi = static_cast<int32_t*>(g.oi.p);
-    da.oj = static_cast<int32_t*>(g.oj.p);
-    da.ov = g.ov.p;
+    da.oi = static_cast<int32_t*>(oi);
+    da.oj = static_cast<int32_t*>(oj);
+    da.ov = ov;
+    if (rec) *rec = RecoverBufs{da.counter, da.oi, da.oj, da.ov};
   }
-  if ((rc = timed("decompress", [&] { return cm::launch_decompress(da, g.stream); }))) return rc;
+  if ((rc = timed("decompress", [&] { return cm::launch_decompress(da, cur().stream); }))) return rc;
   if (C && mem == CM_HOST && sk->m > 0 && sk->p > 0) {
-    CM_HIP(hipMemcpy2DAsync(C, size_t(ldc) * esz, g.C.p, size_t(sk->p) * esz, size_t(sk->p) * esz,
-                            size_t(sk->m), hipMemcpyDeviceToHost, g.stream));
-    CM_HIP(hipStreamSynchronize(g.stream));
+    CM_HIP(hipMemcpy2DAsync(C, size_t(ldc) * esz, dC, size_t(sk->p) * esz, size_t(sk->p) * esz,
+                            size_t(sk->m), hipMemcpyDeviceToHost, cur().stream));
+    CM_HIP(hipStreamSynchronize(cur().stream));
   }
   return CM_OK;
 }
@@ -811,7 +869,7 @@ int matmul_noalloc_locked(int variant, int dtype, int m, int n, int p, const voi
     return fail(CM_EINVAL, "null operand pointer");
   void* base = nullptr;
   size_t cap = 0;
-  if (cm::noalloc_scratch(g.device, &base, &cap))
+  if (cm::noalloc_scratch(cur().device, &base, &cap))
     return fail(CM_EDEVICE, "no-allocation scratch is not set up on this device");
   Workspace ws;
   ws.base = static_cast<char*>(base);
@@ -837,7 +895,7 @@ int matmul_noalloc_locked(int variant, int dtype, int m, int n, int p, const voi
   sk.hA = static_cast<uint32_t*>(hA);
   sk.hB = static_cast<uint32_t*>(hB);
   sk.P = P;
-  if ((rc = build_hashes(family, d, sk.lb, seed, m, p, sk.hA, sk.hB, true)) ||
+  if ((rc = build_hashes(family, d, sk.lb, seed, m, p, sk.hA, sk.hB, ws, true)) ||
       (rc = compress_device(&sk, AT, ld_at, B, ld_b, n, 0, ws)))
     return rc;
   cm::DecompressArgs da{};
@@ -852,8 +910,8 @@ int matmul_noalloc_locked(int variant, int dtype, int m, int n, int p, const voi
   da.b = b;
   da.C = C;
   da.ldc = ldc;
-  CM_HIP(cm::launch_decompress(da, g.stream));
-  CM_HIP(hipStreamSynchronize(g.stream));
+  CM_HIP(cm::launch_decompress(da, cur().stream));
+  CM_HIP(hipStreamSynchronize(cur().stream));
   return CM_OK;
 }
 
@@ -873,13 +931,140 @@ int matmul_dropin(int variant, int n_a, int n_inner, int n_b, double* C, const d
   return rc;
 }
 
+// The recovered entries of a sketch, sorted by (i, j), on the current context.
+int recover_locked(const cm_sketch* sk, double tau, int64_t max_out, int32_t* out_i, int32_t* out_j,
+                   double* out_val, int64_t* count) {
+  RecoverBufs rb;
+  int rc = decompress_locked(sk, nullptr, sk->p, CM_DEVICE, 1, tau, max_out, &rb);
+  if (rc) return rc;
+  unsigned long long found = 0;
+  CM_HIP(hipMemcpyAsync(&found, rb.counter, 8, hipMemcpyDeviceToHost, cur().stream));
+  CM_HIP(hipStreamSynchronize(cur().stream));
+  *count = int64_t(found);
+  const int64_t k = std::min<int64_t>(int64_t(found), max_out);
+  if (k == 0) return CM_OK;
+  std::vector<int32_t> vi(k), vj(k);
+  std::vector<double> vv(k);
+  CM_HIP(hipMemcpyAsync(vi.data(), rb.oi, k * 4, hipMemcpyDeviceToHost, cur().stream));
+  CM_HIP(hipMemcpyAsync(vj.data(), rb.oj, k * 4, hipMemcpyDeviceToHost, cur().stream));
+  if (sk->f64) {
+    CM_HIP(hipMemcpyAsync(vv.data(), rb.ov, k * 8, hipMemcpyDeviceToHost, cur().stream));
+    CM_HIP(hipStreamSynchronize(cur().stream));
+  } else {
+    std::vector<float> vf(k);
+    CM_HIP(hipMemcpyAsync(vf.data(), rb.ov, k * 4, hipMemcpyDeviceToHost, cur().stream));
+    CM_HIP(hipStreamSynchronize(cur().stream));
+    for (int64_t q = 0; q < k; ++q) vv[q] = vf[q];
+  }
+  std::vector<int64_t> order(k);
+  std::iota(order.begin(), order.end(), 0);
+  std::sort(order.begin(), order.end(), [&](int64_t x, int64_t y) {
+    return vi[x] != vi[y] ? vi[x] < vi[y] : vj[x] < vj[y];
+  });
+  for (int64_t q = 0; q < k; ++q) {
+    if (out_i) out_i[q] = vi[order[q]];
+    if (out_j) out_j[q] = vj[order[q]];
+    if (out_val) out_val[q] = vv[order[q]];
+  }
+  return CM_OK;
+}
+
+// Wait until the kernels that wrote a sketch are done: its context's stream
+// is the default one's, or a context's (then the whole of its device).
+int sketch_ready(const cm_sketch* sk) {
+  if (sk->on_default) {
+    CM_HIP(hipStreamSynchronize(g_default.stream));
+    return CM_OK;
+  }
+  int dev = -1;
+  CM_HIP(hipGetDevice(&dev));
+  if (dev != sk->device) CM_HIP(hipSetDevice(sk->device));
+  const hipError_t e = hipDeviceSynchronize();
+  if (dev != sk->device) CM_HIP(hipSetDevice(dev));
+  CM_HIP(e);
+  return CM_OK;
+}
+
+// compress with the handle's reuse rules, synchronous for host operands
+int compress_entry(int variant, int dtype, int n_a, int n_inner, int n_b, const void* AT,
+                   int64_t ld_at, const void* B, int64_t ld_b, int d, int b, uint64_t seed,
+                   int family, int mem, unsigned flags, cm_sketch** inout) {
+  int rc = compress_locked(variant, dtype, n_a, n_inner, n_b, AT, ld_at, B, ld_b, d, b, seed,
+                           family, mem, flags, inout);
+  if (!rc && mem == CM_HOST) CM_HIP(hipStreamSynchronize(cur().stream));
+  if (!rc) (*inout)->on_default = g_cur == &g_default;
+  return rc;
+}
+
+int check_decompress_args(const cm_sketch* sk, void* C, int64_t ldc, int mem) {
+  if (!sk) return fail(CM_EINVAL, "null sketch");
+  if (!C && int64_t(sk->m) * sk->p > 0) return fail(CM_EINVAL, "null output pointer");
+  if (ldc < sk->p) return fail(CM_EINVAL, "ldc smaller than the number of columns");
+  if (mem != CM_HOST && mem != CM_DEVICE) return fail(CM_EINVAL, "mem must be CM_HOST or CM_DEVICE");
+  return CM_OK;
+}
+
+// every buffer, table, event and (if owned) the stream of a context's state
+void release_state(State& st, bool own_stream) {
+  if (st.stream) (void)hipStreamSynchronize(st.stream);
+  for (DevBuf* b : {&st.partial, &st.V, &st.params, &st.tables, &st.AT, &st.B, &st.C, &st.counter,
+                    &st.oi, &st.oj, &st.ov, &st.plainA, &st.plainB, &st.entries, &st.scale}) {
+    if (b->p) (void)hipFree(b->p);
+    *b = DevBuf{};
+  }
+  if (st.fft_t2) (void)hipFree(st.fft_t2);
+  for (auto& kv : st.twiddles) (void)hipFree(kv.second);
+  st.twiddles.clear();
+  if (st.pinned) (void)hipHostFree(st.pinned);
+  for (auto& r : st.pending) {
+    (void)hipEventDestroy(r.a);
+    (void)hipEventDestroy(r.b);
+  }
+  for (hipEvent_t e : st.event_pool) (void)hipEventDestroy(e);
+  if (st.ws_base) (void)hipFree(st.ws_base);
+  if (own_stream && st.stream) (void)hipStreamDestroy(st.stream);
+  st = State{};
+}
+
+}  // namespace
+
+// A context: its own state (device, stream, buffers or workspace), and a
+// lock that serializes the calls made on it.
+struct cm_context {
+  State st;
+  std::recursive_mutex mu;
+  bool own_stream = false;
+};
+
+namespace {
+
+// The scope of one call on a context: its lock, its state as the thread's
+// current context, and its device (the thread's device is restored after).
+struct CtxCall {
+  std::lock_guard<std::recursive_mutex> lk;
+  State* prev;
+  int prev_dev = -1;
+  explicit CtxCall(cm_context* c) : lk(c->mu), prev(g_cur) {
+    if (hipGetDevice(&prev_dev) != hipSuccess) prev_dev = -1;
+    g_cur = &c->st;
+    if (prev_dev != c->st.device) (void)hipSetDevice(c->st.device);
+  }
+  ~CtxCall() {
+    g_cur = prev;
+    int now = -1;
+    if (prev_dev >= 0 && hipGetDevice(&now) == hipSuccess && now != prev_dev) (void)hipSetDevice(prev_dev);
+  }
+  CtxCall(const CtxCall&) = delete;
+  CtxCall& operator=(const CtxCall&) = delete;
+};
+
 }  // namespace
 
 namespace cm {
 namespace rt {
 std::recursive_mutex& mutex() { return g_mu; }
 int init() { return init_locked(-1); }
-hipStream_t stream() { return g.stream; }
+hipStream_t stream() { return cur().stream; }
 int error(int code, const std::string& msg) { return fail(code, msg); }
 int twiddles(int lb, int f64, const void** out) { return ::twiddles(lb, f64, out); }
 int fft_t2_table(const void** out) { return ::fft_t2_table(out); }
@@ -927,21 +1112,23 @@ int cm_gemm(int n_a, int n_inner, int n_b, double* C, const double* AT, const do
   if (rc) return rc;
   if (n_a < 0 || n_inner < 0 || n_b < 0) return fail(CM_EINVAL, "matrix dimensions must be non-negative");
   const void *dAT, *dB;
-  if ((rc = stage_in(g.AT, AT, n_inner, n_a, n_a, 8, &dAT)) || (rc = stage_in(g.B, B, n_inner, n_b, n_b, 8, &dB)))
+  Workspace lib;  // the library's buffers
+  if ((rc = stage_in(lib, cur().AT, AT, n_inner, n_a, n_a, 8, &dAT)) ||
+      (rc = stage_in(lib, cur().B, B, n_inner, n_b, n_b, 8, &dB)))
     return rc;
-  if ((rc = ensure_buf(g.C, std::max<size_t>(8, size_t(n_a) * n_b * 8)))) return rc;
+  if ((rc = ensure_buf(cur().C, std::max<size_t>(8, size_t(n_a) * n_b * 8)))) return rc;
   if (n_inner == 0) {
-    CM_HIP(hipMemsetAsync(g.C.p, 0, size_t(n_a) * n_b * 8, g.stream));
+    CM_HIP(hipMemsetAsync(cur().C.p, 0, size_t(n_a) * n_b * 8, cur().stream));
   } else if ((rc = timed("gemm", [&] {
                 return cm::launch_gemm_f64(n_a, n_inner, n_b, static_cast<const double*>(dAT),
                                            static_cast<const double*>(dB),
-                                           static_cast<double*>(g.C.p), g.stream);
+                                           static_cast<double*>(cur().C.p), cur().stream);
               }))) {
     return rc;
   }
   if (size_t(n_a) * n_b)
-    CM_HIP(hipMemcpyAsync(C, g.C.p, size_t(n_a) * n_b * 8, hipMemcpyDeviceToHost, g.stream));
-  CM_HIP(hipStreamSynchronize(g.stream));
+    CM_HIP(hipMemcpyAsync(C, cur().C.p, size_t(n_a) * n_b * 8, hipMemcpyDeviceToHost, cur().stream));
+  CM_HIP(hipStreamSynchronize(cur().stream));
   return CM_OK;
 }
 
@@ -960,12 +1147,13 @@ int cm_hash_arrays(int family, int d, int b, uint64_t seed, int m, int p, uint32
     dA = static_cast<uint32_t*>(tA.p);
     dB = static_cast<uint32_t*>(tB.p);
   }
-  rc = build_hashes(family, d, ilog2(b), seed, m, p, dA, dB);
+  Workspace lib;  // the library's buffers
+  rc = build_hashes(family, d, ilog2(b), seed, m, p, dA, dB, lib);
   if (!rc && mem == CM_HOST) {
-    if (size_t(d) * m) CM_HIP(hipMemcpyAsync(outA, dA, size_t(d) * m * 4, hipMemcpyDeviceToHost, g.stream));
-    if (size_t(d) * p) CM_HIP(hipMemcpyAsync(outB, dB, size_t(d) * p * 4, hipMemcpyDeviceToHost, g.stream));
+    if (size_t(d) * m) CM_HIP(hipMemcpyAsync(outA, dA, size_t(d) * m * 4, hipMemcpyDeviceToHost, cur().stream));
+    if (size_t(d) * p) CM_HIP(hipMemcpyAsync(outB, dB, size_t(d) * p * 4, hipMemcpyDeviceToHost, cur().stream));
   }
-  (void)hipStreamSynchronize(g.stream);
+  (void)hipStreamSynchronize(cur().stream);
   if (tA.p) (void)hipFree(tA.p);
   if (tB.p) (void)hipFree(tB.p);
   return rc;
@@ -977,10 +1165,8 @@ int cm_compress(int variant, int dtype, int n_a, int n_inner, int n_b, const voi
   if (!out) return fail(CM_EINVAL, "null output handle");
   std::lock_guard<std::recursive_mutex> lk(g_mu);
   *out = nullptr;
-  int rc = compress_locked(variant, dtype, n_a, n_inner, n_b, AT, ld_at, B, ld_b, d, b, seed,
-                           family, mem, 0, out);
-  if (!rc && mem == CM_HOST) CM_HIP(hipStreamSynchronize(g.stream));
-  return rc;
+  return compress_entry(variant, dtype, n_a, n_inner, n_b, AT, ld_at, B, ld_b, d, b, seed, family,
+                        mem, 0, out);
 }
 
 int cm_compress_ex(int variant, int dtype, int n_a, int n_inner, int n_b, const void* AT,
@@ -988,17 +1174,13 @@ int cm_compress_ex(int variant, int dtype, int n_a, int n_inner, int n_b, const 
                    int family, int mem, unsigned flags, cm_sketch** inout) {
   if (!inout) return fail(CM_EINVAL, "null handle pointer");
   std::lock_guard<std::recursive_mutex> lk(g_mu);
-  int rc = compress_locked(variant, dtype, n_a, n_inner, n_b, AT, ld_at, B, ld_b, d, b, seed,
-                           family, mem, flags, inout);
-  if (!rc && mem == CM_HOST) CM_HIP(hipStreamSynchronize(g.stream));
-  return rc;
+  return compress_entry(variant, dtype, n_a, n_inner, n_b, AT, ld_at, B, ld_b, d, b, seed, family,
+                        mem, flags, inout);
 }
 
 int cm_decompress(const cm_sketch* sk, void* C, int64_t ldc, int mem) {
-  if (!sk) return fail(CM_EINVAL, "null sketch");
-  if (!C && int64_t(sk->m) * sk->p > 0) return fail(CM_EINVAL, "null output pointer");
-  if (ldc < sk->p) return fail(CM_EINVAL, "ldc smaller than the number of columns");
-  if (mem != CM_HOST && mem != CM_DEVICE) return fail(CM_EINVAL, "mem must be CM_HOST or CM_DEVICE");
+  int rc = check_decompress_args(sk, C, ldc, mem);
+  if (rc) return rc;
   std::lock_guard<std::recursive_mutex> lk(g_mu);
   return decompress_locked(sk, C, ldc, mem, 0, 0.0, 0);
 }
@@ -1008,43 +1190,14 @@ int cm_recover(const cm_sketch* sk, double tau, int64_t max_out, int32_t* out_i,
   if (!sk || !count) return fail(CM_EINVAL, "null argument");
   if (max_out < 0) return fail(CM_EINVAL, "max_out must be non-negative");
   std::lock_guard<std::recursive_mutex> lk(g_mu);
-  int rc = decompress_locked(sk, nullptr, sk->p, CM_DEVICE, 1, tau, max_out);
-  if (rc) return rc;
-  unsigned long long found = 0;
-  CM_HIP(hipMemcpyAsync(&found, g.counter.p, 8, hipMemcpyDeviceToHost, g.stream));
-  CM_HIP(hipStreamSynchronize(g.stream));
-  *count = int64_t(found);
-  const int64_t k = std::min<int64_t>(int64_t(found), max_out);
-  if (k == 0) return CM_OK;
-  std::vector<int32_t> vi(k), vj(k);
-  std::vector<double> vv(k);
-  CM_HIP(hipMemcpy(vi.data(), g.oi.p, k * 4, hipMemcpyDeviceToHost));
-  CM_HIP(hipMemcpy(vj.data(), g.oj.p, k * 4, hipMemcpyDeviceToHost));
-  if (sk->f64) {
-    CM_HIP(hipMemcpy(vv.data(), g.ov.p, k * 8, hipMemcpyDeviceToHost));
-  } else {
-    std::vector<float> vf(k);
-    CM_HIP(hipMemcpy(vf.data(), g.ov.p, k * 4, hipMemcpyDeviceToHost));
-    for (int64_t q = 0; q < k; ++q) vv[q] = vf[q];
-  }
-  std::vector<int64_t> order(k);
-  std::iota(order.begin(), order.end(), 0);
-  std::sort(order.begin(), order.end(), [&](int64_t x, int64_t y) {
-    return vi[x] != vi[y] ? vi[x] < vi[y] : vj[x] < vj[y];
-  });
-  for (int64_t q = 0; q < k; ++q) {
-    if (out_i) out_i[q] = vi[order[q]];
-    if (out_j) out_j[q] = vj[order[q]];
-    if (out_val) out_val[q] = vv[order[q]];
-  }
-  return CM_OK;
+  return recover_locked(sk, tau, max_out, out_i, out_j, out_val, count);
 }
 
 int cm_sketch_values(const cm_sketch* sk, double* host_out) {
   if (!sk || !host_out) return fail(CM_EINVAL, "null argument");
   std::lock_guard<std::recursive_mutex> lk(g_mu);
   const size_t n = size_t(sk->d) * sk->b;
-  CM_HIP(hipStreamSynchronize(g.stream));
+  if (int rc = sketch_ready(sk)) return rc;
   if (sk->f64) {
     CM_HIP(hipMemcpy(host_out, sk->P, n * 8, hipMemcpyDeviceToHost));
   } else {
@@ -1058,7 +1211,7 @@ int cm_sketch_values(const cm_sketch* sk, double* host_out) {
 int cm_sketch_hashes(const cm_sketch* sk, uint32_t* outA, uint32_t* outB) {
   if (!sk) return fail(CM_EINVAL, "null sketch");
   std::lock_guard<std::recursive_mutex> lk(g_mu);
-  CM_HIP(hipStreamSynchronize(g.stream));
+  if (int rc = sketch_ready(sk)) return rc;
   if (outA && size_t(sk->d) * sk->m)
     CM_HIP(hipMemcpy(outA, sk->hA, size_t(sk->d) * sk->m * 4, hipMemcpyDeviceToHost));
   if (outB && size_t(sk->d) * sk->p)
@@ -1069,7 +1222,7 @@ int cm_sketch_hashes(const cm_sketch* sk, uint32_t* outA, uint32_t* outB) {
 void cm_sketch_free(cm_sketch* sk) {
   if (!sk) return;
   std::lock_guard<std::recursive_mutex> lk(g_mu);
-  if (g.stream) (void)hipStreamSynchronize(g.stream);
+  (void)sketch_ready(sk);
   if (sk->hA) (void)hipFree(sk->hA);
   if (sk->hB) (void)hipFree(sk->hB);
   if (sk->P) (void)hipFree(sk->P);
@@ -1087,7 +1240,7 @@ int cm_dev_alloc(size_t bytes, void** ptr) {
 
 int cm_dev_free(void* ptr) {
   std::lock_guard<std::recursive_mutex> lk(g_mu);
-  if (g.stream) (void)hipStreamSynchronize(g.stream);
+  if (cur().stream) (void)hipStreamSynchronize(cur().stream);
   if (ptr) CM_HIP(hipFree(ptr));
   return CM_OK;
 }
@@ -1116,8 +1269,8 @@ int cm_memcpy(void* dst, const void* src, size_t bytes, int dst_mem, int src_mem
                                                                     : hipMemcpyHostToDevice)
                                             : (src_mem == CM_DEVICE ? hipMemcpyDeviceToHost
                                                                     : hipMemcpyHostToHost);
-  CM_HIP(hipMemcpyAsync(dst, src, bytes, kind, g.stream));
-  CM_HIP(hipStreamSynchronize(g.stream));
+  CM_HIP(hipMemcpyAsync(dst, src, bytes, kind, cur().stream));
+  CM_HIP(hipStreamSynchronize(cur().stream));
   return CM_OK;
 }
 
@@ -1125,28 +1278,28 @@ int cm_synchronize(void) {
   std::lock_guard<std::recursive_mutex> lk(g_mu);
   int rc = init_locked(-1);
   if (rc) return rc;
-  CM_HIP(hipStreamSynchronize(g.stream));
+  CM_HIP(hipStreamSynchronize(cur().stream));
   return CM_OK;
 }
 
 int cm_timing_enable(int on) {
   std::lock_guard<std::recursive_mutex> lk(g_mu);
   resolve_timing();
-  g.timing = on != 0;
+  cur().timing = on != 0;
   return CM_OK;
 }
 
 int cm_timing_only(const char* name) {
   std::lock_guard<std::recursive_mutex> lk(g_mu);
   resolve_timing();
-  g.timing_only = name ? name : "";
+  cur().timing_only = name ? name : "";
   return CM_OK;
 }
 
 int cm_timing_reset(void) {
   std::lock_guard<std::recursive_mutex> lk(g_mu);
   resolve_timing();
-  g.totals.clear();
+  cur().totals.clear();
   return CM_OK;
 }
 
@@ -1154,9 +1307,95 @@ int cm_timing_get(const char* name, double* total_ms, int64_t* launches) {
   if (!name) return fail(CM_EINVAL, "null name");
   std::lock_guard<std::recursive_mutex> lk(g_mu);
   resolve_timing();
-  auto it = g.totals.find(name);
-  if (total_ms) *total_ms = it == g.totals.end() ? 0.0 : it->second.first;
-  if (launches) *launches = it == g.totals.end() ? 0 : it->second.second;
+  auto it = cur().totals.find(name);
+  if (total_ms) *total_ms = it == cur().totals.end() ? 0.0 : it->second.first;
+  if (launches) *launches = it == cur().totals.end() ? 0 : it->second.second;
+  return CM_OK;
+}
+
+// ---------------------------------------------------------------- contexts
+
+int cm_context_create(int device, void* stream, size_t workspace_bytes, cm_context** out) {
+  if (!out) return fail(CM_EINVAL, "null output handle");
+  *out = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count == 0)
+    return fail(CM_EDEVICE, "no HIP device is available (compmatmul_amd needs an MI355X)");
+  if (device < 0 || device >= count) return fail(CM_EINVAL, "device index out of range");
+  hipDeviceProp_t prop;
+  CM_HIP(hipGetDeviceProperties(&prop, device));
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+    return fail(CM_EDEVICE, std::string("compmatmul_amd is built for gfx950, found ") + prop.gcnArchName);
+  auto ctx = std::make_unique<cm_context>();
+  ctx->st.device = device;
+  int rc = CM_OK;
+  {
+    CtxCall call(ctx.get());  // the new context's device, restored after
+    if (stream) {
+      ctx->st.stream = static_cast<hipStream_t>(stream);
+    } else if (hipStreamCreateWithFlags(&ctx->st.stream, hipStreamNonBlocking) != hipSuccess) {
+      ctx->st.stream = nullptr;
+      rc = fail(CM_EDEVICE, "stream creation failed");
+    } else {
+      ctx->own_stream = true;
+    }
+    if (!rc && workspace_bytes) {
+      void* p = nullptr;
+      const hipError_t e = hipMalloc(&p, workspace_bytes);
+      if (e != hipSuccess) {
+        rc = fail(e == hipErrorOutOfMemory ? CM_ENOMEM : CM_EDEVICE,
+                  "context workspace of " + std::to_string(workspace_bytes) + " bytes: " + hipGetErrorString(e));
+      } else {
+        ctx->st.ws_base = static_cast<char*>(p);
+        ctx->st.ws_cap = workspace_bytes;
+      }
+    }
+    if (rc) release_state(ctx->st, ctx->own_stream);
+  }
+  if (rc) return rc;
+  *out = ctx.release();
+  return CM_OK;
+}
+
+int cm_context_destroy(cm_context* ctx) {
+  if (!ctx) return CM_OK;
+  {
+    CtxCall call(ctx);
+    release_state(ctx->st, ctx->own_stream);
+  }
+  delete ctx;
+  return CM_OK;
+}
+
+int cm_ctx_compress(cm_context* ctx, int variant, int dtype, int n_a, int n_inner, int n_b,
+                    const void* AT, int64_t ld_at, const void* B, int64_t ld_b, int d, int b,
+                    uint64_t seed, int family, int mem, unsigned flags, cm_sketch** inout) {
+  if (!ctx || !inout) return fail(CM_EINVAL, "null argument");
+  CtxCall call(ctx);
+  return compress_entry(variant, dtype, n_a, n_inner, n_b, AT, ld_at, B, ld_b, d, b, seed, family,
+                        mem, flags, inout);
+}
+
+int cm_ctx_decompress(cm_context* ctx, const cm_sketch* sk, void* C, int64_t ldc, int mem) {
+  if (!ctx) return fail(CM_EINVAL, "null context");
+  int rc = check_decompress_args(sk, C, ldc, mem);
+  if (rc) return rc;
+  CtxCall call(ctx);
+  return decompress_locked(sk, C, ldc, mem, 0, 0.0, 0);
+}
+
+int cm_ctx_recover(cm_context* ctx, const cm_sketch* sk, double tau, int64_t max_out, int32_t* out_i,
+                   int32_t* out_j, double* out_val, int64_t* count) {
+  if (!ctx || !sk || !count) return fail(CM_EINVAL, "null argument");
+  if (max_out < 0) return fail(CM_EINVAL, "max_out must be non-negative");
+  CtxCall call(ctx);
+  return recover_locked(sk, tau, max_out, out_i, out_j, out_val, count);
+}
+
+int cm_ctx_synchronize(cm_context* ctx) {
+  if (!ctx) return fail(CM_EINVAL, "null context");
+  CtxCall call(ctx);
+  CM_HIP(hipStreamSynchronize(ctx->st.stream));
   return CM_OK;
 }
 

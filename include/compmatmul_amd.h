@@ -174,6 +174,47 @@ int cm_sketch_hashes(const cm_sketch* sk, uint32_t* outA, uint32_t* outB);
 
 void cm_sketch_free(cm_sketch* sk);
 
+/* ---------------------------------------------------------------- contexts
+ * A context holds a device, a stream and the working memory of the products
+ * run through it; the entry points without a context argument use the
+ * library's default context (device 0 on first use, or cm_init's).  Calls on
+ * one context are serialized by its own lock, and calls on different
+ * contexts run concurrently (from different threads, on their streams).
+ * The reference has no such object: each compmatmul call allocates its
+ * sketch and per-thread scratch and frees them on return
+ * (compmatmul.cpp:107-147, matmul_fft.hpp:57-70, matmul_fwht.hpp:53-63);
+ * a context keeps that state on the device between calls. */
+typedef struct cm_context cm_context;
+
+/* device: HIP device index.  stream: the caller's hipStream_t, or NULL for
+ * a non-blocking stream the context creates and destroys.  workspace_bytes:
+ * 0 = per-call buffers grow on demand, as the default context's do; > 0 =
+ * one device region of that size, allocated here, from which every per-call
+ * intermediate (staged host operands, hash parameters and tables, balance
+ * scales, twiddle tables, scatter schedules, partial spectra, inverse,
+ * decompression and recovery buffers) is carved; a call whose
+ * intermediates do not fit fails with CM_ENOMEM, naming the size it needs,
+ * and the plain path and the matrix-core modes (which allocate their own
+ * buffers) fail with CM_ENOTSUP.  Sketches are owned by their handles, not
+ * by the workspace. */
+int cm_context_create(int device, void* stream, size_t workspace_bytes, cm_context** out);
+/* Waits for the context's work and frees its memory and (if it made it) its
+ * stream.  Sketches made on it stay valid. */
+int cm_context_destroy(cm_context* ctx);
+/* cm_compress_ex, cm_decompress and cm_recover on a context: the same
+ * arguments and results (replacing the same reference functions), on the
+ * context's device and stream.  Compress from host operands, decompress
+ * into host memory and recover are synchronous; compress of device operands
+ * and decompress into device memory are ordered on the stream
+ * (cm_ctx_synchronize waits for them). */
+int cm_ctx_compress(cm_context* ctx, int variant, int dtype, int n_a, int n_inner, int n_b,
+                    const void* AT, int64_t ld_at, const void* B, int64_t ld_b, int d, int b,
+                    uint64_t seed, int family, int mem, unsigned flags, cm_sketch** inout);
+int cm_ctx_decompress(cm_context* ctx, const cm_sketch* sk, void* C, int64_t ldc, int mem);
+int cm_ctx_recover(cm_context* ctx, const cm_sketch* sk, double tau, int64_t max_out,
+                   int32_t* out_i, int32_t* out_j, double* out_val, int64_t* count);
+int cm_ctx_synchronize(cm_context* ctx);
+
 /* ---------------------------------------------------------------- memory
  * Device buffers for callers that keep inputs resident in HBM. */
 int cm_dev_alloc(size_t bytes, void** ptr);

@@ -64,6 +64,22 @@ def test_no_silent_cpu_fallback():
     A[...] = 1
     with pytest.raises(RuntimeError):
         mod.matmul_fft(C, A, A, 3, 32, 5)
+    with pytest.raises(cma.CMError):  # a context needs the device too
+        cma.Context(0)
+
+
+def test_context_entry_points_reject_null_handles():
+    # argument checks that run before any device work
+    L = cma.lib()
+    out = ctypes.c_void_p()
+    assert L.cm_context_create(0, None, 0, None) == cma.EINVAL
+    assert L.cm_context_destroy(None) == cma.OK
+    assert L.cm_ctx_synchronize(None) == cma.EINVAL
+    assert L.cm_ctx_decompress(None, None, None, 0, cma.HOST) == cma.EINVAL
+    cnt = ctypes.c_int64()
+    assert L.cm_ctx_recover(None, None, 1.0, 0, None, None, None, ctypes.byref(cnt)) == cma.EINVAL
+    assert L.cm_ctx_compress(None, 0, 1, 4, 4, 4, None, 4, None, 4, 3, 32, 1, 0, cma.HOST, 0,
+                             ctypes.byref(out)) == cma.EINVAL
 
 
 def test_dropin_validation_messages_without_gpu():
