@@ -512,7 +512,8 @@ class _Request(ctypes.Structure):
 class _ServerStats(ctypes.Structure):
     _fields_ = [("waves", ctypes.c_int64), ("requests_done", ctypes.c_int64),
                 ("requests_failed", ctypes.c_int64), ("max_waves_in_flight", ctypes.c_int),
-                ("hbm_bytes", ctypes.c_size_t), ("peak_bytes", ctypes.c_size_t)]
+                ("hbm_bytes", ctypes.c_size_t), ("peak_bytes", ctypes.c_size_t),
+                ("launch_s", ctypes.c_double), ("retire_s", ctypes.c_double)]
 
 
 DENSE, CSR = 0, 1

@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "host_hash.hpp"
+#include "recover_sort.hpp"
 #include "kernels.hpp"
 #include "runtime.hpp"
 
@@ -956,15 +957,11 @@ int recover_locked(const cm_sketch* sk, double tau, int64_t max_out, int32_t* ou
     CM_HIP(hipStreamSynchronize(cur().stream));
     for (int64_t q = 0; q < k; ++q) vv[q] = vf[q];
   }
-  std::vector<int64_t> order(k);
-  std::iota(order.begin(), order.end(), 0);
-  std::sort(order.begin(), order.end(), [&](int64_t x, int64_t y) {
-    return vi[x] != vi[y] ? vi[x] < vi[y] : vj[x] < vj[y];
-  });
+  cm::sort_recovered(k, sk->p, vi.data(), vj.data(), vv.data());
   for (int64_t q = 0; q < k; ++q) {
-    if (out_i) out_i[q] = vi[order[q]];
-    if (out_j) out_j[q] = vj[order[q]];
-    if (out_val) out_val[q] = vv[order[q]];
+    if (out_i) out_i[q] = vi[q];
+    if (out_j) out_j[q] = vj[q];
+    if (out_val) out_val[q] = vv[q];
   }
   return CM_OK;
 }

@@ -36,6 +36,7 @@
 
 #include "../../include/compmatmul_amd.h"
 #include "batch_internal.hpp"
+#include "recover_sort.hpp"
 #include "host_hash.hpp"
 #include "kernels.hpp"
 #include "runtime.hpp"
@@ -573,19 +574,32 @@ int build_plan(int count, const cm_product* ps, unsigned flags, int grain, const
     put(bt->comb_blocks, comb_blocks.data(), comb_blocks.size() * sizeof(int2));
     for (size_t c = 0; c < bt->classes.size(); ++c)
       put(bt->classes[c].blocks, cls_host[c]->data(), cls_host[c]->size() * sizeof(int2));
-    // stream-ordered uploads, complete when the calls return (pageable
-    // host memory), so the host images may go out of scope
-    hipError_t e = hipMemcpyAsync(cv.base + tables_begin, img.data(), img.size(),
-                                  hipMemcpyHostToDevice, bt->stream);
+    // stream-ordered uploads: from the caller's pinned staging when they
+    // fit (nothing to wait for; the caller keeps the staging until the
+    // stream has passed them), else from the pageable host images, waited
+    // for before they go out of scope
+    size_t upload = img.size();
+    for (int i = 0; i < count; ++i) {
+      if (hp.mul[i].size() > 256) upload += hp.mul[i].size() * 8;
+      upload += hp.tabs[i].size() * 4;
+    }
+    const bool staged = env.staging && upload <= env.staging_bytes;
+    size_t soff = 0;
+    auto copy_in = [&](void* dst, const void* src, size_t bytes) {
+      if (!staged) return hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, bt->stream);
+      std::memcpy(env.staging + soff, src, bytes);
+      const hipError_t r = hipMemcpyAsync(dst, env.staging + soff, bytes, hipMemcpyHostToDevice, bt->stream);
+      soff += bytes;
+      return r;
+    };
+    hipError_t e = copy_in(cv.base + tables_begin, img.data(), img.size());
     for (int i = 0; i < count && e == hipSuccess; ++i) {
       if (hp.mul[i].size() > 256)
-        e = hipMemcpyAsync(const_cast<uint64_t*>(mulp[i]), hp.mul[i].data(), hp.mul[i].size() * 8,
-                           hipMemcpyHostToDevice, bt->stream);
+        e = copy_in(const_cast<uint64_t*>(mulp[i]), hp.mul[i].data(), hp.mul[i].size() * 8);
       if (e == hipSuccess && !hp.tabs[i].empty())
-        e = hipMemcpyAsync(const_cast<uint32_t*>(tabp[i]), hp.tabs[i].data(), hp.tabs[i].size() * 4,
-                           hipMemcpyHostToDevice, bt->stream);
+        e = copy_in(const_cast<uint32_t*>(tabp[i]), hp.tabs[i].data(), hp.tabs[i].size() * 4);
     }
-    if (e == hipSuccess) e = hipStreamSynchronize(bt->stream);
+    if (e == hipSuccess && !staged) e = hipStreamSynchronize(bt->stream);
     if (e != hipSuccess) {
       if (bt->owns_arena) (void)hipFree(bt->arena);
       delete bt;
@@ -741,6 +755,10 @@ ProductView batch_product(const cm_batch* bt, int i) {
   return ProductView{P.counter, P.oi, P.oj, P.ov};
 }
 
+const unsigned long long* batch_counters(const cm_batch* bt) {
+  return reinterpret_cast<const unsigned long long*>(bt->counters + 8 * kKinds);
+}
+
 void batch_destroy(cm_batch* bt) {
   if (!bt) return;
   if (bt->arena && bt->owns_arena) (void)hipFree(bt->arena);
@@ -804,15 +822,11 @@ int cm_batch_recovered(cm_batch* batch, int i, int64_t max_out, int32_t* out_i, 
       hipMemcpy(vj.data(), P.oj, k * 4, hipMemcpyDeviceToHost) != hipSuccess ||
       hipMemcpy(vv.data(), P.ov, k * 4, hipMemcpyDeviceToHost) != hipSuccess)
     return error(CM_EDEVICE, "reading the recovered list failed");
-  std::vector<int64_t> order(k);
-  std::iota(order.begin(), order.end(), 0);
-  std::sort(order.begin(), order.end(), [&](int64_t x, int64_t y) {
-    return vi[x] != vi[y] ? vi[x] < vi[y] : vj[x] < vj[y];
-  });
+  cm::sort_recovered(k, q.p, vi.data(), vj.data(), vv.data());
   for (int64_t t = 0; t < k; ++t) {
-    if (out_i) out_i[t] = vi[order[t]];
-    if (out_j) out_j[t] = vj[order[t]];
-    if (out_val) out_val[t] = vv[order[t]];
+    if (out_i) out_i[t] = vi[t];
+    if (out_j) out_j[t] = vj[t];
+    if (out_val) out_val[t] = vv[t];
   }
   return CM_OK;
 }

@@ -34,6 +34,7 @@
 
 #include "../../include/compmatmul_amd.h"
 #include "batch_internal.hpp"
+#include "recover_sort.hpp"
 #include "kernels.hpp"
 #include "runtime.hpp"
 
@@ -126,6 +127,12 @@ struct Request {
   int64_t found = 0;
 };
 
+// Pinned host memory of a slot: the planner's uploads (reused once the
+// slot's wave has retired, so its stream has passed them), the wave's
+// recovery counters (copied on the wave's stream before its completion
+// event) and the recovered lists read back at retirement.
+constexpr size_t kStagingBytes = size_t(16) << 20;
+
 struct Slot {
   hipStream_t s = nullptr;
   hipEvent_t done = nullptr;
@@ -134,6 +141,10 @@ struct Slot {
   std::vector<float*> dev_C;  // per request, device C or nullptr
   cm_batch* bt = nullptr;
   size_t off = 0;
+  char* staging = nullptr;                // kStagingBytes
+  unsigned long long* counts = nullptr;   // max_wave recovery counters
+  char* lists = nullptr;                  // recovered lists, grown on demand
+  size_t lists_bytes = 0;
 };
 
 }  // namespace
@@ -227,50 +238,64 @@ void finish_request(cm_server* sv, Request* q, int rc, const std::string& err) {
   if (q->release_when_done) sv->reqs.erase(q->ticket);
 }
 
-// read back and retire a finished wave (dispatcher thread)
+// read back and retire a finished wave (dispatcher thread): the counters
+// are already in the slot's pinned memory; every list of the wave is copied
+// behind them with one wait
 void retire(cm_server* sv, Slot& w) {
+  const double t0 = now_s();
   std::vector<int> rcs(w.reqs.size(), CM_OK);
   std::vector<std::string> errs(w.reqs.size());
   {
     std::lock_guard<std::recursive_mutex> lk(cm::rt::mutex());
+    std::vector<int64_t> keep(w.reqs.size(), 0), at(w.reqs.size(), 0);
+    size_t need = 0;
+    for (size_t k = 0; k < w.reqs.size(); ++k) {
+      Request* q = w.reqs[k];
+      if (std::isnan(q->r.tau)) continue;
+      q->found = int64_t(w.counts[k]);
+      keep[k] = std::min<int64_t>(q->found, q->r.max_out);
+      at[k] = int64_t(need);
+      need += size_t(keep[k]) * 12;
+    }
+    hipError_t e = hipSuccess;
+    if (need > w.lists_bytes) {
+      if (w.lists) (void)hipHostFree(w.lists);
+      w.lists = nullptr;
+      w.lists_bytes = 0;
+      e = hipHostMalloc(reinterpret_cast<void**>(&w.lists), std::max<size_t>(need, size_t(1) << 20),
+                        hipHostMallocDefault);
+      if (e == hipSuccess) w.lists_bytes = std::max<size_t>(need, size_t(1) << 20);
+    }
+    bool copies = false;
+    for (size_t k = 0; k < w.reqs.size() && e == hipSuccess; ++k) {
+      if (keep[k] == 0) continue;
+      const cm::ProductView v = cm::batch_product(w.bt, int(k));
+      char* dst = w.lists + at[k];
+      e = hipMemcpyAsync(dst, v.oi, keep[k] * 4, hipMemcpyDeviceToHost, w.s);
+      if (e == hipSuccess) e = hipMemcpyAsync(dst + keep[k] * 4, v.oj, keep[k] * 4, hipMemcpyDeviceToHost, w.s);
+      if (e == hipSuccess) e = hipMemcpyAsync(dst + keep[k] * 8, v.ov, keep[k] * 4, hipMemcpyDeviceToHost, w.s);
+      copies = true;
+    }
+    if (e == hipSuccess && copies) e = hipStreamSynchronize(w.s);
     for (size_t k = 0; k < w.reqs.size(); ++k) {
       Request* q = w.reqs[k];
       if (!std::isnan(q->r.tau)) {
-        const cm::ProductView v = cm::batch_product(w.bt, int(k));
-        unsigned long long found = 0;
-        hipError_t e = hipMemcpyAsync(&found, v.counter, 8, hipMemcpyDeviceToHost, w.s);
-        if (e == hipSuccess) e = hipStreamSynchronize(w.s);
-        const int64_t keep = std::min<int64_t>(int64_t(found), q->r.max_out);
-        q->found = int64_t(found);
-        q->oi.resize(keep);
-        q->oj.resize(keep);
-        q->ov.resize(keep);
-        if (e == hipSuccess && keep > 0) {
-          e = hipMemcpyAsync(q->oi.data(), v.oi, keep * 4, hipMemcpyDeviceToHost, w.s);
-          if (e == hipSuccess) e = hipMemcpyAsync(q->oj.data(), v.oj, keep * 4, hipMemcpyDeviceToHost, w.s);
-          if (e == hipSuccess) e = hipMemcpyAsync(q->ov.data(), v.ov, keep * 4, hipMemcpyDeviceToHost, w.s);
-          if (e == hipSuccess) e = hipStreamSynchronize(w.s);
-        }
         if (e != hipSuccess) {
           rcs[k] = CM_EDEVICE;
           errs[k] = std::string("reading results: ") + hipGetErrorString(e);
           continue;
         }
-        std::vector<int64_t> order(keep);
-        std::iota(order.begin(), order.end(), 0);
-        std::sort(order.begin(), order.end(), [&](int64_t a, int64_t b) {
-          return q->oi[a] != q->oi[b] ? q->oi[a] < q->oi[b] : q->oj[a] < q->oj[b];
-        });
-        std::vector<int32_t> si(keep), sj(keep);
-        std::vector<float> sv_(keep);
-        for (int64_t t = 0; t < keep; ++t) {
-          si[t] = q->oi[order[t]];
-          sj[t] = q->oj[order[t]];
-          sv_[t] = q->ov[order[t]];
+        const int64_t kk = keep[k];
+        q->oi.resize(kk);
+        q->oj.resize(kk);
+        q->ov.resize(kk);
+        if (kk > 0) {
+          const char* src = w.lists + at[k];
+          std::memcpy(q->oi.data(), src, kk * 4);
+          std::memcpy(q->oj.data(), src + kk * 4, kk * 4);
+          std::memcpy(q->ov.data(), src + kk * 8, kk * 4);
         }
-        q->oi.swap(si);
-        q->oj.swap(sj);
-        q->ov.swap(sv_);
+        cm::sort_recovered(kk, q->r.p, q->oi.data(), q->oj.data(), q->ov.data());
       }
       if (q->r.C && w.dev_C[k] && q->r.m > 0 && q->r.p > 0) {
         hipError_t e = hipMemcpyAsync(q->r.C, w.dev_C[k], size_t(q->r.m) * q->r.p * 4,
@@ -286,6 +311,7 @@ void retire(cm_server* sv, Slot& w) {
     w.bt = nullptr;
   }
   std::lock_guard<std::mutex> lk(sv->mu);
+  sv->stats.retire_s += now_s() - t0;
   for (size_t k = 0; k < w.reqs.size(); ++k) finish_request(sv, w.reqs[k], rcs[k], errs[k]);
   sv->pool->release(w.off);
   w.reqs.clear();
@@ -297,6 +323,7 @@ void retire(cm_server* sv, Slot& w) {
 // admit, upload, plan and launch one wave on slot w (dispatcher thread);
 // returns false when nothing was launched
 bool launch_wave(cm_server* sv, Slot& w) {
+  const double t0 = now_s();
   std::vector<Request*> take;
   size_t wave_off = 0, wave_bytes = 0;
   {
@@ -429,8 +456,15 @@ bool launch_wave(cm_server* sv, Slot& w) {
     env.arena = cur;
     env.arena_bytes = wave_bytes > used ? wave_bytes - used : 0;
     env.keep = keep.data();
+    env.staging = w.staging;
+    env.staging_bytes = kStagingBytes;
     rc = cm::batch_plan_env(int(prods.size()), prods.data(), 0, sv->grain, env, &w.bt);
     if (rc == CM_OK) rc = cm::batch_enqueue(w.bt);
+    // the recovery counters follow the wave on its stream, so they are on
+    // the host when its completion event fires
+    if (rc == CM_OK && hipMemcpyAsync(w.counts, cm::batch_counters(w.bt), prods.size() * 8,
+                                      hipMemcpyDeviceToHost, w.s) != hipSuccess)
+      rc = error(CM_EDEVICE, "counter read-back failed");
     if (rc == CM_OK && hipEventRecord(w.done, w.s) != hipSuccess) rc = error(CM_EDEVICE, "event record failed");
     if (rc) err = cm_last_error();
   }
@@ -448,6 +482,7 @@ bool launch_wave(cm_server* sv, Slot& w) {
   w.off = wave_off;
   w.busy = true;
   std::lock_guard<std::mutex> lk2(sv->mu);
+  sv->stats.launch_s += now_s() - t0;
   ++sv->stats.waves;
   int inflight = 0;
   for (const Slot& s : sv->slots) inflight += s.busy;
@@ -490,6 +525,19 @@ void dispatcher(cm_server* sv) {
   sv->cv_done.notify_all();
 }
 
+// a slot's stream (after its work), event and pinned buffers
+void release_slot(Slot& w) {
+  if (w.s) {
+    (void)hipStreamSynchronize(w.s);
+    (void)hipStreamDestroy(w.s);
+  }
+  if (w.done) (void)hipEventDestroy(w.done);
+  if (w.staging) (void)hipHostFree(w.staging);
+  if (w.counts) (void)hipHostFree(w.counts);
+  if (w.lists) (void)hipHostFree(w.lists);
+  w = Slot{};
+}
+
 Request* find(cm_server* sv, uint64_t ticket) {
   auto it = sv->reqs.find(ticket);
   return it == sv->reqs.end() ? nullptr : it->second.get();
@@ -524,13 +572,13 @@ int cm_server_create(size_t hbm_bytes, int max_waves, int max_wave_products, int
   sv->slots.resize(max_waves);
   for (Slot& w : sv->slots) {
     if (hipStreamCreateWithFlags(&w.s, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreateWithFlags(&w.done, hipEventDisableTiming) != hipSuccess) {
-      for (Slot& x : sv->slots) {
-        if (x.s) (void)hipStreamDestroy(x.s);
-        if (x.done) (void)hipEventDestroy(x.done);
-      }
+        hipEventCreateWithFlags(&w.done, hipEventDisableTiming) != hipSuccess ||
+        hipHostMalloc(reinterpret_cast<void**>(&w.staging), kStagingBytes, hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc(reinterpret_cast<void**>(&w.counts), size_t(max_wave_products) * 8,
+                      hipHostMallocDefault) != hipSuccess) {
+      for (Slot& x : sv->slots) release_slot(x);
       (void)hipFree(sv->base);
-      return error(CM_EDEVICE, "stream or event creation failed");
+      return error(CM_EDEVICE, "stream, event or pinned staging creation failed");
     }
   }
   sv->stats.hbm_bytes = sv->budget;
@@ -717,13 +765,7 @@ int cm_server_destroy(cm_server* sv) {
   if (sv->thread.joinable()) sv->thread.join();
   {
     std::lock_guard<std::recursive_mutex> lk(cm::rt::mutex());
-    for (Slot& w : sv->slots) {
-      if (w.s) {
-        (void)hipStreamSynchronize(w.s);
-        (void)hipStreamDestroy(w.s);
-      }
-      if (w.done) (void)hipEventDestroy(w.done);
-    }
+    for (Slot& w : sv->slots) release_slot(w);
     if (sv->base) (void)hipFree(sv->base);
   }
   delete sv;

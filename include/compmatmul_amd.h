@@ -8,8 +8,10 @@
  * Notation follows the reference: m = n_a (rows of C), n = n_inner,
  * p = n_b (columns of C).  AT is n x m row-major (A passed transposed),
  * B is n x p row-major, C is m x p row-major (compmatmul.cpp:55-63).
- * The library runs everything on one HIP stream per device; calls are
- * synchronous with respect to the host unless stated otherwise.
+ * Calls without a context argument run on the library's default context
+ * (one HIP stream on device 0, or cm_init's device); contexts (below) have
+ * their own device, stream and memory.  Calls are synchronous with respect
+ * to the host unless stated otherwise.
  */
 #ifndef COMPMATMUL_AMD_H
 #define COMPMATMUL_AMD_H
@@ -364,6 +366,9 @@ typedef struct cm_server_stats {
   int64_t waves, requests_done, requests_failed;
   int max_waves_in_flight;
   size_t hbm_bytes, peak_bytes;
+  /* dispatcher thread seconds: admitting, uploading, planning and launching
+   * waves; reading back and retiring them */
+  double launch_s, retire_s;
 } cm_server_stats;
 int cm_server_stats_get(cm_server* server, cm_server_stats* out);
 /* Finish the waves in flight, fail the queued requests, free everything. */
