@@ -43,62 +43,6 @@ struct LdsCfg {
   static constexpr int MAXR = (1 << LB2MAX) / NT;
 };
 
-// in-LDS natural-order (Sylvester) Walsh-Hadamard transform of two vectors
-template <typename T>
-__device__ void lds_wht2(T* sa, T* sb, int b2, int tid) {
-  for (int len = 1; len < b2; len <<= 1) {
-    for (int q = tid; q < (b2 >> 1); q += NT) {
-      const int j = ((q & ~(len - 1)) << 1) | (q & (len - 1));
-      const T x0 = sa[j], x1 = sa[j + len];
-      sa[j] = x0 + x1;
-      sa[j + len] = x0 - x1;
-      if (sb) {
-        const T y0 = sb[j], y1 = sb[j + len];
-        sb[j] = y0 + y1;
-        sb[j + len] = y0 - y1;
-      }
-    }
-    __syncthreads();
-  }
-}
-
-// As lds_dif, with the twiddles from an LDS table tws[k] = W_b2^k, k < b2/2
-// (the same values: W_b^(pos b / 2len) = W_b2^(pos b2 / 2len)), so no stage
-// waits on a global load per butterfly.
-template <typename T, bool CONJ>
-__device__ void lds_dif_tab(cpx<T>* z, const cpx<T>* tws, int b2, int lb2, int tid) {
-  for (int len = b2 >> 1; len >= 1; len >>= 1) {
-    const int lstride = lb2 - __ffs(len);  // log2(b2 / (2 len))
-    for (int q = tid; q < (b2 >> 1); q += NT) {
-      const int pos = q & (len - 1);
-      const int j = ((q & ~(len - 1)) << 1) | pos;
-      const cpx<T> x0 = z[j], x1 = z[j + len];
-      const cpx<T> w = tws[pos << lstride];
-      z[j] = cadd(x0, x1);
-      z[j + len] = CONJ ? cmulc(csub(x0, x1), w) : cmul(csub(x0, x1), w);
-    }
-    __syncthreads();
-  }
-}
-
-// in-LDS radix-2 decimation-in-frequency DFT of length b2 (natural order in,
-// bit-reversed order out).  tw holds W_b^j for j < b; conj selects W^-1.
-template <typename T, bool CONJ>
-__device__ void lds_dif(cpx<T>* z, int b2, int lb, const cpx<T>* __restrict__ tw, int tid) {
-  for (int len = b2 >> 1; len >= 1; len >>= 1) {
-    const int lstride = lb - __ffs(len);  // log2(b / (2 len))
-    for (int q = tid; q < (b2 >> 1); q += NT) {
-      const int pos = q & (len - 1);
-      const int j = ((q & ~(len - 1)) << 1) | pos;
-      const cpx<T> x0 = z[j], x1 = z[j + len];
-      const cpx<T> w = tw[int64_t(pos) << lstride];
-      z[j] = cadd(x0, x1);
-      z[j + len] = CONJ ? cmulc(csub(x0, x1), w) : cmul(csub(x0, x1), w);
-    }
-    __syncthreads();
-  }
-}
-
 // sum of the per-chunk partial spectra, written over chunk 0 (each thread
 // owns one element and reads every chunk in order: deterministic)
 template <typename T, typename ACC>
@@ -159,8 +103,15 @@ __global__ void __launch_bounds__(NT) k_chunk_sum(T* __restrict__ P, int64_t ele
   chunk_sum_body<T, ACC>(P, elems, nchunks, blockIdx.x);
 }
 
+// Sum of the chunks of one (t, slice) spectrum, then its inverse transform
+// over the slice in LDS with the radix-2^r passes of lds_transform.hpp
+// (three or four passes and barriers where radix-2 stages took eleven or
+// twelve): the natural-order WHT, and for the FFT the inverse DFT as
+// conj(DFT(conj Q)), the conjugations being exact sign flips.  The slice is
+// written in natural order.
 template <typename T, typename ACC, bool FWHT>
 __device__ __forceinline__ void reduce_body(const ReduceArgs& a, int blk) {
+  using ldsx::pad;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int b2 = 1 << a.lb2;
   const int lbs = a.lb - a.lb2;
@@ -183,23 +134,25 @@ __device__ __forceinline__ void reduce_body(const ReduceArgs& a, int blk) {
         for (int u = 0; u < U; ++u) v[u] = e0 + u * NT < b2 ? P[off + e0 + u * NT] : T(0);
 #pragma unroll
         for (int u = 0; u < U; ++u)
-          if (e0 + u * NT < b2) s[e0 + u * NT] = v[u];
+          if (e0 + u * NT < b2) s[pad(e0 + u * NT)] = v[u];
       }
     } else {
       for (int e = tid; e < b2; e += NT) {
         ACC sum = ACC(0);
         for (int c = 0; c < a.nchunks; ++c) sum += ACC(P[c * chunk_stride + off + e]);
-        s[e] = T(sum);
+        s[pad(e)] = T(sum);
       }
     }
-    __syncthreads();
-    lds_wht2<T>(s, nullptr, b2, tid);
+    ldsx::lds_transform<NT, T, true>(s, a.lb2, 1, 0, nullptr, nullptr);
     T* V = static_cast<T*>(a.V) + off;
-    for (int e = tid; e < b2; e += NT) V[e] = s[e];
+    for (int e = tid; e < b2; e += NT) V[e] = s[pad(e)];
   } else {
+    constexpr int RM = sizeof(T) == 8 ? 3 : 4;  // radix 8 for FP64 (registers), 16 for FP32
     cpx<T>* z = reinterpret_cast<cpx<T>*>(smem_raw);
+    cpx<T>* lo = ldsx::twiddle_lds<T, false>(smem_raw, a.lb2);
+    cpx<T>* hi = lo + ldsx::TW_LO;
     const cpx<T>* __restrict__ P = static_cast<const cpx<T>*>(a.partial);
-    const cpx<T>* __restrict__ tw = static_cast<const cpx<T>*>(a.tw);
+    ldsx::fill_twiddles<NT, T>(lo, hi, a.lb2, static_cast<const cpx<T>*>(a.tw), a.lb);
     if (a.nchunks == 1) {
       for (int e0 = tid; e0 < b2; e0 += U * NT) {
         cpx<T> v[U];
@@ -207,7 +160,7 @@ __device__ __forceinline__ void reduce_body(const ReduceArgs& a, int blk) {
         for (int u = 0; u < U; ++u) v[u] = e0 + u * NT < b2 ? P[off + e0 + u * NT] : cpx<T>{T(0), T(0)};
 #pragma unroll
         for (int u = 0; u < U; ++u)
-          if (e0 + u * NT < b2) z[e0 + u * NT] = v[u];
+          if (e0 + u * NT < b2) z[pad(e0 + u * NT)] = cpx<T>{v[u].x, -v[u].y};
       }
     } else {
       for (int e = tid; e < b2; e += NT) {
@@ -217,20 +170,15 @@ __device__ __forceinline__ void reduce_body(const ReduceArgs& a, int blk) {
           sx += ACC(v.x);
           sy += ACC(v.y);
         }
-        z[e] = {T(sx), T(sy)};
+        z[pad(e)] = {T(sx), -T(sy)};
       }
     }
-    if (reduce_twiddle_table(sizeof(T), a.lb2)) {
-      cpx<T>* tws = z + b2;  // W_b2^k = W_b^(k b / b2), k < b2 / 2
-      for (int k = tid; k < (b2 >> 1); k += NT) tws[k] = tw[int64_t(k) << (a.lb - a.lb2)];
-      __syncthreads();
-      lds_dif_tab<T, true>(z, tws, b2, a.lb2, tid);
-    } else {
-      __syncthreads();
-      lds_dif<T, true>(z, b2, a.lb, tw, tid);
-    }
+    ldsx::lds_transform<NT, T, false, RM>(z, a.lb2, 1, 0, lo, hi);
     cpx<T>* V = static_cast<cpx<T>*>(a.V) + off;
-    for (int e = tid; e < b2; e += NT) V[bitrev(e, a.lb2)] = z[e];
+    for (int e = tid; e < b2; e += NT) {
+      const cpx<T> v = z[pad(e)];
+      V[ldsx::dif_nat(e, a.lb2, RM)] = cpx<T>{v.x, -v.y};
+    }
   }
 }
 
@@ -545,6 +493,11 @@ hipError_t reduce_t(const ReduceArgs& a0, hipStream_t s) {
   const int b1 = 1 << (a.lb - a.lb2);
   const size_t lds = reduce_lds_bytes(FWHT, sizeof(T), a.lb2);
   const unsigned grid = unsigned(a.d * b1);
+  // slices of 64 KB need a little more than the default 64 KB of LDS
+  const void* fn = wide ? reinterpret_cast<const void*>(&k_reduce_inverse<T, double, FWHT>)
+                        : reinterpret_cast<const void*>(&k_reduce_inverse<T, T, FWHT>);
+  if (lds > 65536 && hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)) != hipSuccess)
+    return hipErrorInvalidValue;
   if (wide)
     k_reduce_inverse<T, double, FWHT><<<dim3(grid), dim3(NT), lds, s>>>(a);
   else

@@ -195,17 +195,14 @@ hipError_t launch_combine(const CombineArgs& a, hipStream_t s);
 // the per-job counts from finish_batch_blocks() (returns the combine count).
 struct FinishJob { ReduceArgs r; CombineArgs c; };
 int finish_batch_blocks(const FinishJob& j, int* sum_blocks, int* red_blocks);
-// The FFT reduce keeps a half-length twiddle table W_b2^k, k < b2/2, in LDS
-// beside the complex slice when both fit the default 64 KB.
-__host__ __device__ inline bool reduce_twiddle_table(size_t real_bytes, int lb2) {
-  return (2 * real_bytes << lb2) + (lb2 > 0 ? 2 * real_bytes << (lb2 - 1) : 0) <= 65536;
-}
-// LDS bytes of one reduce workgroup: the slice (complex for FFT) and the
-// twiddle table where it is used.
+// LDS bytes of one reduce workgroup: the slice in a tile padded by one
+// element every 16 (complex for FFT, rounded to 16 bytes), and for FFT the
+// two-level twiddle tables of its length (64 + b2 / 64 + 1 entries).
 inline size_t reduce_lds_bytes(bool fwht, size_t real_bytes, int lb2) {
-  if (fwht) return real_bytes << lb2;
-  const size_t z = 2 * real_bytes << lb2;
-  return reduce_twiddle_table(real_bytes, lb2) && lb2 > 0 ? z + (2 * real_bytes << (lb2 - 1)) : z;
+  const size_t b2 = size_t(1) << lb2;
+  const size_t elem = fwht ? real_bytes : 2 * real_bytes;
+  const size_t tile = ((b2 + b2 / 16) * elem + 15) & ~size_t(15);
+  return fwht ? tile : tile + (64 + b2 / 64 + 1) * elem;
 }
 
 hipError_t launch_finish_batch(const FinishJob* jobs, const int2* sum_blocks, int n_sum,
