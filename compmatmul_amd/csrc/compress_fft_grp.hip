@@ -489,7 +489,9 @@ __device__ __forceinline__ void fft_grp_unit(const RegJob& j, int u, float2* S, 
 #pragma unroll
           for (int q = 0; q < SEG / (64 * G); ++q) {
             const int key = base0 + min(q * 64 * G + tid, sl - 1);
-            v[q] = key < j.m ? ar[key] * bal.x : br[key - j.m] * bal.y;
+            const bool from_a = key < j.m;  // one load, from a selected address
+            const float* src = from_a ? ar + key : br + (key - j.m);
+            v[q] = *src * (from_a ? bal.x : bal.y);
           }
         }
         __syncthreads();  // every wave is done with the previous segment

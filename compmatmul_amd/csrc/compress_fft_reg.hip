@@ -181,7 +181,11 @@ __device__ __forceinline__ void stage_rows(const float* __restrict__ ar,
 #pragma unroll
   for (int q = 0; q < SEG / 64; ++q) {
     const int key = s0 + min(q * 64 + lane, sl - 1);
-    v[q] = key < m ? ar[key] * bal.x : br[key - m] * bal.y;
+    // one load from a selected address and a selected scale: a select of
+    // two scaled loads compiles to a divergent branch per load
+    const bool from_a = key < m;
+    const float* src = from_a ? ar + key : br + (key - m);
+    v[q] = *src * (from_a ? bal.x : bal.y);
   }
 #pragma unroll
   for (int q = 0; q < SEG / 64; ++q) Rb[q * 64 + lane] = v[q];
