@@ -49,6 +49,8 @@ PEAK_HBM_GBS = 8000.0
 COMPUTE_BOUND = "valu"
 PEAK_NOTE = ("FP32 vector ALU: 157.3 TFLOP/s is the packed-FMA peak; with packed FP32 "
              "disabled, add-only work peaks at 39.3 TFLOP/s")
+PEAK_NOTE_FP64 = ("FP64 vector ALU: 78.6 TFLOP/s is the FMA peak; add-only FP64 work "
+                  "peaks at 39.3 TFLOP/s")
 
 
 def compress_flops(variant, m, n, p, d, b):
@@ -625,7 +627,9 @@ def bench_single(args, name, plain=False, quiet=False):
         k_ms, k_n = totals["compress"]
         k_avg_s = (k_ms / max(1, k_n)) * 1e-3
         achieved = flops / k_avg_s / 1e12 if k_avg_s > 0 else 0.0
-        roof = {"bound": COMPUTE_BOUND, "peak_note": PEAK_NOTE, "achieved": achieved, "peak": peak,
+        roof = {"bound": COMPUTE_BOUND,
+                "peak_note": PEAK_NOTE if dtype == np.float32 else PEAK_NOTE_FP64,
+                "achieved": achieved, "peak": peak,
                 "unit": "TFLOP/s", "frac": achieved / peak,
                 "traffic": traffic_from_profile(name),
                 "traffic_source": "profile constant: rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE "
