@@ -298,15 +298,22 @@ __global__ void __launch_bounds__(NT) k_combine_batch(const FinishJob* __restric
 // three LDS passes; two workgroups (LDS tiles) share a CU
 constexpr int LT = 256;
 constexpr int KU = 8;  // scatter keys per thread with loads in flight together
+constexpr int KH = 4;  // keys per thread held in registers across k (m + p <= KH LT)
 template <typename T>
 constexpr int fft_rm() { return 4; }
+
+// log2 of the radix of the transform passes (the last pass may be narrower)
+template <typename T, bool FWHT>
+constexpr int lds2_rm() { return FWHT ? ldsx::rmax<T, true>() : fft_rm<T>(); }
 
 template <typename T, bool FWHT>
 __host__ __device__ __forceinline__ size_t lds2_bytes(int lb2) {
   return FWHT ? ldsx::tile_bytes<T, true>(lb2 + 1) : ldsx::transform_bytes<T, false>(lb2);
 }
 
-template <typename T, bool FWHT>
+// RL: log2 of the radix of the last transform pass, last_radix_log2(lb2, RM)
+// (one instantiation each: a runtime choice among them costs registers)
+template <typename T, bool FWHT, int RL>
 __global__ void __launch_bounds__(LT) __attribute__((amdgpu_waves_per_eu(2, 2))) k_compress_lds2(CompressArgs a) {
   using namespace ldsx;
   using E = Elem<T, FWHT>;
@@ -323,143 +330,238 @@ __global__ void __launch_bounds__(LT) __attribute__((amdgpu_waves_per_eu(2, 2)))
   const int k0 = chunk * a.kper;
   const int k1 = min(a.n, k0 + a.kper);
   const int tid = threadIdx.x;
+  const int keys = a.m + a.p;  // keys [0, m) of A then [0, p) of B
   const T* __restrict__ AT = static_cast<const T*>(a.AT);
   const T* __restrict__ B = static_cast<const T*>(a.B);
   const uint32_t* __restrict__ hA = a.hA + int64_t(t) * a.m;
   const uint32_t* __restrict__ hB = a.hB + int64_t(t) * a.p;
   const int64_t out_off = ((int64_t(chunk) * a.d + t) * b1 + slice) * b2;
+  // hash entry and row-k value of a key below `keys`, each one load from a
+  // selected address (a select of two loads compiles to a branch per load)
+  auto key_entry = [&](int key) -> uint32_t {
+    const uint32_t* src = key < a.m ? hA + key : hB + (key - a.m);
+    return *src;
+  };
+  auto row_value = [&](int k, int key) -> T {
+    const T* src = key < a.m ? AT + int64_t(k) * a.ld_at + key : B + int64_t(k) * a.ld_b + (key - a.m);
+    return *src;
+  };
+  // When all keys fit KH per thread (uniform), their hash entries stay in
+  // registers for the whole chunk and the row values of k + 1 are loaded
+  // while k is scattered and transformed.  Key u LT + tid is read at a
+  // clamped index, so every load is in bounds.
+  const int nu = (keys + LT - 1) / LT;
+  const bool held = nu <= KH;
+  uint32_t eh[KH];
+  T vn[KH];
+#pragma unroll
+  for (int u = 0; u < KH; ++u) {
+    eh[u] = 0u;
+    vn[u] = T(0);
+    if (held && u < nu) eh[u] = key_entry(min(u * LT + tid, keys - 1));
+  }
+  auto load_row = [&](int k) {
+#pragma unroll
+    for (int u = 0; u < KH; ++u)
+      if (u < nu) vn[u] = row_value(k, min(u * LT + tid, keys - 1));
+  };
   E acc[PER];
 #pragma unroll
   for (int r = 0; r < PER; ++r) acc[r] = E{};
+  // The tile is cleared once; afterwards each k's accumulation clears the
+  // elements it reads (the scatter and the transform touch no padding slot).
   if constexpr (FWHT) {
     // A's sketch at elements [0, b2), B's at [b2, 2 b2), one padded tile
     T* sa = reinterpret_cast<T*>(smem_raw);
+    constexpr int R = 1 << RL;  // the last pass runs in registers on R elements per group
     const int wwords = int(tile_bytes<T, true>(lb2 + 1) / 16);
-    for (int k = k0; k < k1; ++k) {
-      __syncthreads();  // the previous k's products are read
-      for (int w = tid; w < wwords; w += LT) reinterpret_cast<float4*>(smem_raw)[w] = make_float4(0.f, 0.f, 0.f, 0.f);
-      __syncthreads();
-      const T* __restrict__ ar = AT + int64_t(k) * a.ld_at;
-      const T* __restrict__ br = B + int64_t(k) * a.ld_b;
-      // keys [0, m) of A then [0, p) of B, KU per thread with all their
-      // loads in flight before the LDS adds
-      for (int base = 0; base < a.m + a.p; base += LT * KU) {
-        uint32_t e[KU];
-        T v[KU];
+    for (int w = tid; w < wwords; w += LT) reinterpret_cast<float4*>(smem_raw)[w] = make_float4(0.f, 0.f, 0.f, 0.f);
+    auto scatter = [&](const uint32_t(&e)[KU], const T(&v)[KU], int base) {
 #pragma unroll
-        for (int u = 0; u < KU; ++u) {
-          const int key = base + u * LT + tid;
-          e[u] = 0u;
-          v[u] = T(0);
-          if (key < a.m) {
-            e[u] = hA[key];
-            v[u] = ar[key];
-          } else if (key < a.m + a.p) {
-            e[u] = hB[key - a.m];
-            v[u] = br[key - a.m];
-          }
-        }
-#pragma unroll
-        for (int u = 0; u < KU; ++u) {
-          const int key = base + u * LT + tid;
-          if (key < a.m + a.p) {
-            const uint32_t h = hbucket(e[u]);
-            const uint32_t neg = hneg(e[u]) ^ (__popc((h >> lb2) & slice) & 1);
-            atomicAdd(&sa[pad((key < a.m ? 0 : b2) + int(h & (b2 - 1)))], apply_sign(v[u], neg));
-          }
+      for (int u = 0; u < KU; ++u) {
+        const int key = base + u * LT + tid;
+        if (key < keys) {
+          const uint32_t h = hbucket(e[u]);
+          const uint32_t neg = hneg(e[u]) ^ (__popc((h >> lb2) & slice) & 1);
+          atomicAdd(&sa[pad((key < a.m ? 0 : b2) + int(h & (b2 - 1)))], apply_sign(v[u], neg));
         }
       }
-      lds_transform<LT, T, true>(sa, lb2, 2, b2, nullptr, nullptr);  // both sketches
+    };
+    if (held && k0 < k1) load_row(k0);
+    for (int k = k0; k < k1; ++k) {
+      __syncthreads();  // the previous k's products are read and their elements cleared
+      // KU keys per thread with all their loads in flight before the LDS adds
+      // (held keys: one round, from registers)
+      for (int base = 0; base < keys; base += LT * KU) {
+        uint32_t e[KU];
+        T v[KU];
+        if (held) {
 #pragma unroll
-      for (int r = 0; r < PER; ++r) {
-        const int e = tid + r * LT;
-        if (e < b2) acc[r] += sa[pad(e)] * sa[pad(b2 + e)];
+          for (int u = 0; u < KU; ++u) {
+            e[u] = u < KH ? eh[u] : 0u;
+            v[u] = u < KH ? vn[u] : T(0);
+          }
+          if (k + 1 < k1) load_row(k + 1);
+        } else {
+#pragma unroll
+          for (int u = 0; u < KU; ++u) {
+            const int key = base + u * LT + tid;
+            e[u] = 0u;
+            v[u] = T(0);
+            if (key < keys) {
+              e[u] = key_entry(key);
+              v[u] = row_value(k, key);
+            }
+          }
+        }
+        scatter(e, v, base);
+      }
+      lds_transform_head<LT, T, true>(sa, lb2, 2, b2, nullptr, nullptr);  // both sketches
+#pragma unroll
+      for (int i = 0; i < PER / R; ++i) {  // group g = tid + i LT holds elements g R + j
+        const int g = tid + i * LT;
+        if (g < (b2 >> RL)) {
+          T va[R], vb[R];
+#pragma unroll
+          for (int j = 0; j < R; ++j) {
+            va[j] = sa[pad(g * R + j)];
+            vb[j] = sa[pad(b2 + g * R + j)];
+          }
+          last_pass<R, T, true>(va);
+          last_pass<R, T, true>(vb);
+#pragma unroll
+          for (int j = 0; j < R; ++j) {
+            acc[i * R + j] += va[j] * vb[j];
+            sa[pad(g * R + j)] = T(0);
+            sa[pad(b2 + g * R + j)] = T(0);
+          }
+        }
       }
     }
     T* out = static_cast<T*>(a.partial) + out_off;
 #pragma unroll
-    for (int r = 0; r < PER; ++r) {
-      const int e = tid + r * LT;
-      if (e < b2) out[e] = acc[r];
+    for (int i = 0; i < PER / R; ++i) {
+      const int g = tid + i * LT;
+      if (g < (b2 >> RL)) {
+#pragma unroll
+        for (int j = 0; j < R; ++j) out[g * R + j] = acc[i * R + j];
+      }
     }
   } else {
     constexpr int FRM = fft_rm<T>();
+    constexpr int R = 1 << RL;  // the last pass runs in registers on R elements per group
     cpx<T>* z = reinterpret_cast<cpx<T>*>(smem_raw);
     cpx<T>* lo = twiddle_lds<T, false>(smem_raw, lb2);
     cpx<T>* hi = lo + TW_LO;
     const cpx<T>* __restrict__ tw = static_cast<const cpx<T>*>(a.tw);
     fill_twiddles<LT, T>(lo, hi, lb2, tw, a.lb);
+    for (int w = tid; w < int(tile_bytes<T, false>(lb2) / 16); w += LT)
+      reinterpret_cast<float4*>(smem_raw)[w] = make_float4(0.f, 0.f, 0.f, 0.f);
     const uint64_t bmask = (uint64_t(1) << a.lb) - 1;
     const cpx<T>* __restrict__ bal = static_cast<const cpx<T>*>(a.scale);
-    for (int k = k0; k < k1; ++k) {
-      __syncthreads();  // the previous k's squares are read
-      for (int w = tid; w < int(tile_bytes<T, false>(lb2) / 16); w += LT)
-        reinterpret_cast<float4*>(smem_raw)[w] = make_float4(0.f, 0.f, 0.f, 0.f);
-      __syncthreads();
-      const T* __restrict__ ar = AT + int64_t(k) * a.ld_at;
-      const T* __restrict__ br = B + int64_t(k) * a.ld_b;
-      const cpx<T> bk = bal[k];  // power-of-two balance of row k (balance.hip)
-      // keys [0, m) of A then [0, p) of B, KU per thread with all their
-      // loads in flight before the LDS adds
-      for (int base = 0; base < a.m + a.p; base += LT * KU) {
-        uint32_t e[KU];
-        T v[KU];
+    // v: balanced values (power-of-two scales of row k, balance.hip)
+    auto scatter = [&](const uint32_t(&e)[KU], const T(&v)[KU], int base) {
+      if (slice == 0) {  // modulation W_b^0 = 1: A adds to Re, B to Im only
 #pragma unroll
         for (int u = 0; u < KU; ++u) {
           const int key = base + u * LT + tid;
-          e[u] = 0u;
-          v[u] = T(0);
-          if (key < a.m) {
-            e[u] = hA[key];
-            v[u] = ar[key] * bk.x;
-          } else if (key < a.m + a.p) {
-            e[u] = hB[key - a.m];
-            v[u] = br[key - a.m] * bk.y;
+          if (key < keys) {
+            cpx<T>* dst = &z[pad(hbucket(e[u]) & (b2 - 1))];
+            atomicAdd(key < a.m ? &dst->x : &dst->y, apply_sign(v[u], hneg(e[u])));
           }
         }
-        if (slice == 0) {  // modulation W_b^0 = 1: A adds to Re, B to Im only
+      } else {
+        cpx<T> w[KU];
 #pragma unroll
-          for (int u = 0; u < KU; ++u) {
-            const int key = base + u * LT + tid;
-            if (key < a.m + a.p) {
-              cpx<T>* dst = &z[pad(hbucket(e[u]) & (b2 - 1))];
-              atomicAdd(key < a.m ? &dst->x : &dst->y, apply_sign(v[u], hneg(e[u])));
-            }
-          }
-        } else {
-          cpx<T> w[KU];
+        for (int u = 0; u < KU; ++u) w[u] = tw[(uint64_t(hbucket(e[u])) * uint64_t(slice)) & bmask];
 #pragma unroll
-          for (int u = 0; u < KU; ++u) w[u] = tw[(uint64_t(hbucket(e[u])) * uint64_t(slice)) & bmask];
-#pragma unroll
-          for (int u = 0; u < KU; ++u) {
-            const int key = base + u * LT + tid;
-            if (key < a.m + a.p) {
-              const T x = apply_sign(v[u], hneg(e[u]));
-              cpx<T>* dst = &z[pad(hbucket(e[u]) & (b2 - 1))];
-              if (key < a.m) {
-                atomicAdd(&dst->x, x * w[u].x);
-                atomicAdd(&dst->y, x * w[u].y);
-              } else {  // i x w
-                atomicAdd(&dst->x, -x * w[u].y);
-                atomicAdd(&dst->y, x * w[u].x);
-              }
+        for (int u = 0; u < KU; ++u) {
+          const int key = base + u * LT + tid;
+          if (key < keys) {
+            const T x = apply_sign(v[u], hneg(e[u]));
+            cpx<T>* dst = &z[pad(hbucket(e[u]) & (b2 - 1))];
+            if (key < a.m) {
+              atomicAdd(&dst->x, x * w[u].x);
+              atomicAdd(&dst->y, x * w[u].y);
+            } else {  // i x w
+              atomicAdd(&dst->x, -x * w[u].y);
+              atomicAdd(&dst->y, x * w[u].x);
             }
           }
         }
       }
-      lds_transform<LT, T, false, FRM>(z, lb2, 1, 0, lo, hi);
+    };
+    if (held && k0 < k1) load_row(k0);
+    for (int k = k0; k < k1; ++k) {
+      __syncthreads();  // the previous k's squares are read and their elements cleared
+      const cpx<T> bk = bal[k];
+      // KU keys per thread with all their loads in flight before the LDS adds
+      // (held keys: one round, from registers)
+      for (int base = 0; base < keys; base += LT * KU) {
+        uint32_t e[KU];
+        T v[KU];
+        if (held) {
 #pragma unroll
-      for (int r = 0; r < PER; ++r) {
-        const int e = tid + r * LT;
-        if (e < b2) acc[r] = cadd(acc[r], csqr(z[pad(e)]));
+          for (int u = 0; u < KU; ++u) {
+            e[u] = u < KH ? eh[u] : 0u;
+            v[u] = u < KH ? vn[u] : T(0);
+          }
+          if (k + 1 < k1) load_row(k + 1);
+        } else {
+#pragma unroll
+          for (int u = 0; u < KU; ++u) {
+            const int key = base + u * LT + tid;
+            e[u] = 0u;
+            v[u] = T(0);
+            if (key < keys) {
+              e[u] = key_entry(key);
+              v[u] = row_value(k, key);
+            }
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < KU; ++u) v[u] *= base + u * LT + tid < a.m ? bk.x : bk.y;
+        scatter(e, v, base);
+      }
+      lds_transform_head<LT, T, false, FRM>(z, lb2, 1, 0, lo, hi);
+#pragma unroll
+      for (int i = 0; i < PER / R; ++i) {  // group g = tid + i LT holds elements g R + j
+        const int g = tid + i * LT;
+        if (g < (b2 >> RL)) {
+          cpx<T> v[R];
+#pragma unroll
+          for (int j = 0; j < R; ++j) v[j] = z[pad(g * R + j)];
+          last_pass<R, T, false>(v);
+#pragma unroll
+          for (int j = 0; j < R; ++j) {
+            acc[i * R + j] = cadd(acc[i * R + j], csqr(v[j]));
+            z[pad(g * R + j)] = cpx<T>{T(0), T(0)};
+          }
+        }
       }
     }
     cpx<T>* out = static_cast<cpx<T>*>(a.partial) + out_off;
 #pragma unroll
-    for (int r = 0; r < PER; ++r) {
-      const int e = tid + r * LT;
-      if (e < b2) out[dif_nat(e, lb2, FRM)] = acc[r];
+    for (int i = 0; i < PER / R; ++i) {
+      const int g = tid + i * LT;
+      if (g < (b2 >> RL)) {
+#pragma unroll
+        for (int j = 0; j < R; ++j) out[dif_nat(g * R + j, lb2, FRM)] = acc[i * R + j];
+      }
     }
+  }
+}
+
+// the instantiation of the general kernel for slices of 2^lb2
+using Lds2Fn = void (*)(CompressArgs);
+template <typename T, bool FWHT>
+Lds2Fn lds2_kernel(int lb2) {
+  switch (ldsx::last_radix_log2(lb2, lds2_rm<T, FWHT>())) {
+    case 4: return &k_compress_lds2<T, FWHT, 4>;
+    case 3: return &k_compress_lds2<T, FWHT, 3>;
+    case 2: return &k_compress_lds2<T, FWHT, 2>;
+    case 1: return &k_compress_lds2<T, FWHT, 1>;
+    default: return &k_compress_lds2<T, FWHT, 0>;
   }
 }
 
@@ -468,11 +570,12 @@ hipError_t compress_lds_t(const CompressArgs& a, hipStream_t s) {
   const int b1 = 1 << (a.lb - a.lb2);
   const int64_t grid = int64_t(a.nchunks) * a.d * b1;
   const size_t lds = lds2_bytes<T, FWHT>(a.lb2);
+  const Lds2Fn fn = lds2_kernel<T, FWHT>(a.lb2);
   if (lds > 65536 &&
-      hipFuncSetAttribute(reinterpret_cast<const void*>(&k_compress_lds2<T, FWHT>),
-                          hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)) != hipSuccess)
+      hipFuncSetAttribute(reinterpret_cast<const void*>(fn), hipFuncAttributeMaxDynamicSharedMemorySize,
+                          int(lds)) != hipSuccess)
     return hipErrorInvalidValue;
-  k_compress_lds2<T, FWHT><<<dim3(unsigned(grid)), dim3(LT), lds, s>>>(a);
+  fn<<<dim3(unsigned(grid)), dim3(LT), lds, s>>>(a);
   return hipGetLastError();
 }
 
@@ -519,12 +622,13 @@ template <typename T, bool FWHT>
 int lds2_slots(int lb2) {
   int dev = 0, cus = 0, per_cu = 0;
   const size_t lds = lds2_bytes<T, FWHT>(lb2);
+  const Lds2Fn fn = lds2_kernel<T, FWHT>(lb2);
   if (lds > 65536)
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_compress_lds2<T, FWHT>),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(fn), hipFuncAttributeMaxDynamicSharedMemorySize,
+                              int(lds));
   if (hipGetDevice(&dev) != hipSuccess ||
       hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_compress_lds2<T, FWHT>, LT, lds) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, LT, lds) != hipSuccess ||
       cus <= 0 || per_cu <= 0)
     return 256;
   return cus * per_cu;

@@ -185,6 +185,36 @@ __device__ __forceinline__ void lds_transform(Elem<T, FWHT>* z, int l, int count
   __syncthreads();
 }
 
+// log2 of the radix of lds_transform's last pass (the lowest bits)
+__host__ __device__ __forceinline__ constexpr int last_radix_log2(int l, int rm) {
+  return l > 0 ? (l - 1) % rm + 1 : 0;
+}
+
+// lds_transform without its last pass: the passes over the bits above the
+// lowest last_radix_log2(l, RM), each after a barrier, then a barrier.  The
+// last pass (stride 1, no twiddles) transforms the R = 2^last_radix_log2
+// elements g R + j of each group g; a caller that consumes the transform
+// right away runs it in registers (last_pass) instead of through LDS.
+template <int NT, typename T, bool FWHT, int RM = rmax<T, FWHT>()>
+__device__ __forceinline__ void lds_transform_head(Elem<T, FWHT>* z, int l, int count, int cs,
+                                                   const cpx<T>* lo, const cpx<T>* hi) {
+  const int rl = last_radix_log2(l, RM);
+  for (int s = l - RM; s >= rl; s -= RM) {
+    __syncthreads();
+    lds_pass<NT, T, FWHT, (1 << RM)>(z, l, s, count, cs, lo, hi);
+  }
+  __syncthreads();
+}
+
+// the last pass of lds_transform on one group's R elements, in registers
+template <int R, typename T, bool FWHT>
+__device__ __forceinline__ void last_pass(Elem<T, FWHT> (&v)[R]) {
+  if constexpr (FWHT)
+    wht_reg<R>(v);
+  else
+    dft_reg<R>(v);
+}
+
 // LDS of a tile of 2^l elements (padded), rounded to 16 bytes, then (FFT)
 // the twiddle tables lo (64 entries) and hi (2^l / 64 + 1)
 template <typename T, bool FWHT>
