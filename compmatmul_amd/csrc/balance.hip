@@ -21,6 +21,8 @@
 //
 // One wave per inner index k; the output is (scale of a_k, scale of b_k)
 // in the element type, read once per k by the compress kernels.
+#include <climits>
+
 #include "device_common.hpp"
 #include "kernels.hpp"
 
@@ -36,65 +38,103 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-__device__ __forceinline__ double wave_max(double v) {
+__device__ __forceinline__ int wave_max_int(int v) {
 #pragma unroll
-  for (int o = 32; o >= 1; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  for (int o = 32; o >= 1; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
   return v;
 }
 
-// log2 of the Euclidean norm of row[0, len): returns 0 with *l2 set for a
-// finite nonzero norm, 1 for a zero row, 2 for a non-finite norm.
-// float rows: one pass, squares summed in double (no overflow or underflow
-// for any float).  double rows: the largest magnitude first, then the sum
-// of squares of the row scaled by its binary exponent.
-template <typename T>
-__device__ int row_log2_norm(const T* __restrict__ row, int len, int lane, double* l2) {
-  constexpr int U = 8;
+// A lane's sum of squares as s 4^e.  float values: s in double, e = 0 (no
+// square of a float overflows or underflows a double).  double values: each
+// batch of U values is scaled by the power of two of its largest magnitude
+// before squaring, and the batch sums are combined at the larger exponent;
+// all scalings are exact powers of two.  bad: a value is infinite or NaN.
+struct SumSq {
+  double s = 0.0;
+  int e = 0;
+  bool bad = false;
+};
+
+template <int U, typename T>
+__device__ __forceinline__ void add_squares(SumSq& a, const T (&v)[U]) {
   if constexpr (sizeof(T) == 4) {
-    double s = 0.0;
-    int i = lane;
-    for (; i + 64 * (U - 1) < len; i += 64 * U) {
-      float v[U];
 #pragma unroll
-      for (int u = 0; u < U; ++u) v[u] = row[i + 64 * u];
-#pragma unroll
-      for (int u = 0; u < U; ++u) s = fma(double(v[u]), double(v[u]), s);
+    for (int u = 0; u < U; ++u) {
+      a.bad |= !(fabsf(v[u]) <= 3.4028234663852886e38f);
+      a.s = fma(double(v[u]), double(v[u]), a.s);
     }
-    for (; i < len; i += 64) s = fma(double(row[i]), double(row[i]), s);
-    s = wave_sum(s);
-    if (s == 0.0) return 1;
-    if (!isfinite(s)) return 2;
-    *l2 = 0.5 * log2(s);
-    return 0;
   } else {
     double mx = 0.0;
-    for (int i = lane; i < len; i += 64) mx = fmax(mx, fabs(double(row[i])));
-    mx = wave_max(mx);
-    if (mx == 0.0) return 1;
-    if (!isfinite(mx)) return 2;
-    const int ex = ilogb(mx);
-    double s = 0.0;
-    for (int i = lane; i < len; i += 64) {
-      const double v = ldexp(double(row[i]), -ex);
-      s = fma(v, v, s);
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      a.bad |= !(fabs(v[u]) <= 1.7976931348623157e308);
+      mx = fmax(mx, fabs(v[u]));
     }
-    s = wave_sum(s);
-    *l2 = double(ex) + 0.5 * log2(s);
-    return 0;
+    if (a.bad || mx == 0.0) return;
+    const int ec = ilogb(mx);
+    double cs = 0.0;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const double x = ldexp(double(v[u]), -ec);
+      cs = fma(x, x, cs);
+    }
+    if (a.s == 0.0) {
+      a.s = cs;
+      a.e = ec;
+    } else if (ec > a.e) {
+      a.s = ldexp(a.s, 2 * (a.e - ec)) + cs;
+      a.e = ec;
+    } else {
+      a.s += ldexp(cs, 2 * (ec - a.e));
+    }
   }
+}
+
+// log2 of the Euclidean norm from the lanes' sums: returns 0 with *l2 set
+// for a finite nonzero norm, 1 for a zero row, 2 for a non-finite one.
+__device__ int wave_log2_norm(const SumSq& a, double* l2) {
+  const bool bad = __any(a.bad);
+  if (bad) return 2;
+  const int e = wave_max_int(a.s > 0.0 ? a.e : INT_MIN);
+  if (e == INT_MIN) return 1;
+  const double s = wave_sum(a.s > 0.0 ? ldexp(a.s, 2 * (a.e - e)) : 0.0);
+  if (!(s <= 1.7976931348623157e308)) return 2;  // float rows whose squares overflow the sum
+  *l2 = double(e) + 0.5 * log2(s);
+  return 0;
 }
 
 template <typename T>
 __device__ __forceinline__ void pair_scale_body(const PairScaleArgs& a, int blk) {
+  constexpr int U = 8;  // values of each row per lane with their loads in flight together
   const int w = __builtin_amdgcn_readfirstlane(int(threadIdx.x) >> 6);
   const int lane = threadIdx.x & 63;
   const int k = blk * ROWS + w;
   if (k >= a.n) return;
   const T* ar = static_cast<const T*>(a.AT) + int64_t(k) * a.ld_at;
   const T* br = static_cast<const T*>(a.B) + int64_t(k) * a.ld_b;
+  SumSq qa, qb;
+  // rows of A and B in one pass, U values of each per lane and batch; the
+  // index is clamped into the row, so every load is in bounds
+  for (int i0 = 0; i0 < max(a.m, a.p); i0 += 64 * U) {
+    T va[U], vb[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int i = i0 + 64 * u + lane;
+      va[u] = a.m > 0 ? ar[min(i, a.m - 1)] : T(0);
+      vb[u] = a.p > 0 ? br[min(i, a.p - 1)] : T(0);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int i = i0 + 64 * u + lane;
+      va[u] = i < a.m ? va[u] : T(0);
+      vb[u] = i < a.p ? vb[u] : T(0);
+    }
+    add_squares<U>(qa, va);
+    add_squares<U>(qb, vb);
+  }
   double la = 0.0, lb = 0.0;
-  const int sa = a.m > 0 ? row_log2_norm<T>(ar, a.m, lane, &la) : 1;
-  const int sb = a.p > 0 ? row_log2_norm<T>(br, a.p, lane, &lb) : 1;
+  const int sa = wave_log2_norm(qa, &la);
+  const int sb = wave_log2_norm(qb, &lb);
   T ka = T(1), kb = T(1);
   if (sa == 1 || sb == 1) {
     ka = kb = T(0);

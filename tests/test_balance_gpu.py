@@ -146,3 +146,16 @@ def test_batch_fft_imbalanced():
         print(f"batch product {i} b={b}: rel err {err:.2e}")
         assert err <= 1e-5
     bt.free()
+
+
+@pytest.mark.parametrize("dtype,b", [(np.float32, 1 << 14), (np.float64, 1 << 12)])
+def test_fft_nan_row_spreads_as_without_balance(dtype, b):
+    # a NaN in a_k keeps the row pair's scales at 1: the transform spreads it
+    # over every frequency, so every sketch entry is NaN, as when the two
+    # sketches are transformed separately (matmul_fft.hpp:95-105)
+    AT, B = imbalanced(120, 48, 90, 1e3, "per_k", 17, dtype)
+    AT[7, 11] = np.nan
+    sk = cma.compress("fft", AT, B, 3, b, seed=9, dtype=dtype)
+    got = sk.values()
+    sk.free()
+    assert np.isnan(got).all()
