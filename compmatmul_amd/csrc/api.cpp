@@ -733,11 +733,17 @@ int compress_device(cm_sketch* sk, const void* AT, int64_t ld_at, const void* B,
   const int f64_reduce = (flags & CM_FLAG_F64_REDUCE) || n > 16384;
   cm::ReduceArgs ra{fwht, f64, f64_reduce, partial, int(nchunks), d, lb, lb2, tw, V};
   cm::CombineArgs ca{fwht, f64, V, d, lb, lb2, tw, sk->P, lb};
-  return run_stages(ws, {{"compress", [&] {
-                            const hipError_t e = with_scales();
-                            return e != hipSuccess ? e : cm::launch_compress_lds(caa, cur().stream);
-                          }},
-                         {"reduce", [&] { return cm::launch_reduce_inverse(ra, cur().stream); }},
+  auto compress = [&] {
+    const hipError_t e = with_scales();
+    return e != hipSuccess ? e : cm::launch_compress_lds(caa, cur().stream);
+  };
+  auto reduce = [&] { return cm::launch_reduce_inverse(ra, cur().stream); };
+  if (lb2 == lb) {  // one slice: the inverse writes the sketch, no combine
+    ra.P = sk->P;
+    return run_stages(ws, {{"compress", compress}, {"reduce", reduce}});
+  }
+  return run_stages(ws, {{"compress", compress},
+                         {"reduce", reduce},
                          {"combine", [&] { return cm::launch_combine(ca, cur().stream); }}});
 }
 

@@ -103,12 +103,40 @@ __global__ void __launch_bounds__(NT) k_chunk_sum(T* __restrict__ P, int64_t ele
   chunk_sum_body<T, ACC>(P, elems, nchunks, blockIdx.x);
 }
 
+// The last radix-R pass of the inverse in registers, each group's outputs
+// passed to out(position, value) (stride 1: group g holds g R + j)
+template <int R, typename T, bool FWHT, typename F>
+__device__ __forceinline__ void reduce_last(ldsx::Elem<T, FWHT>* z, int b2, F&& out) {
+  for (int g = threadIdx.x; g < b2 / R; g += NT) {
+    ldsx::Elem<T, FWHT> v[R];
+#pragma unroll
+    for (int j = 0; j < R; ++j) v[j] = z[ldsx::pad(g * R + j)];
+    ldsx::last_pass<R, T, FWHT>(v);
+#pragma unroll
+    for (int j = 0; j < R; ++j) out(g * R + j, v[j]);
+  }
+}
+
+template <int RM, typename T, bool FWHT, typename F>
+__device__ __forceinline__ void reduce_last_any(ldsx::Elem<T, FWHT>* z, int lb2, F&& out) {
+  switch (ldsx::last_radix_log2(lb2, RM)) {
+    case 4: reduce_last<16, T, FWHT>(z, 1 << lb2, out); break;
+    case 3: reduce_last<8, T, FWHT>(z, 1 << lb2, out); break;
+    case 2: reduce_last<4, T, FWHT>(z, 1 << lb2, out); break;
+    case 1: reduce_last<2, T, FWHT>(z, 1 << lb2, out); break;
+    default: reduce_last<1, T, FWHT>(z, 1 << lb2, out); break;
+  }
+}
+
 // Sum of the chunks of one (t, slice) spectrum, then its inverse transform
-// over the slice in LDS with the radix-2^r passes of lds_transform.hpp
-// (three or four passes and barriers where radix-2 stages took eleven or
-// twelve): the natural-order WHT, and for the FFT the inverse DFT as
-// conj(DFT(conj Q)), the conjugations being exact sign flips.  The slice is
-// written in natural order.
+// over the slice in LDS with the radix-16 passes of lds_transform.hpp, the
+// last one in registers and written straight out (two LDS passes and three
+// barriers for 4096 points, where radix-2 stages took twelve): the
+// natural-order WHT, and for the FFT the inverse DFT as conj(DFT(conj Q)),
+// the conjugations being exact sign flips.  The slice is written in natural
+// order; with one slice and a.P set, the sketch is written instead, as the
+// combine kernel computes it (0 + v / b for FWHT, 0 + Im(V conj(W^0)) / 2b
+// for FFT, with the same operations so the bits agree).
 template <typename T, typename ACC, bool FWHT>
 __device__ __forceinline__ void reduce_body(const ReduceArgs& a, int blk) {
   using ldsx::pad;
@@ -143,11 +171,20 @@ __device__ __forceinline__ void reduce_body(const ReduceArgs& a, int blk) {
         s[pad(e)] = T(sum);
       }
     }
-    ldsx::lds_transform<NT, T, true>(s, a.lb2, 1, 0, nullptr, nullptr);
-    T* V = static_cast<T*>(a.V) + off;
-    for (int e = tid; e < b2; e += NT) V[e] = s[pad(e)];
+    ldsx::lds_transform_head<NT, T, true>(s, a.lb2, 1, 0, nullptr, nullptr);
+    if (a.P && b1 == 1) {
+      T* P = static_cast<T*>(a.P) + off;
+      const T inv = T(1) / T(b2);
+      reduce_last_any<ldsx::rmax<T, true>(), T, true>(s, a.lb2, [&](int e, T v) {
+        const T sum = T(0) + v;
+        P[e] = T(0) + sum * inv;
+      });
+    } else {
+      T* V = static_cast<T*>(a.V) + off;
+      reduce_last_any<ldsx::rmax<T, true>(), T, true>(s, a.lb2, [&](int e, T v) { V[e] = v; });
+    }
   } else {
-    constexpr int RM = sizeof(T) == 8 ? 3 : 4;  // radix 8 for FP64 (registers), 16 for FP32
+    constexpr int RM = 4;  // radix 16
     cpx<T>* z = reinterpret_cast<cpx<T>*>(smem_raw);
     cpx<T>* lo = ldsx::twiddle_lds<T, false>(smem_raw, a.lb2);
     cpx<T>* hi = lo + ldsx::TW_LO;
@@ -173,11 +210,22 @@ __device__ __forceinline__ void reduce_body(const ReduceArgs& a, int blk) {
         z[pad(e)] = {T(sx), -T(sy)};
       }
     }
-    ldsx::lds_transform<NT, T, false, RM>(z, a.lb2, 1, 0, lo, hi);
-    cpx<T>* V = static_cast<cpx<T>*>(a.V) + off;
-    for (int e = tid; e < b2; e += NT) {
-      const cpx<T> v = z[pad(e)];
-      V[ldsx::dif_nat(e, a.lb2, RM)] = cpx<T>{v.x, -v.y};
+    ldsx::lds_transform_head<NT, T, false, RM>(z, a.lb2, 1, 0, lo, hi);
+    if (a.P && b1 == 1) {
+      T* P = static_cast<T*>(a.P) + off;
+      const cpx<T> w = static_cast<const cpx<T>*>(a.tw)[0];  // W^0, as the combine reads it
+      const T inv = T(1) / T(2 * b2);
+      reduce_last_any<RM, T, false>(z, a.lb2, [&](int e, cpx<T> v) {
+        const cpx<T> c{v.x, -v.y};
+        T im = T(0);
+        im += c.y * w.x - c.x * w.y;
+        P[ldsx::dif_nat(e, a.lb2, RM)] = T(0) + im * inv;
+      });
+    } else {
+      cpx<T>* V = static_cast<cpx<T>*>(a.V) + off;
+      reduce_last_any<RM, T, false>(z, a.lb2, [&](int e, cpx<T> v) {
+        V[ldsx::dif_nat(e, a.lb2, RM)] = cpx<T>{v.x, -v.y};
+      });
     }
   }
 }

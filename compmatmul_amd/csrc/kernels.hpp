@@ -176,6 +176,9 @@ struct ReduceArgs {
   int d, lb, lb2;
   const void* tw;
   void* V;             // [d][b1][b2] (x2 for complex), element type
+  // one slice (lb2 = lb) and no padding fold: the sketch p (d x b), written
+  // by the inverse itself with the combine kernel's arithmetic (no combine)
+  void* P = nullptr;
 };
 hipError_t launch_reduce_inverse(const ReduceArgs& a, hipStream_t s);
 
