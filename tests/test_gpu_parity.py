@@ -372,3 +372,27 @@ def test_plain_path_empty_and_multi_pass():
     sk = cma.compress("fft", AT, B, 3, 1 << 15, seed=1, dtype=np.float32, flags=cma.FLAG_PLAIN)
     assert not sk.values().any()
     sk.free()
+
+
+# ------------------------------------------------- general kernel, every b
+@pytest.mark.parametrize("variant", ["fft", "fwht"])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("b", [2, 4, 8, 16, 128, 2048, 4096, 1 << 13, 1 << 14])
+def test_general_kernel_every_last_pass(variant, dtype, b):
+    # the general kernel runs its last transform pass in registers, one
+    # instantiation per radix 2^((log2 b2 - 1) mod 4 + 1); with one slice the
+    # inverse writes the sketch itself.  These b reach every radix, one and
+    # several slices, in both precisions, against the oracle's closed form
+    m, n, p, d = 90, 61, 75, 5
+    print(f"general {variant} {np.dtype(dtype).name} b={b}", flush=True)
+    AT, B, _ = workloads.planted(m, n, p, 4, 5.0, 7 + b % 89, dtype)
+    sk = cma.compress(variant, AT, B, d, b, seed=2024, dtype=dtype, flags=cma.FLAG_LDS_KERNEL)
+    hA, hB = sk.hashes()
+    want = orc.sketch_from_product(variant, AT.astype(np.float64).T @ B.astype(np.float64), hA, hB, b)
+    got = sk.values()
+    if dtype == np.float64:
+        assert np.abs(got - want).max() <= 1e-10 * max(np.abs(want).max(), 1.0)
+    else:
+        assert rel_l2(got, want) <= 1e-5
+    assert np.array_equal(sk.decompress().astype(np.float64), orc.decompress(variant, got.astype(np.float64), hA, hB))
+    sk.free()
