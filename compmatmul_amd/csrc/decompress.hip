@@ -338,11 +338,12 @@ constexpr int ring_issue() {
   return RING_HALF / 16 / NT;
 }
 
-template <typename T, int NT>
+// copy 32 KB piece k (piece k % HP of row k / HP) into slot k mod 5
+template <typename T, int NT, int HP>
 __device__ __forceinline__ void stage_half_async(const T* __restrict__ P, unsigned char* ring, int b,
                                                  int k, int tid) {
   const unsigned char* src =
-      reinterpret_cast<const unsigned char*>(P + int64_t(k >> 1) * b) + (k & 1) * RING_HALF;
+      reinterpret_cast<const unsigned char*>(P + int64_t(k / HP) * b) + (k % HP) * RING_HALF;
   unsigned char* dst = ring + (k % RING_SLOTS) * RING_HALF;
   const int wave = tid >> 6, lane = tid & 63;
 #pragma unroll
@@ -353,12 +354,16 @@ __device__ __forceinline__ void stage_half_async(const T* __restrict__ P, unsign
   }
 }
 
-template <typename T, bool FWHT, int D, int NT>
+// HP = 2: 64 KB rows in two 32 KB halves, as above.  HP = 1: 32 KB rows
+// (FP64 b = 2^12, FP32 b = 2^13), one slot each, so rows t+1 .. t+4 are in
+// flight while row t is gathered.  In both, the pieces of row t - 1 are
+// refilled after the barrier of row t.
+template <typename T, bool FWHT, int D, int NT, int HP>
 __device__ __forceinline__ void decompress_ring_body(const DecompressArgs& a, int blk) {
   constexpr int ER = rows_per_thread<T, D>();
-  constexpr int NH = 2 * D;  // halves to copy
+  constexpr int NH = HP * D;  // pieces to copy
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  const int b = a.b;  // b * sizeof(T) == 65536
+  const int b = a.b;  // b * sizeof(T) == HP * 32768
   const int tiles = (a.p + NT - 1) / NT;
   const int tid = threadIdx.x;
   const int j = int(blk % tiles) * NT + tid;
@@ -369,24 +374,28 @@ __device__ __forceinline__ void decompress_ring_body(const DecompressArgs& a, in
   uint32_t eb[D];
 #pragma unroll
   for (int t = 0; t < D; ++t) eb[t] = jok ? a.hB[int64_t(t) * a.p + j] : 0u;
+  // pieces 0 .. 4 - HP first; row t's step issues pieces HP t + 5 - HP ..
+  // HP t + 4 into the slots row t - 1 held
 #pragma unroll
-  for (int k = 0; k < (NH < 3 ? NH : 3); ++k) stage_half_async<T, NT>(P, smem_raw, b, k, tid);
+  for (int k = 0; k < (NH < RING_SLOTS - HP ? NH : RING_SLOTS - HP); ++k)
+    stage_half_async<T, NT, HP>(P, smem_raw, b, k, tid);
   T v[ER][keep_count<T, D>()];
   static_for<0, D>([&](auto tc) {
     constexpr int t = decltype(tc)::value;
-    // halves 0 .. 2t+1 landed: at most one later half (ring_issue copies of
-    // this wave) may still be in flight, none at the last row
-    if (t + 1 < D)
-      __builtin_amdgcn_s_waitcnt(0x0070 | ring_issue<NT>());  // vmcnt(ring_issue) lgkmcnt(0)
-    else
-      __builtin_amdgcn_s_waitcnt(0x0070);               // vmcnt(0) lgkmcnt(0)
+    // pieces 0 .. HP t + HP - 1 landed; the ones issued after them (up to
+    // HP t + 4 - HP, ring_issue copies of this wave each) may be in flight
+    constexpr int issued = (HP * t + RING_SLOTS - 1 - HP < NH - 1) ? HP * t + RING_SLOTS - 1 - HP : NH - 1;
+    constexpr int ahead = issued - (HP * t + HP - 1) > 0 ? issued - (HP * t + HP - 1) : 0;
+    static_assert(ahead * ring_issue<NT>() < 16, "vmcnt in the low four bits");
+    __builtin_amdgcn_s_waitcnt(0x0070 | (ahead * ring_issue<NT>()));  // vmcnt(...) lgkmcnt(0)
     __syncthreads();  // every wave's copies landed, and row t-1 is no longer read
-    if (2 * t + 3 < NH) stage_half_async<T, NT>(P, smem_raw, b, 2 * t + 3, tid);
-    if (2 * t + 4 < NH) stage_half_async<T, NT>(P, smem_raw, b, 2 * t + 4, tid);
-    // row t's halves sit in slots 2t mod 5 and 2t+1 mod 5: adjacent, so the
+#pragma unroll
+    for (int k = HP * t + RING_SLOTS - HP; k <= HP * t + RING_SLOTS - 1; ++k)
+      if (k < NH) stage_half_async<T, NT, HP>(P, smem_raw, b, k, tid);
+    // row t's pieces sit in slots HP t mod 5 (and the next): adjacent, so the
     // byte offset is lo + h sizeof(T), except where slot 4 wraps to slot 0
-    const uint32_t lo = uint32_t((2 * t) % RING_SLOTS) * RING_HALF;
-    const bool wraps = (2 * t) % RING_SLOTS == RING_SLOTS - 1;
+    const uint32_t lo = uint32_t((HP * t) % RING_SLOTS) * RING_HALF;
+    const bool wraps = HP == 2 && (HP * t) % RING_SLOTS == RING_SLOTS - 1;
     const uint32_t* hA = a.hA;
     asm volatile("" : "+s"(hA));  // row t's entries are loaded here, not all hoisted to the start
     T x[ER];
@@ -416,9 +425,9 @@ __device__ __forceinline__ void decompress_ring_body(const DecompressArgs& a, in
   }
 }
 
-template <typename T, bool FWHT, int D, int NT>
+template <typename T, bool FWHT, int D, int NT, int HP>
 __global__ void __launch_bounds__(NT) k_decompress_ring(DecompressArgs a) {
-  decompress_ring_body<T, FWHT, D, NT>(a, blockIdx.x);
+  decompress_ring_body<T, FWHT, D, NT, HP>(a, blockIdx.x);
 }
 
 template <typename T, bool FWHT, int D>
@@ -470,35 +479,40 @@ __global__ void __launch_bounds__(NT) k_decompress_any(DecompressArgs a) {
 }
 
 // 1024 threads leave 128 registers per thread: every ring instantiation fits
-// but FP32 at d = 10, whose 12 rows x 10 values plus hash entries spill.
-template <typename T, int D>
+// but FP32 at d = 10, whose 12 rows x 10 values plus hash entries spill, and
+// FP32 at d = 13 with 32 KB rows (one register more)
+template <typename T, int D, int HP>
 constexpr bool ring_wide_fits() {
-  return !(sizeof(T) == 4 && D == 10);
+  return !(sizeof(T) == 4 && (D == 10 || (D == 13 && HP == 1)));
 }
 
 // The ring kernel with NT threads: false when the 160 KB of LDS is refused.
-template <typename T, bool FWHT, int D, int NT>
+template <typename T, bool FWHT, int D, int NT, int HP>
 bool launch_ring(const DecompressArgs& a, hipStream_t s) {
-  if (hipFuncSetAttribute(reinterpret_cast<const void*>(&k_decompress_ring<T, FWHT, D, NT>),
+  if (hipFuncSetAttribute(reinterpret_cast<const void*>(&k_decompress_ring<T, FWHT, D, NT, HP>),
                           hipFuncAttributeMaxDynamicSharedMemorySize, RING_LDS) != hipSuccess)
     return false;
   constexpr int er = rows_per_thread<T, D>();
   const int64_t blocks = int64_t((a.p + NT - 1) / NT) * ((a.m + er - 1) / er);
-  k_decompress_ring<T, FWHT, D, NT><<<dim3(unsigned(blocks)), dim3(NT), RING_LDS, s>>>(a);
+  k_decompress_ring<T, FWHT, D, NT, HP><<<dim3(unsigned(blocks)), dim3(NT), RING_LDS, s>>>(a);
   return true;
 }
 
 template <typename T, bool FWHT>
 hipError_t launch_t(const DecompressArgs& a, hipStream_t s) {
-  if (a.d <= 16 && size_t(a.b) * sizeof(T) == 65536) {  // 64 KB rows: the half-row ring
+  const size_t row_bytes = size_t(a.b) * sizeof(T);
+  if (a.d <= 16 && (row_bytes == 65536 || row_bytes == 32768)) {  // the ring of 32 KB slots
     // with at least 1024 columns: 1024 threads, so each workgroup's staging
     // of the d rows serves twice the entries
     const bool wide = a.p >= 1024;
+    const bool halves = row_bytes == 65536;
     switch (a.d) {
 #define CM_D(N)                                                                               \
   case N:                                                                                     \
-    if (wide && ring_wide_fits<T, N>() ? launch_ring<T, FWHT, N, 1024>(a, s)                  \
-                                       : launch_ring<T, FWHT, N, LNT>(a, s))                  \
+    if (halves ? (wide && ring_wide_fits<T, N, 2>() ? launch_ring<T, FWHT, N, 1024, 2>(a, s)  \
+                                                    : launch_ring<T, FWHT, N, LNT, 2>(a, s))  \
+               : (wide && ring_wide_fits<T, N, 1>() ? launch_ring<T, FWHT, N, 1024, 1>(a, s)  \
+                                                    : launch_ring<T, FWHT, N, LNT, 1>(a, s))) \
       return hipGetLastError();                                                               \
     break;
       CM_D(1) CM_D(2) CM_D(3) CM_D(4) CM_D(5) CM_D(6) CM_D(7) CM_D(8)

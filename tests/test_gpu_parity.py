@@ -155,6 +155,27 @@ def test_decompress_f32_64k_rows(variant, d):
 
 
 @pytest.mark.parametrize("variant", ["fft", "fwht"])
+@pytest.mark.parametrize("dtype,b", [(np.float64, 1 << 12), (np.float32, 1 << 13)])
+@pytest.mark.parametrize("d", [1, 2, 3, 4, 5, 9, 13, 16])
+@pytest.mark.parametrize("p", [530, 1300])
+def test_decompress_32k_rows(variant, dtype, b, d, p):
+    # 32 KB sketch rows (FP64 b = 2^12, FP32 b = 2^13): one ring slot per
+    # row, up to four rows in flight, so the wait before each row counts the
+    # copies of 0 .. 3 later rows; 512 and 1024 threads (p >= 1024; FP32
+    # d = 13 keeps 512), ragged last tile and row block
+    print(f"{variant} d={d} b={b} {np.dtype(dtype).name} p={p}", flush=True)
+    rng = np.random.default_rng(300 + d)
+    AT = rng.standard_normal((20, 45)).astype(dtype)
+    B = rng.standard_normal((20, p)).astype(dtype)
+    sk = cma.compress(variant, AT, B, d, b, seed=11 + d, dtype=dtype)
+    vals = sk.values()
+    hA, hB = sk.hashes()
+    want = orc.decompress(variant, vals, hA, hB)
+    assert np.array_equal(sk.decompress().astype(np.float64), want)
+    sk.free()
+
+
+@pytest.mark.parametrize("variant", ["fft", "fwht"])
 @pytest.mark.parametrize("dtype,b,d", [(np.float32, 1 << 14, 1), (np.float32, 1 << 14, 10),
                                        (np.float32, 1 << 14, 15), (np.float32, 1 << 14, 16),
                                        (np.float64, 1 << 13, 9), (np.float64, 1 << 13, 16)])
