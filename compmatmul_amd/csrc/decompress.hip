@@ -324,11 +324,13 @@ __device__ __forceinline__ void decompress_lds_body(const DecompressArgs& a, int
   }
 }
 
-// Ring variant for 64 KB sketch rows (the two-buffer form above keeps one
-// row in flight): five 32 KB half-row slots fill the 160 KB of LDS, half k
-// of the d x 2 halves goes to slot k mod 5, and while row t is gathered the
-// halves 2t+2 .. 2t+4 are in flight (1.5 rows).  Row t's slots are refilled
-// after the barrier of row t + 1, so one barrier per row still suffices.
+// Ring variant for 64 KB and 32 KB sketch rows (the two-buffer form above
+// keeps one row in flight): five 32 KB slots fill the 160 KB of LDS.  A
+// 64 KB row is two halves, half k of the d x 2 goes to slot k mod 5, and
+// while row t is gathered the halves 2t+2 .. 2t+4 are in flight (1.5 rows);
+// a 32 KB row takes one slot, and rows t+1 .. t+4 are in flight.  Row t's
+// slots are refilled after the barrier of row t + 1, so one barrier per row
+// still suffices.
 constexpr int RING_SLOTS = 5;
 constexpr int RING_HALF = 32768;                  // bytes per half row
 constexpr int RING_LDS = RING_SLOTS * RING_HALF;  // 160 KB
