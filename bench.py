@@ -698,7 +698,8 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
-              "(the product shards by inner dimension or by product; see DESIGN.md)",
+              "(a product would shard by ranges of the inner dimension, a batch or "
+              "a request stream by product)",
               file=sys.stderr)
         sys.exit(2)
 
