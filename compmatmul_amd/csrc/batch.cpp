@@ -10,7 +10,7 @@
 //   compress    (product, unit) work items in eight XCD queues per variant,
 //               drained by one persistent launch per variant
 //   finish      chunk sums, per-slice inverses, combines (one launch each)
-//   decompress  one launch per (variant, d, staged) class
+//   decompress  one launch per (variant, d, form) class
 // Run: that launch sequence on the library stream, captured once into a HIP
 // graph with CM_BATCH_GRAPH.  Counters are cleared by a fill kernel, so the
 // graph holds no memset node.
@@ -83,7 +83,7 @@ struct cm_batch {
   size_t red_lds = 0;
   cm::DecompressArgs* dec = nullptr;
   struct DecClass {
-    int fwht, d, staged;
+    int fwht, d, form;
     size_t lds;
     int2* blocks;
     int nblocks;
@@ -506,12 +506,13 @@ int build_plan(int count, const cm_product* ps, unsigned flags, int grain, const
     std::map<std::tuple<int, int, int>, size_t> cls_lds;
     for (int i = 0; i < count; ++i) {
       const cm_product& q = ps[i];
-      const int staged = size_t(q.b) * sizeof(float) >= 16 && size_t(q.b) * sizeof(float) <= 32768;
-      const auto key = std::make_tuple(int(q.variant == CM_FWHT), q.d, staged);
-      const int nb = cm::decompress_batch_blocks(q.d, staged, q.m, q.p);
+      const int fw = int(q.variant == CM_FWHT);
+      const int form = cm::decompress_batch_form(fw, q.d, q.b);
+      const auto key = std::make_tuple(fw, q.d, form);
+      const int nb = cm::decompress_batch_blocks(q.d, form, q.m, q.p);
       auto& v = cls[key];
       for (int g = 0; g < nb; ++g) v.push_back(make_int2(i, g));
-      cls_lds[key] = std::max(cls_lds[key], staged ? 2 * size_t(q.b) * sizeof(float) : size_t(0));
+      cls_lds[key] = std::max(cls_lds[key], cm::decompress_batch_lds(form, q.b));
     }
     bt->classes.clear();
     std::vector<std::vector<int2>*> cls_host;
@@ -666,7 +667,7 @@ int enqueue(cm_batch* bt, bool untimed) {
     return rc;
   for (const auto& c : bt->classes)
     if ((rc = run("decompress", [&] {
-           return cm::launch_decompress_batch(c.fwht, c.d, c.staged, c.lds, bt->dec, c.blocks, c.nblocks, s);
+           return cm::launch_decompress_batch(c.fwht, c.d, c.form, c.lds, bt->dec, c.blocks, c.nblocks, s);
          })))
       return rc;
   return CM_OK;

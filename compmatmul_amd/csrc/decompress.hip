@@ -14,6 +14,8 @@
 // counter; the host sorts the list by (i, j).
 #include <type_traits>
 
+#include <atomic>
+
 #include "device_common.hpp"
 #include "kernels.hpp"
 
@@ -451,6 +453,14 @@ __global__ void __launch_bounds__(LNT) k_decompress_lds_batch(const DecompressAr
   decompress_lds_body<float, FWHT, D>(jobs[e.x], e.y);
 }
 
+// the ring with 512 threads; every job of a class has the same row length
+template <bool FWHT, int D, int HP>
+__global__ void __launch_bounds__(LNT) k_decompress_ring_batch(const DecompressArgs* __restrict__ jobs,
+                                                               const int2* __restrict__ blocks) {
+  const int2 e = blocks[blockIdx.x];
+  decompress_ring_body<float, FWHT, D, LNT, HP>(jobs[e.x], e.y);
+}
+
 // any d: the value of rank d/2 by counting, O(d^2) gathers per entry
 template <typename T, bool FWHT>
 __global__ void __launch_bounds__(NT) k_decompress_any(DecompressArgs a) {
@@ -558,9 +568,55 @@ hipError_t launch_t(const DecompressArgs& a, hipStream_t s) {
 
 }  // namespace
 
-int decompress_batch_blocks(int d, int staged, int m, int p) {
+namespace {
+template <bool FWHT, int D, int HP>
+const void* ring_batch_fn() {
+  return reinterpret_cast<const void*>(&k_decompress_ring_batch<FWHT, D, HP>);
+}
+
+const void* ring_batch_kernel(int fwht, int d, int hp) {
+  switch (d) {
+#define CM_D(N)                                                                                \
+  case N:                                                                                      \
+    return fwht ? (hp == 2 ? ring_batch_fn<true, N, 2>() : ring_batch_fn<true, N, 1>())        \
+                : (hp == 2 ? ring_batch_fn<false, N, 2>() : ring_batch_fn<false, N, 1>());
+    CM_D(1) CM_D(2) CM_D(3) CM_D(4) CM_D(5) CM_D(6) CM_D(7) CM_D(8)
+    CM_D(9) CM_D(10) CM_D(11) CM_D(12) CM_D(13) CM_D(14) CM_D(15) CM_D(16)
+#undef CM_D
+    default: return nullptr;
+  }
+}
+}  // namespace
+
+int decompress_batch_form(int fwht, int d, int b) {
+  const size_t row = size_t(b) * sizeof(float);
+  if (d < 1 || d > 16 || row < 16 || row > 65536) return 0;
+  if (row <= 16384) return 1;
+  const int hp = row == 65536 ? 2 : 1;
+  // one attribute call per kernel: 0 not yet set, 1 set, -1 refused
+  static std::atomic<int> state[2][17][3];
+  std::atomic<int>& st = state[fwht != 0][d][hp];
+  int v = st.load(std::memory_order_acquire);
+  if (v == 0) {
+    v = hipFuncSetAttribute(ring_batch_kernel(fwht, d, hp), hipFuncAttributeMaxDynamicSharedMemorySize,
+                            RING_LDS) == hipSuccess
+            ? 1
+            : -1;
+    (void)hipGetLastError();
+    st.store(v, std::memory_order_release);
+  }
+  if (v > 0) return hp == 2 ? 3 : 2;
+  return row <= 32768 ? 1 : 0;  // without the ring: two buffers up to 32 KB rows, else L2
+}
+
+size_t decompress_batch_lds(int form, int b) {
+  if (form == 1) return 2 * size_t(b) * sizeof(float);
+  return form >= 2 ? size_t(RING_LDS) : 0;
+}
+
+int decompress_batch_blocks(int d, int form, int m, int p) {
   if (m <= 0 || p <= 0) return 0;
-  if (!staged) return int(int64_t((p + NT - 1) / NT) * ((m + ROWS - 1) / ROWS));
+  if (form == 0) return int(int64_t((p + NT - 1) / NT) * ((m + ROWS - 1) / ROWS));
   int er = 1;
   switch (d) {
 #define CM_D(N) case N: er = rows_per_thread<float, N>(); break;
@@ -572,22 +628,30 @@ int decompress_batch_blocks(int d, int staged, int m, int p) {
   return int(int64_t((p + LNT - 1) / LNT) * ((m + er - 1) / er));
 }
 
-hipError_t launch_decompress_batch(int fwht, int d, int staged, size_t lds,
+hipError_t launch_decompress_batch(int fwht, int d, int form, size_t lds,
                                    const DecompressArgs* jobs, const int2* blocks, int nblocks,
                                    hipStream_t s) {
   if (nblocks <= 0) return hipSuccess;
-  if (d < 1 || d > 16 || (staged && lds > 65536)) return hipErrorInvalidValue;
+  if (d < 1 || d > 16 || form < 0 || form > 3 || (form == 1 && lds > 65536) ||
+      (form >= 2 && lds != size_t(RING_LDS)))
+    return hipErrorInvalidValue;
   const dim3 grid = dim3(unsigned(nblocks));
   switch (d) {
-#define CM_D(N)                                                                               \
-  case N:                                                                                     \
-    if (staged) {                                                                             \
-      if (fwht) k_decompress_lds_batch<true, N><<<grid, dim3(LNT), lds, s>>>(jobs, blocks);   \
-      else k_decompress_lds_batch<false, N><<<grid, dim3(LNT), lds, s>>>(jobs, blocks);       \
-    } else {                                                                                  \
-      if (fwht) k_decompress_batch<true, N><<<grid, dim3(NT), 0, s>>>(jobs, blocks);          \
-      else k_decompress_batch<false, N><<<grid, dim3(NT), 0, s>>>(jobs, blocks);              \
-    }                                                                                         \
+#define CM_D(N)                                                                                   \
+  case N:                                                                                         \
+    if (form == 3) {                                                                              \
+      if (fwht) k_decompress_ring_batch<true, N, 2><<<grid, dim3(LNT), lds, s>>>(jobs, blocks);   \
+      else k_decompress_ring_batch<false, N, 2><<<grid, dim3(LNT), lds, s>>>(jobs, blocks);       \
+    } else if (form == 2) {                                                                       \
+      if (fwht) k_decompress_ring_batch<true, N, 1><<<grid, dim3(LNT), lds, s>>>(jobs, blocks);   \
+      else k_decompress_ring_batch<false, N, 1><<<grid, dim3(LNT), lds, s>>>(jobs, blocks);       \
+    } else if (form == 1) {                                                                       \
+      if (fwht) k_decompress_lds_batch<true, N><<<grid, dim3(LNT), lds, s>>>(jobs, blocks);       \
+      else k_decompress_lds_batch<false, N><<<grid, dim3(LNT), lds, s>>>(jobs, blocks);           \
+    } else {                                                                                      \
+      if (fwht) k_decompress_batch<true, N><<<grid, dim3(NT), 0, s>>>(jobs, blocks);              \
+      else k_decompress_batch<false, N><<<grid, dim3(NT), 0, s>>>(jobs, blocks);                  \
+    }                                                                                             \
     break;
     CM_D(1) CM_D(2) CM_D(3) CM_D(4) CM_D(5) CM_D(6) CM_D(7) CM_D(8)
     CM_D(9) CM_D(10) CM_D(11) CM_D(12) CM_D(13) CM_D(14) CM_D(15) CM_D(16)

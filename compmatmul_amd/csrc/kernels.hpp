@@ -223,12 +223,19 @@ struct DecompressArgs {
   int64_t max_out; int32_t* oi; int32_t* oj; void* ov;
 };
 hipError_t launch_decompress(const DecompressArgs& a, hipStream_t s);
-// Many FP32 products of one class (variant, d <= 16, staged through LDS or
-// gathered from L2): blocks[g] = (job, workgroup within the job), with
-// decompress_batch_blocks() workgroups per job; `lds` = 2 * max b * 4 bytes
-// for the staged form (at most 64 KB).
-int decompress_batch_blocks(int d, int staged, int m, int p);
-hipError_t launch_decompress_batch(int fwht, int d, int staged, size_t lds,
+// Many FP32 products of one class (variant, d <= 16, form): blocks[g] =
+// (job, workgroup within the job), with decompress_batch_blocks() workgroups
+// per job.  Forms, by the sketch row's length (decompress_batch_form):
+// 0 gathers from L2 (rows above 64 KB, or d > 16), 1 two LDS buffers (rows up
+// to 16 KB), 2 the ring of five 32 KB LDS slots for 32 KB rows, 3 the ring
+// for 64 KB rows (half rows).  The ring forms need their 160 KB LDS
+// attribute, which decompress_batch_form sets (at plan time, outside any
+// graph capture); it returns a gathering form where the attribute is refused.
+// `lds` = decompress_batch_lds() of the class's largest row.
+int decompress_batch_form(int fwht, int d, int b);
+size_t decompress_batch_lds(int form, int b);
+int decompress_batch_blocks(int d, int form, int m, int p);
+hipError_t launch_decompress_batch(int fwht, int d, int form, size_t lds,
                                    const DecompressArgs* jobs, const int2* blocks, int nblocks,
                                    hipStream_t s);
 
