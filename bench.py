@@ -573,9 +573,14 @@ def bench_single(args, name, plain=False, quiet=False):
     # `rotate` copies of the operands, used in turn: with more bytes than the
     # 256 MB MALL, each product reads its inputs from HBM (traffic counters)
     rot = max(1, getattr(args, "rotate", 1))
-    dATs = [cma.DeviceArray.from_numpy(AT) for _ in range(rot)]
-    dBs = [cma.DeviceArray.from_numpy(B) for _ in range(rot)]
-    dC = cma.DeviceArray((m, p), dtype)
+    # --host: operands handed over in host memory and C returned there, as
+    # the drop-in entry points take them (the timed products include the
+    # transfers over PCIe)
+    host = bool(getattr(args, "host", False))
+    dATs = [AT] if host else [cma.DeviceArray.from_numpy(AT) for _ in range(rot)]
+    dBs = [B] if host else [cma.DeviceArray.from_numpy(B) for _ in range(rot)]
+    rot = 1 if host else rot
+    dC = np.empty((m, p), dtype) if host else cma.DeviceArray((m, p), dtype)
     flags = cma.FLAG_PLAIN if plain else 0
     sk = None
     calls = 0
@@ -606,7 +611,7 @@ def bench_single(args, name, plain=False, quiet=False):
     cma.timing_enable(False)
     print("[bench] per-kernel ms/launch: " + ", ".join(
         f"{k}={v[0] / v[1]:.4f}" for k, v in totals.items() if v[1]), file=sys.stderr, flush=True)
-    C_est = dC.numpy()
+    C_est = dC if host else dC.numpy()
     C_true = AT.astype(np.float64).T @ B.astype(np.float64)
     truth = set(zip(*np.nonzero(C_true > c["tau"])))
     ii, jj, _ = sk.recover(c["tau"])
@@ -653,6 +658,7 @@ def bench_single(args, name, plain=False, quiet=False):
                                f"{c['plants']} plants, tau={c['tau']}",
                    "m": m, "n": n, "p": p, "b": b, "d": d, "variant": c["variant"],
                    "hash": "multiply add shift", "path": "plain" if plain else "fused",
+                   "operands": "host (PCIe-inclusive)" if host else "device-resident",
                    "parallelism": "single GPU"},
         "roofline": roof,
         "detail": {"recall": recall, "precision": precision, "recovered": len(got),
@@ -685,6 +691,9 @@ def main():
     ap.add_argument("--kernel-times", action="store_true",
                     help="time every kernel class with HIP events (default: only the "
                          "dominant one, since each event pair slows the stream)")
+    ap.add_argument("--host", action="store_true",
+                    help="single-product workloads: operands in host memory and C returned "
+                         "there (the drop-in entry points' hand-over; transfers timed)")
     ap.add_argument("--rotate", type=int, default=1, metavar="N",
                     help="single-product workloads: cycle through N device copies of the "
                          "operands (N x their bytes above the 256 MB MALL: inputs come from HBM)")
