@@ -417,3 +417,24 @@ def test_general_kernel_every_last_pass(variant, dtype, b):
         assert rel_l2(got, want) <= 1e-5
     assert np.array_equal(sk.decompress().astype(np.float64), orc.decompress(variant, got.astype(np.float64), hA, hB))
     sk.free()
+
+
+# --------------------------------------------------- host transfers in chunks
+@pytest.mark.parametrize("dtype,m,n,p", [(np.float32, 1500, 777, 900),   # 4.7 MB, 2 MiB chunks
+                                         (np.float64, 700, 450, 530),    # 2.5 MB, 1 MiB chunks
+                                         (np.float32, 300, 800, 200)])   # below 1 MiB: one copy
+def test_host_operands_and_C_equal_device_path(dtype, m, n, p):
+    # host operands go up, and a host C comes back, through two pinned slots
+    # in chunks that do not divide the sizes; every value must arrive as on
+    # the device-resident path
+    AT, B, _ = workloads.planted(m, n, p, 4, 5.0, m + p, dtype)
+    host = cma.compress("fft", AT, B, 5, 1 << 12, seed=99, dtype=dtype)
+    dev = cma.compress("fft", cma.DeviceArray.from_numpy(AT), cma.DeviceArray.from_numpy(B), 5,
+                       1 << 12, seed=99, dtype=dtype)
+    assert np.array_equal(host.values(), dev.values())
+    C_host = host.decompress()
+    dC = cma.DeviceArray((m, p), dtype)
+    dev.decompress(dC)
+    assert np.array_equal(C_host, dC.numpy())
+    host.free()
+    dev.free()
