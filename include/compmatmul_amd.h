@@ -141,7 +141,9 @@ int cm_hash_arrays(int family, int d, int b, uint64_t seed, int m, int p,
  * Compressed product of AT (n x m, leading dimension ld_at) and B (n x p,
  * leading dimension ld_b) into a device-resident sketch of d x b values.
  * dtype CM_F64 or CM_F32 (inputs and arithmetic); mem CM_HOST or CM_DEVICE.
- * *out receives a new handle (free with cm_sketch_free).
+ * Host operands are read before the call returns (through pinned staging),
+ * so their memory may be reused at once; device operands are read on the
+ * stream.  *out receives a new handle (free with cm_sketch_free).
  * Replaces: matmul_compressed_product_fft<Hash> (matmul_fft.hpp:25-129) and
  * matmul_compressed_product_fwht<Hash> (matmul_fwht.hpp:25-114). */
 int cm_compress(int variant, int dtype, int n_a, int n_inner, int n_b,
