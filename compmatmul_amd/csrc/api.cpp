@@ -1223,8 +1223,7 @@ int cm_gemm(int n_a, int n_inner, int n_b, double* C, const double* AT, const do
               }))) {
     return rc;
   }
-  if (size_t(n_a) * n_b)
-    CM_HIP(hipMemcpyAsync(C, cur().C.p, size_t(n_a) * n_b * 8, hipMemcpyDeviceToHost, cur().stream));
+  if (size_t(n_a) * n_b && (rc = d2h(C, cur().C.p, size_t(n_a) * n_b * 8))) return rc;
   CM_HIP(hipStreamSynchronize(cur().stream));
   return CM_OK;
 }

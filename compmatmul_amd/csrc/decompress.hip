@@ -593,9 +593,13 @@ int decompress_batch_form(int fwht, int d, int b) {
   if (d < 1 || d > 16 || row < 16 || row > 65536) return 0;
   if (row <= 16384) return 1;
   const int hp = row == 65536 ? 2 : 1;
-  // one attribute call per kernel: 0 not yet set, 1 set, -1 refused
-  static std::atomic<int> state[2][17][3];
-  std::atomic<int>& st = state[fwht != 0][d][hp];
+  // one attribute call per kernel and device: 0 not yet set, 1 set, -1 refused
+  constexpr int kDevs = 16;
+  static std::atomic<int> state[kDevs][2][17][3];
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0) dev = 0;
+  std::atomic<int> local{0};  // devices past kDevs: set the attribute every time
+  std::atomic<int>& st = dev < kDevs ? state[dev][fwht != 0][d][hp] : local;
   int v = st.load(std::memory_order_acquire);
   if (v == 0) {
     v = hipFuncSetAttribute(ring_batch_kernel(fwht, d, hp), hipFuncAttributeMaxDynamicSharedMemorySize,

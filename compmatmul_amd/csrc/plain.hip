@@ -372,25 +372,107 @@ hipError_t mac_t(const PlainArgs& a, hipStream_t s) {
   return hipGetLastError();
 }
 
-// C[i][j] = sum_k AT[k][i] B[k][j], 16x16 output tiles staged through LDS
+// C[i][j] = sum_k AT[k][i] B[k][j] (FP64, all row-major).  A workgroup of
+// 256 threads owns a 128 x 128 tile of C; thread (tx, ty) accumulates the
+// 8 x 8 outputs at rows 2 ty + 32 r + u and columns 2 tx + 32 c + v
+// (r, c < 4; u, v < 2) in registers.  Tiles of 8 inner indices of AT and B
+// pass through two LDS buffers (k-major rows of 128), loaded one step
+// ahead into registers; each thread reads its operands as 16-byte pairs,
+// lanes 16 bytes apart (conflict-free), and does 64 FMAs per 8 values read.
+// Each output is one sequential FMA chain over k (the same order for any
+// tiling), padded loads read zeros.
+constexpr int GT = 128, GK = 8;
+
 __global__ void __launch_bounds__(256) k_gemm_f64(int m, int n, int p,
                                                   const double* __restrict__ AT,
                                                   const double* __restrict__ B,
                                                   double* __restrict__ C) {
-  __shared__ double sa[16][17], sb[16][17];
-  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
-  const int i = blockIdx.y * 16 + ty, j = blockIdx.x * 16 + tx;
-  double acc = 0.0;
-  for (int k0 = 0; k0 < n; k0 += 16) {
-    const int ka = k0 + ty;
-    sa[ty][tx] = (ka < n && blockIdx.y * 16 + tx < m) ? AT[int64_t(ka) * m + blockIdx.y * 16 + tx] : 0.0;
-    sb[ty][tx] = (ka < n && j < p) ? B[int64_t(ka) * p + j] : 0.0;
-    __syncthreads();
+  __shared__ __align__(16) double sa[2][GK][GT];
+  __shared__ __align__(16) double sb[2][GK][GT];
+  const int tid = threadIdx.x;
+  const int tx = tid & 15, ty = tid >> 4;
+  const int i0 = blockIdx.y * GT, j0 = blockIdx.x * GT;
+  // global -> LDS: each thread moves two pairs of AT and two of B per step
+  // (row kr, columns 2 cq and 2 cq + 64)
+  const int kr = tid >> 5, cq = tid & 31;
+  double2 ra[2], rb[2];
+  auto load = [&](int k0) {
+    const int k = k0 + kr;
 #pragma unroll
-    for (int kk = 0; kk < 16; ++kk) acc = fma(sa[kk][ty], sb[kk][tx], acc);
+    for (int h = 0; h < 2; ++h) {
+      const int ia = i0 + 2 * cq + 64 * h, jb = j0 + 2 * cq + 64 * h;
+      double2 a = make_double2(0.0, 0.0), bv = make_double2(0.0, 0.0);
+      if (k < n) {
+        const double* arow = AT + int64_t(k) * m;
+        const double* brow = B + int64_t(k) * p;
+        if (ia + 1 < m && ((m & 1) == 0)) {
+          a = *reinterpret_cast<const double2*>(arow + ia);
+        } else {
+          if (ia < m) a.x = arow[ia];
+          if (ia + 1 < m) a.y = arow[ia + 1];
+        }
+        if (jb + 1 < p && ((p & 1) == 0)) {
+          bv = *reinterpret_cast<const double2*>(brow + jb);
+        } else {
+          if (jb < p) bv.x = brow[jb];
+          if (jb + 1 < p) bv.y = brow[jb + 1];
+        }
+      }
+      ra[h] = a;
+      rb[h] = bv;
+    }
+  };
+  auto store = [&](int buf) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      *reinterpret_cast<double2*>(&sa[buf][kr][2 * cq + 64 * h]) = ra[h];
+      *reinterpret_cast<double2*>(&sb[buf][kr][2 * cq + 64 * h]) = rb[h];
+    }
+  };
+  double acc[8][8];
+#pragma unroll
+  for (int r = 0; r < 8; ++r)
+#pragma unroll
+    for (int c = 0; c < 8; ++c) acc[r][c] = 0.0;
+  const int steps = (n + GK - 1) / GK;
+  if (steps > 0) {
+    load(0);
+    store(0);
+  }
+  __syncthreads();
+  for (int t = 0; t < steps; ++t) {
+    const int buf = t & 1;
+    if (t + 1 < steps) load((t + 1) * GK);
+#pragma unroll
+    for (int kk = 0; kk < GK; ++kk) {
+      double a[8], bv[8];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const double2 x = *reinterpret_cast<const double2*>(&sa[buf][kk][2 * ty + 32 * q]);
+        const double2 y = *reinterpret_cast<const double2*>(&sb[buf][kk][2 * tx + 32 * q]);
+        a[2 * q] = x.x;
+        a[2 * q + 1] = x.y;
+        bv[2 * q] = y.x;
+        bv[2 * q + 1] = y.y;
+      }
+#pragma unroll
+      for (int r = 0; r < 8; ++r)
+#pragma unroll
+        for (int c = 0; c < 8; ++c) acc[r][c] = fma(a[r], bv[c], acc[r][c]);
+    }
+    if (t + 1 < steps) store(buf ^ 1);  // the other buffer was last read before the previous barrier
     __syncthreads();
   }
-  if (i < m && j < p) C[int64_t(i) * p + j] = acc;
+#pragma unroll
+  for (int r = 0; r < 8; ++r) {
+    const int i = i0 + 2 * ty + 32 * (r >> 1) + (r & 1);
+    if (i >= m) continue;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      const int j = j0 + 2 * tx + 32 * (c >> 1) + (c & 1);
+      if (j < p) C[int64_t(i) * p + j] = acc[r][c];
+    }
+  }
 }
 
 
@@ -423,7 +505,7 @@ hipError_t launch_plain_mac(const PlainArgs& a, hipStream_t s) {
 hipError_t launch_gemm_f64(int m, int n, int p, const double* AT, const double* B, double* C,
                            hipStream_t s) {
   if (m == 0 || p == 0) return hipSuccess;
-  const dim3 grid(unsigned((p + 15) / 16), unsigned((m + 15) / 16));
+  const dim3 grid(unsigned((p + GT - 1) / GT), unsigned((m + GT - 1) / GT));
   k_gemm_f64<<<grid, dim3(256), 0, s>>>(m, n, p, AT, B, C);
   return hipGetLastError();
 }
