@@ -18,6 +18,7 @@
 #include <string>
 #include <vector>
 
+#include "host_copy.hpp"
 #include "host_hash.hpp"
 #include "recover_sort.hpp"
 #include "kernels.hpp"
@@ -125,8 +126,9 @@ int ensure_pinned(size_t bytes) {
 // Large transfers between pageable host memory and the device go through
 // two pinned slots, the host copy of one chunk overlapping the DMA of the
 // other: pageable copies run at about 31 GB/s to the device and 18 GB/s
-// back on MI355X, pinned DMA at about 56 GB/s.  The caller's host memory is
-// read or written by the host only, so it may be reused on return.  Chunks
+// back on MI355X, pinned DMA at about 56 GB/s; the host copies run on a few
+// threads (host_copy.hpp).  The caller's host memory is read or written by
+// the host only, so it may be reused on return.  Chunks
 // of 1 MiB up to 4 MiB transfers (the overlap needs several chunks), 2 MiB
 // above (measured: 2 MB operands 1 MiB, 17 MB operands 2 MiB).
 constexpr size_t kXferChunk = size_t(2) << 20;  // slot size, the largest chunk
@@ -154,7 +156,7 @@ int h2d(void* dev, const void* host, size_t bytes) {
     const size_t len = std::min(chunk, bytes - off);
     char* slot = static_cast<char*>(st.xfer) + (i & 1) * kXferChunk;
     CM_HIP(hipEventSynchronize(st.xfer_ev[i & 1]));  // the slot's last DMA is done
-    std::memcpy(slot, static_cast<const char*>(host) + off, len);
+    cm::parallel_copy(slot, static_cast<const char*>(host) + off, len);
     CM_HIP(hipMemcpyAsync(static_cast<char*>(dev) + off, slot, len, hipMemcpyHostToDevice, st.stream));
     CM_HIP(hipEventRecord(st.xfer_ev[i & 1], st.stream));
   }
@@ -183,7 +185,7 @@ int d2h(void* host, const void* dev, size_t bytes) {
     if (i + 1 < n) CM_HIP(issue(i + 1));  // its slot's chunk i - 1 was copied out
     CM_HIP(hipEventSynchronize(st.xfer_ev[i & 1]));
     const size_t off = i * chunk, len = std::min(chunk, bytes - off);
-    std::memcpy(static_cast<char*>(host) + off, static_cast<const char*>(st.xfer) + (i & 1) * kXferChunk, len);
+    cm::parallel_copy(static_cast<char*>(host) + off, static_cast<const char*>(st.xfer) + (i & 1) * kXferChunk, len);
   }
   return CM_OK;
 }
