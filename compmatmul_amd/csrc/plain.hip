@@ -373,33 +373,35 @@ hipError_t mac_t(const PlainArgs& a, hipStream_t s) {
 }
 
 // C[i][j] = sum_k AT[k][i] B[k][j] (FP64, all row-major).  A workgroup of
-// 256 threads owns a 128 x 128 tile of C; thread (tx, ty) accumulates the
-// 8 x 8 outputs at rows 2 ty + 32 r + u and columns 2 tx + 32 c + v
-// (r, c < 4; u, v < 2) in registers.  Tiles of 8 inner indices of AT and B
-// pass through two LDS buffers (k-major rows of 128), loaded one step
-// ahead into registers; each thread reads its operands as 16-byte pairs,
-// lanes 16 bytes apart (conflict-free), and does 64 FMAs per 8 values read.
-// Each output is one sequential FMA chain over k (the same order for any
-// tiling), padded loads read zeros.
-constexpr int GT = 128, GK = 8;
+// 256 threads owns a GT x GT tile of C (GT = 128, or 64 where 128-tiles
+// would leave CUs idle); thread (tx, ty) accumulates the (GT/16)^2 outputs
+// at rows 2 ty + 32 r + u and columns 2 tx + 32 c + v (r, c < GT/32; u, v
+// < 2) in registers.  Tiles of 8 inner indices of AT and B pass through two
+// LDS buffers (k-major rows of GT), loaded one step ahead into registers;
+// each thread reads its operands as 16-byte pairs, lanes 16 bytes apart
+// (conflict-free).  Each output is one sequential FMA chain over k (the
+// same order for any tiling), padded loads read zeros.
+constexpr int GK = 8;
 
+template <int GT>
 __global__ void __launch_bounds__(256) k_gemm_f64(int m, int n, int p,
                                                   const double* __restrict__ AT,
                                                   const double* __restrict__ B,
                                                   double* __restrict__ C) {
+  constexpr int NO = GT / 16;  // outputs per thread in each direction
+  constexpr int H = GT / 64;   // pairs of each operand a thread moves per step
   __shared__ __align__(16) double sa[2][GK][GT];
   __shared__ __align__(16) double sb[2][GK][GT];
   const int tid = threadIdx.x;
   const int tx = tid & 15, ty = tid >> 4;
   const int i0 = blockIdx.y * GT, j0 = blockIdx.x * GT;
-  // global -> LDS: each thread moves two pairs of AT and two of B per step
-  // (row kr, columns 2 cq and 2 cq + 64)
+  // global -> LDS: row kr of the step, columns 2 cq + 64 h
   const int kr = tid >> 5, cq = tid & 31;
-  double2 ra[2], rb[2];
+  double2 ra[H], rb[H];
   auto load = [&](int k0) {
     const int k = k0 + kr;
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
+    for (int h = 0; h < H; ++h) {
       const int ia = i0 + 2 * cq + 64 * h, jb = j0 + 2 * cq + 64 * h;
       double2 a = make_double2(0.0, 0.0), bv = make_double2(0.0, 0.0);
       if (k < n) {
@@ -424,16 +426,16 @@ __global__ void __launch_bounds__(256) k_gemm_f64(int m, int n, int p,
   };
   auto store = [&](int buf) {
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
+    for (int h = 0; h < H; ++h) {
       *reinterpret_cast<double2*>(&sa[buf][kr][2 * cq + 64 * h]) = ra[h];
       *reinterpret_cast<double2*>(&sb[buf][kr][2 * cq + 64 * h]) = rb[h];
     }
   };
-  double acc[8][8];
+  double acc[NO][NO];
 #pragma unroll
-  for (int r = 0; r < 8; ++r)
+  for (int r = 0; r < NO; ++r)
 #pragma unroll
-    for (int c = 0; c < 8; ++c) acc[r][c] = 0.0;
+    for (int c = 0; c < NO; ++c) acc[r][c] = 0.0;
   const int steps = (n + GK - 1) / GK;
   if (steps > 0) {
     load(0);
@@ -445,9 +447,9 @@ __global__ void __launch_bounds__(256) k_gemm_f64(int m, int n, int p,
     if (t + 1 < steps) load((t + 1) * GK);
 #pragma unroll
     for (int kk = 0; kk < GK; ++kk) {
-      double a[8], bv[8];
+      double a[NO], bv[NO];
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
+      for (int q = 0; q < NO / 2; ++q) {
         const double2 x = *reinterpret_cast<const double2*>(&sa[buf][kk][2 * ty + 32 * q]);
         const double2 y = *reinterpret_cast<const double2*>(&sb[buf][kk][2 * tx + 32 * q]);
         a[2 * q] = x.x;
@@ -456,19 +458,19 @@ __global__ void __launch_bounds__(256) k_gemm_f64(int m, int n, int p,
         bv[2 * q + 1] = y.y;
       }
 #pragma unroll
-      for (int r = 0; r < 8; ++r)
+      for (int r = 0; r < NO; ++r)
 #pragma unroll
-        for (int c = 0; c < 8; ++c) acc[r][c] = fma(a[r], bv[c], acc[r][c]);
+        for (int c = 0; c < NO; ++c) acc[r][c] = fma(a[r], bv[c], acc[r][c]);
     }
     if (t + 1 < steps) store(buf ^ 1);  // the other buffer was last read before the previous barrier
     __syncthreads();
   }
 #pragma unroll
-  for (int r = 0; r < 8; ++r) {
+  for (int r = 0; r < NO; ++r) {
     const int i = i0 + 2 * ty + 32 * (r >> 1) + (r & 1);
     if (i >= m) continue;
 #pragma unroll
-    for (int c = 0; c < 8; ++c) {
+    for (int c = 0; c < NO; ++c) {
       const int j = j0 + 2 * tx + 32 * (c >> 1) + (c & 1);
       if (j < p) C[int64_t(i) * p + j] = acc[r][c];
     }
@@ -505,8 +507,13 @@ hipError_t launch_plain_mac(const PlainArgs& a, hipStream_t s) {
 hipError_t launch_gemm_f64(int m, int n, int p, const double* AT, const double* B, double* C,
                            hipStream_t s) {
   if (m == 0 || p == 0) return hipSuccess;
-  const dim3 grid(unsigned((p + GT - 1) / GT), unsigned((m + GT - 1) / GT));
-  k_gemm_f64<<<grid, dim3(256), 0, s>>>(m, n, p, AT, B, C);
+  // 128-tiles where they give every CU one, else 64-tiles
+  const int64_t big = int64_t((p + 127) / 128) * ((m + 127) / 128);
+  if (big >= 256) {
+    k_gemm_f64<128><<<dim3(unsigned((p + 127) / 128), unsigned((m + 127) / 128)), dim3(256), 0, s>>>(m, n, p, AT, B, C);
+  } else {
+    k_gemm_f64<64><<<dim3(unsigned((p + 63) / 64), unsigned((m + 63) / 64)), dim3(256), 0, s>>>(m, n, p, AT, B, C);
+  }
   return hipGetLastError();
 }
 
