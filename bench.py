@@ -98,6 +98,48 @@ def traffic_from_profile(name):
     return t.get("traffic_bytes")
 
 
+def achievable_peaks(nbytes=1 << 30):
+    """Rates this device sustains on calibration kernels of known work: FP32
+    and FP64 add and FMA loops (16 independent chains per thread, 16 waves
+    per CU, no packed FP32 instructions), and a streaming copy, read and
+    fill of 1 GiB (four times the 256 MB MALL, so HBM serves them).  They
+    sit beside the published peaks as the denominators of `achievable`."""
+    out = {}
+    for kind, iters in (("fp32_add", 4096), ("fp32_fma", 4096), ("fp64_add", 2048),
+                        ("fp64_fma", 2048)):
+        ms, flops = cma.probe_alu(kind, iters=iters)
+        out[f"{kind}_tflops"] = flops / (ms * 1e-3) / 1e12
+    src = cma.DeviceArray((nbytes // 4,), np.float32)
+    dst = cma.DeviceArray((nbytes // 4,), np.float32)
+    try:
+        cma.probe_stream("fill", src, None, nbytes, reps=1)  # finite data for the reads
+        for kind in ("copy", "read", "fill"):
+            ms, moved = cma.probe_stream(kind, dst, src, nbytes)
+            out[f"hbm_{kind}_gbs"] = moved / (ms * 1e-3) / 1e9
+    finally:
+        src.free()
+        dst.free()
+    return out
+
+
+def add_achievable(roof, peaks, f64=False):
+    """Puts the sustained rate of the calibration kernel that bounds `roof`
+    beside its published peak: the streaming copy for an HBM-bound kernel;
+    for a compute-bound one the add loop (the transforms' butterflies are
+    adds and subtractions) and the FMA loop (twiddle products can fuse)."""
+    if roof is None or not peaks:
+        return
+    if roof.get("unit") == "GB/s":
+        bw = peaks["hbm_copy_gbs"]
+        roof["achievable"] = {"copy_gbs": bw, "frac_copy": roof["achieved"] / bw}
+    else:
+        pre = "fp64" if f64 else "fp32"
+        add, fma = peaks[f"{pre}_add_tflops"], peaks[f"{pre}_fma_tflops"]
+        roof["achievable"] = {f"{pre}_add_loop_tflops": add, f"{pre}_fma_loop_tflops": fma,
+                              "frac_add_loop": roof["achieved"] / add,
+                              "frac_fma_loop": roof["achieved"] / fma}
+
+
 def bench_batch(args):
     """512 products of uneven shape in one batch (HIP graph replay of one
     plan): products/s over the whole batch, recall on every 16th product."""
@@ -701,6 +743,8 @@ def main():
                     help="only run the kernels of known traffic that scale FETCH_SIZE / "
                          "WRITE_SIZE counter readings: 5 device-to-device copies, and with "
                          "torch 5 sums and 5 fills, of MIB MiB each")
+    ap.add_argument("--no-peaks", action="store_true",
+                    help="skip the calibration kernels (sustained ALU and HBM rates)")
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe(args.copy_probe << 20)
@@ -712,12 +756,24 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
+    peaks = None
+    if not args.no_peaks:
+        peaks = achievable_peaks()
+        print("[bench] sustained rates: " + ", ".join(f"{k}={v:.1f}" for k, v in peaks.items()),
+              file=sys.stderr, flush=True)
+
+    def finish(out, f64=False):
+        if peaks:
+            add_achievable(out.get("roofline"), peaks, f64)
+            out["detail"]["achievable_peaks"] = peaks
+        print(json.dumps(out), flush=True)
+
     name = args.workload
     if name == "batch512":
-        print(json.dumps(bench_batch(args)), flush=True)
+        finish(bench_batch(args))
         return
     if name == "precision":
-        print(json.dumps(bench_precision(args)), flush=True)
+        finish(bench_precision(args))
         return
     if name != "serve":
         out = bench_single(args, name, plain=args.plain)
@@ -728,7 +784,7 @@ def main():
                                    "sample": f"FP64 oracle (C++/OpenMP restatement, not FFTW/MKL): "
                                              f"the full product, 1 warm-up, p50 of "
                                              f"{len(full['runs_s'])} runs"}
-        print(json.dumps(out), flush=True)
+        finish(out, f64=out["dtype"] == "f64")
         return
 
     out, templates = bench_serve(args)
@@ -743,11 +799,14 @@ def main():
             "products_per_s": f["value"], "roofline": f["roofline"], "recall": f["detail"]["recall"],
             "plain_products_per_s": pl["value"], "plain_roofline": pl["roofline"],
             "fused_vs_plain": f["value"] / pl["value"]}
+        if peaks:
+            add_achievable(f["roofline"], peaks)
+            add_achievable(pl["roofline"], peaks)
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline_serving(templates)
         out["detail"]["cpu_full_configs"] = cpu_full_configs(cpu_threads())
-    print(json.dumps(out), flush=True)
+    finish(out)
 
 
 if __name__ == "__main__":

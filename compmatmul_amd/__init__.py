@@ -80,6 +80,9 @@ def lib():
     L.cm_timing_enable.argtypes = [ctypes.c_int]
     L.cm_timing_only.argtypes = [ctypes.c_char_p]
     L.cm_timing_get.argtypes = [ctypes.c_char_p, _f64p, _i64p]
+    L.cm_probe_alu.argtypes = [ctypes.c_int] * 4 + [_f64p, _f64p]
+    L.cm_probe_stream.argtypes = [ctypes.c_int, _vp, _vp, ctypes.c_size_t, ctypes.c_int, _f64p,
+                                  _f64p]
     L.cm_batch_plan.argtypes = [ctypes.c_int, _vp, ctypes.c_uint, ctypes.c_int,
                                 ctypes.POINTER(_vp)]
     L.cm_batch_run.argtypes = [_vp]
@@ -371,6 +374,30 @@ def timing_get(name):
     n = ctypes.c_int64()
     check(lib().cm_timing_get(name.encode(), ctypes.byref(ms), ctypes.byref(n)))
     return ms.value, n.value
+
+
+PROBE_ALU = {"fp32_add": 0, "fp32_fma": 1, "fp64_add": 2, "fp64_fma": 3}
+PROBE_STREAM = {"copy": 0, "read": 1, "fill": 2}
+
+
+def probe_alu(kind, blocks=4096, iters=4096, reps=5):
+    """Calibration loop of known work (`kind` in PROBE_ALU): returns (ms per
+    launch, flops per launch)."""
+    ms, flops = ctypes.c_double(), ctypes.c_double()
+    check(lib().cm_probe_alu(PROBE_ALU[kind], blocks, iters, reps, ctypes.byref(ms),
+                             ctypes.byref(flops)))
+    return ms.value, flops.value
+
+
+def probe_stream(kind, dst, src, nbytes, reps=5):
+    """Streaming copy, read or fill (`kind` in PROBE_STREAM) over `nbytes` of
+    the DeviceArrays dst / src: returns (ms per launch, bytes moved per
+    launch)."""
+    ms, moved = ctypes.c_double(), ctypes.c_double()
+    check(lib().cm_probe_stream(PROBE_STREAM[kind], dst.ptr if dst is not None else None,
+                                src.ptr if src is not None else None, nbytes, reps,
+                                ctypes.byref(ms), ctypes.byref(moved)))
+    return ms.value, moved.value
 
 
 class _Product(ctypes.Structure):

@@ -35,7 +35,7 @@ HIP_FLAGS = [
 ]
 
 SOURCES = ["hash.hip", "balance.hip", "tables.hip", "noalloc.hip", "compress.hip", "compress_reg.hip", "compress_grp.hip", "compress_fft_reg.hip", "compress_fft_grp.hip", "decompress.hip",
-           "plain.hip", "sparse.hip", "fft_mfma.hip", "api.cpp", "batch.cpp", "serve.cpp", "host_hash.cpp", "host_copy.cpp"]
+           "plain.hip", "sparse.hip", "fft_mfma.hip", "api.cpp", "batch.cpp", "serve.cpp", "host_hash.cpp", "host_copy.cpp", "probe.hip"]
 
 
 def _run(cmd):

@@ -244,6 +244,34 @@ int cm_timing_only(const char* name);
 int cm_timing_reset(void);
 int cm_timing_get(const char* name, double* total_ms, int64_t* launches);
 
+/* ------------------------------------------------------------ calibration
+ * Kernels of known work that give the rates the device sustains, so that a
+ * kernel's roofline fraction can be read against them as well as against
+ * the published peaks.  No reference counterpart: they serve the
+ * measurement harness.  Each runs one untimed launch, then `reps` launches
+ * between two HIP events on the library stream; *ms is the mean per launch.
+ *
+ * cm_probe_alu: `blocks` workgroups of 256 threads, each thread `iters`
+ * trips of 16 independent adds (x + c) or FMAs (fma(x, a, c)); *flops is the
+ * work of one launch (an FMA counts 2).  Packed FP32 instructions are not
+ * used.
+ *
+ * cm_probe_stream: 16-byte accesses over `bytes` (a multiple of 16) of
+ * 16-byte aligned device memory: copy src to dst (non-overlapping), read
+ * src, or fill dst; *moved is the bytes read plus written by one launch.
+ * Buffers that are not device allocations of at least `bytes` are rejected
+ * with CM_EINVAL before any launch. */
+#define CM_PROBE_F32_ADD 0
+#define CM_PROBE_F32_FMA 1
+#define CM_PROBE_F64_ADD 2
+#define CM_PROBE_F64_FMA 3
+#define CM_PROBE_COPY 0
+#define CM_PROBE_READ 1
+#define CM_PROBE_FILL 2
+int cm_probe_alu(int kind, int blocks, int iters, int reps, double* ms, double* flops);
+int cm_probe_stream(int kind, void* dst, const void* src, size_t bytes, int reps, double* ms,
+                    double* moved);
+
 /* ------------------------------------------------------------------ batches
  * Many independent FP32 products of uneven shape in one pass.  The reference
  * has no counterpart: it computes one product per call
