@@ -9,7 +9,7 @@
 // folds at compile time; 16 chains per thread and 16 waves per CU hide the
 // ALU latency.  Packed FP32 instructions are disabled for this build, so
 // the FP32 rates are those of one operation per lane per instruction.
-// Streams: 16-byte vector accesses, four in flight per thread, grid-stride.
+// Streams: 16-byte nontemporal vector accesses over a grid-stride loop.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -53,46 +53,48 @@ __global__ __launch_bounds__(256) void k_alu_loop(int iters, T a, T c) {
 }
 
 constexpr int kStreamThreads = 256;
-constexpr int kStreamUnroll = 4;
+typedef float v4f __attribute__((ext_vector_type(4)));
 
-__global__ __launch_bounds__(kStreamThreads) void k_stream_copy(const float4* __restrict__ src,
-                                                                float4* __restrict__ dst,
+// Grid-stride streams of 16-byte nontemporal accesses, U per thread per
+// trip.  Launch shapes measured best on MI355X over 1 GiB (unroll 1, 2, 4,
+// 8, 16; 512 .. 16384 workgroups; plain or nontemporal): copy U = 2 with 3
+// workgroups per CU (about 6.0 TB/s read + write), read U = 1 with 6 per CU
+// (about 7.4 TB/s), fill U = 1 with 3 per CU (about 6.3 TB/s); the
+// runtime's own device-to-device copy reached about 5.1 TB/s.
+template <int U>
+__global__ __launch_bounds__(kStreamThreads) void k_stream_copy(const v4f* __restrict__ src,
+                                                                v4f* __restrict__ dst,
                                                                 size_t n4) {
   const size_t stride = size_t(gridDim.x) * kStreamThreads;
   size_t i = size_t(blockIdx.x) * kStreamThreads + threadIdx.x;
-  for (; i + (kStreamUnroll - 1) * stride < n4; i += kStreamUnroll * stride) {
-    float4 v[kStreamUnroll];
+  for (; i + (U - 1) * stride < n4; i += U * stride) {
+    v4f v[U];
 #pragma unroll
-    for (int u = 0; u < kStreamUnroll; ++u) v[u] = src[i + u * stride];
+    for (int u = 0; u < U; ++u) v[u] = __builtin_nontemporal_load(src + i + u * stride);
 #pragma unroll
-    for (int u = 0; u < kStreamUnroll; ++u) dst[i + u * stride] = v[u];
+    for (int u = 0; u < U; ++u) __builtin_nontemporal_store(v[u], dst + i + u * stride);
   }
-  for (; i < n4; i += stride) dst[i] = src[i];
+  for (; i < n4; i += stride) __builtin_nontemporal_store(__builtin_nontemporal_load(src + i), dst + i);
 }
 
-__global__ __launch_bounds__(kStreamThreads) void k_stream_read(const float4* __restrict__ src,
+__global__ __launch_bounds__(kStreamThreads) void k_stream_read(const v4f* __restrict__ src,
                                                                 size_t n4) {
   const size_t stride = size_t(gridDim.x) * kStreamThreads;
-  size_t i = size_t(blockIdx.x) * kStreamThreads + threadIdx.x;
   float acc = 0.f;
-  for (; i + (kStreamUnroll - 1) * stride < n4; i += kStreamUnroll * stride) {
-    float4 v[kStreamUnroll];
-#pragma unroll
-    for (int u = 0; u < kStreamUnroll; ++u) v[u] = src[i + u * stride];
-#pragma unroll
-    for (int u = 0; u < kStreamUnroll; ++u) acc += v[u].x + v[u].y + v[u].z + v[u].w;
+  for (size_t i = size_t(blockIdx.x) * kStreamThreads + threadIdx.x; i < n4; i += stride) {
+    const v4f v = __builtin_nontemporal_load(src + i);
+    acc += v.x + v.y + v.z + v.w;
   }
-  for (; i < n4; i += stride) acc += src[i].x + src[i].y + src[i].z + src[i].w;
   // never true for finite data; keeps the loads live
   if (acc != acc) g_probe_sink_f = acc;
 }
 
-__global__ __launch_bounds__(kStreamThreads) void k_stream_fill(float4* __restrict__ dst,
+__global__ __launch_bounds__(kStreamThreads) void k_stream_fill(v4f* __restrict__ dst,
                                                                 size_t n4, float value) {
   const size_t stride = size_t(gridDim.x) * kStreamThreads;
-  const float4 v = make_float4(value, value, value, value);
+  const v4f v = {value, value, value, value};
   for (size_t i = size_t(blockIdx.x) * kStreamThreads + threadIdx.x; i < n4; i += stride)
-    dst[i] = v;
+    __builtin_nontemporal_store(v, dst + i);
 }
 
 // true when [p, p + bytes) lies inside one device allocation
@@ -188,20 +190,20 @@ extern "C" int cm_probe_stream(int kind, void* dst, const void* src, size_t byte
       hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
     return rt::error(CM_EDEVICE, "cm_probe_stream: device query failed");
   const size_t n4 = bytes / 16;
-  const size_t per_block = size_t(kStreamThreads) * kStreamUnroll;
+  const size_t per_cu = kind == CM_PROBE_READ ? 6 : 3;
   const unsigned grid = unsigned(std::max<size_t>(
-      1, std::min<size_t>(size_t(cus) * 16, (n4 + per_block - 1) / per_block)));
+      1, std::min<size_t>(size_t(cus) * per_cu, (n4 + kStreamThreads - 1) / kStreamThreads)));
   auto launch = [&](hipStream_t s) -> hipError_t {
     switch (kind) {
       case CM_PROBE_COPY:
-        k_stream_copy<<<grid, kStreamThreads, 0, s>>>(static_cast<const float4*>(src),
-                                                      static_cast<float4*>(dst), n4);
+        k_stream_copy<2><<<grid, kStreamThreads, 0, s>>>(static_cast<const v4f*>(src),
+                                                         static_cast<v4f*>(dst), n4);
         break;
       case CM_PROBE_READ:
-        k_stream_read<<<grid, kStreamThreads, 0, s>>>(static_cast<const float4*>(src), n4);
+        k_stream_read<<<grid, kStreamThreads, 0, s>>>(static_cast<const v4f*>(src), n4);
         break;
       default:
-        k_stream_fill<<<grid, kStreamThreads, 0, s>>>(static_cast<float4*>(dst), n4, 0.5f);
+        k_stream_fill<<<grid, kStreamThreads, 0, s>>>(static_cast<v4f*>(dst), n4, 0.5f);
         break;
     }
     return hipGetLastError();
