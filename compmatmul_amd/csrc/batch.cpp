@@ -19,8 +19,10 @@
 // a unit is (chunk, repetition, slice) and costs about kper_q * c_q with
 // c_q = 1 + (m_q + p_q) / 1700 (one k-step of both transforms plus the
 // scatter of m + p keys, in units of a bare transform pair).  kper_q depends
-// only on the product and the grain, so a product's result does not depend
-// on what else is in the batch.  Chunks go to the queue with the least work
+// only on the product, the grain and the device (a product that cannot fill
+// the GPU alone at that grain is cut into at most one round of resident
+// workgroups), so a product's result does not depend on what else is in the
+// batch.  Chunks go to the queue with the least work
 // (largest units first), keeping all units of a chunk on one XCD so they
 // share its L2 copy of the chunk's rows of AT and B.
 #include <hip/hip_runtime.h>
@@ -225,6 +227,7 @@ int build_plan(int count, const cm_product* ps, unsigned flags, int grain, const
   hp.tabs.resize(count);
   hp.tab_len.assign(count, 0);
   int njobs[kKinds] = {};
+  int slots_of[kKinds] = {};
   for (int i = 0; i < count; ++i) {
     const cm_product& q = ps[i];
     const int fw = q.variant == CM_FWHT;
@@ -232,6 +235,18 @@ int build_plan(int count, const cm_product* ps, unsigned flags, int grain, const
     hp.lbv[i] = std::max(lb, cm::kRegLb2);
     int kper = int(std::lround(double(grain) / step_cost(q)));
     kper = std::max(1, std::min(kper, std::max(1, q.n)));
+    // A product too small to fill the GPU alone at grain-sized chunks is cut
+    // finer, into at most one round of resident workgroups (the rule of the
+    // single-product path): alone or in a small wave its units would
+    // otherwise run long serial k loops on a few CUs.  The rule depends
+    // only on the product and the device, not on the rest of the batch.
+    if (q.n > 0) {
+      const int kd = kind_of(q);
+      if (!slots_of[kd]) slots_of[kd] = kind_slots(kd);
+      const int64_t per_chunk = std::max<int64_t>(1, (int64_t(q.d) << cm::reg_lbs(lb)) >> (kd % 3));
+      const int64_t fill = std::max<int64_t>(1, std::min<int64_t>(q.n, slots_of[kd] / per_chunk));
+      kper = int(std::min<int64_t>(kper, (q.n + fill - 1) / fill));
+    }
     const size_t chunk_bytes = (size_t(q.d) << hp.lbv[i]) * (fw ? 1 : 2) * sizeof(float);
     const int64_t max_chunks = std::max<int64_t>(1, int64_t(kPartialCap / chunk_bytes));
     if (q.n > 0 && (q.n + kper - 1) / kper > max_chunks) kper = int((q.n + max_chunks - 1) / max_chunks);

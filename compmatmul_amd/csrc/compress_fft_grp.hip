@@ -619,8 +619,10 @@ k_compress_fft_grp_batch(const RegJob* __restrict__ jobs, const int2* __restrict
     const int q = (home + qi) & 7;
     const int base = qstart[q];
     const int len = qstart[q + 1] - base;
+    bool probe = true;
     for (;;) {
-      if (threadIdx.x == 0) item = int(atomicAdd(&counters[q], 1u));
+      if (threadIdx.x == 0) item = int(claim_item(&counters[q], len, probe));
+      probe = false;
       __syncthreads();
       const int it = __builtin_amdgcn_readfirstlane(item);  // uniform: the job lives in scalar registers
       __syncthreads();  // every thread has read `item` before it is rewritten

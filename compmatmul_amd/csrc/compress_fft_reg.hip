@@ -338,9 +338,11 @@ __global__ void __launch_bounds__(64) k_compress_fft_batch(const RegJob* __restr
     const int q = (home + qi) & 7;
     const int base = qstart[q];
     const int len = qstart[q + 1] - base;
+    bool probe = true;
     for (;;) {
       unsigned it = 0;
-      if (threadIdx.x == 0) it = atomicAdd(&counters[q], 1u);
+      if (threadIdx.x == 0) it = claim_item(&counters[q], len, probe);
+      probe = false;
       it = __builtin_amdgcn_readfirstlane(it);
       if (int(it) >= len) break;
       const int2 e = items[base + int(it)];

@@ -357,9 +357,11 @@ __global__ void __launch_bounds__(64) k_compress_fwht_batch(const RegJob* __rest
     const int q = (home + qi) & 7;
     const int base = qstart[q];
     const int len = qstart[q + 1] - base;
+    bool probe = true;
     for (;;) {
       unsigned it = 0;
-      if (threadIdx.x == 0) it = atomicAdd(&counters[q], 1u);
+      if (threadIdx.x == 0) it = claim_item(&counters[q], len, probe);
+      probe = false;
       it = __builtin_amdgcn_readfirstlane(it);
       if (int(it) >= len) break;
       const int2 e = items[base + int(it)];

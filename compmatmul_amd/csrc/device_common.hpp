@@ -5,6 +5,21 @@
 
 namespace cm {
 
+// Claims the next item of a work queue whose counter is `c` and which holds
+// `len` items: returns the item's index, or a value >= len once the queue
+// is exhausted.  With `probe` (a workgroup's first claim on a queue) a
+// relaxed load first passes over an exhausted queue without an atomic,
+// since the failed claims of every resident workgroup on one counter would
+// serialize in L2; later claims on a queue that just gave an item go
+// straight to the atomic.
+__device__ __forceinline__ unsigned claim_item(unsigned* c, int len, bool probe) {
+  if (probe) {
+    const unsigned seen = __hip_atomic_load(c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (int(seen) >= len) return seen;
+  }
+  return atomicAdd(c, 1u);
+}
+
 template <typename T>
 struct cpx {
   T x, y;

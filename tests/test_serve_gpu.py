@@ -110,17 +110,24 @@ def test_csr_operands_match_dense():
 
 
 def test_admission_by_footprint():
-    # each request needs a few MB; the budget admits only a couple at a time
-    sv = cma.Server(hbm_bytes=6 << 20, max_waves=3, max_wave_products=8, grain=GRAIN)
-    tickets = []
-    for k in range(6):
-        AT, B = data(40 + k, 256, 512, 256)
-        tickets.append(sv.submit("fwht", AT, B, 5, 1 << 12, k, tau=12.5))
+    # the footprint of one request, measured alone; then a budget of two and
+    # a half footprints admits at most two requests at a time
+    ops = [data(40 + k, 256, 512, 256) for k in range(6)]
+    sv = cma.Server(hbm_bytes=1 << 30, max_waves=3, max_wave_products=8, grain=GRAIN)
+    t = sv.submit("fwht", *ops[0], 5, 1 << 12, 0, tau=12.5)
+    assert sv.wait(t, timeout=120)
+    sv.release(t)
+    one = sv.stats()["peak_bytes"]
+    sv.close()
+    assert 0 < one < 64 << 20
+    budget = one * 5 // 2
+    sv = cma.Server(hbm_bytes=budget, max_waves=3, max_wave_products=8, grain=GRAIN)
+    tickets = [sv.submit("fwht", AT, B, 5, 1 << 12, k, tau=12.5) for k, (AT, B) in enumerate(ops)]
     for t in tickets:
         assert sv.wait(t, timeout=120)
         sv.release(t)
     st = sv.stats()
-    assert st["requests_done"] == 6 and st["peak_bytes"] <= 6 << 20
+    assert st["requests_done"] == 6 and st["peak_bytes"] <= budget
     # a request larger than the whole budget fails instead of waiting forever
     AT, B = data(99, 2048, 2048, 2048)
     t = sv.submit("fwht", AT, B, 15, 1 << 14, 1, tau=18.0)
