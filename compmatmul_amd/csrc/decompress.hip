@@ -11,7 +11,8 @@
 // the sketch (d x b) is gathered from L2, and the median comes from a
 // register selection network (d <= 16) or a rank count (larger d).
 // Recovery appends (i, j, value) for every estimate > tau through a global
-// counter; the host sorts the list by (i, j).
+// counter (one atomic per wave in the staged kernels); the host sorts the
+// list by (i, j).
 #include <type_traits>
 
 #include <atomic>
@@ -204,6 +205,49 @@ __device__ __forceinline__ void emit(const DecompressArgs& a, int i, int j, T me
   }
 }
 
+// The staged kernels' form of emit for a thread's ER rows of column j: C is
+// written, and the estimates above tau are appended with one atomic per
+// wave.  Each lane counts its passing rows, a prefix sum over the wave
+// places them, and lane 63 claims the wave's slots.  Every lane of the wave
+// calls it (rows outside C have `jok` false or i >= m).  An atomic per
+// (wave, row) serialized on the counter when most estimates pass (small b
+// and large m p): about 40 ns each, milliseconds per product.
+template <typename T, int ER>
+__device__ __forceinline__ void emit_rows(const DecompressArgs& a, int i0, int j, bool jok,
+                                          const T (&med)[ER]) {
+  unsigned pass = 0;
+#pragma unroll
+  for (int e = 0; e < ER; ++e) {
+    const bool in = jok && i0 + e < a.m;
+    if (in && a.C) static_cast<T*>(a.C)[int64_t(i0 + e) * a.ldc + j] = med[e];
+    if (in && a.recover && double(med[e]) > a.tau) pass |= 1u << e;
+  }
+  if (!a.recover) return;
+  const int lane = int(threadIdx.x) & 63;
+  const int cnt = __popc(pass);
+  int incl = cnt;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(incl, o, 64);
+    if (lane >= o) incl += y;
+  }
+  const int total = __shfl(incl, 63, 64);
+  if (total == 0) return;
+  unsigned long long base = 0;
+  if (lane == 63) base = atomicAdd(a.counter, (unsigned long long)total);
+  unsigned long long slot = __shfl(base, 63, 64) + unsigned(incl - cnt);
+#pragma unroll
+  for (int e = 0; e < ER; ++e) {
+    if (!((pass >> e) & 1u)) continue;
+    if (int64_t(slot) < a.max_out) {
+      a.oi[slot] = i0 + e;
+      a.oj[slot] = j;
+      static_cast<T*>(a.ov)[slot] = med[e];
+    }
+    ++slot;
+  }
+}
+
 template <typename T, bool FWHT, int D>
 __device__ __forceinline__ void decompress_body(const DecompressArgs& a, int blk) {
   const int tiles = (a.p + NT - 1) / NT;
@@ -318,12 +362,10 @@ __device__ __forceinline__ void decompress_lds_body(const DecompressArgs& a, int
 #pragma unroll
     for (int e = 0; e < ER; ++e) select_push<T, D, t>(v[e], x[e]);
   });
-  if (!jok) return;
+  T med[ER];
 #pragma unroll
-  for (int e = 0; e < ER; ++e) {
-    if (i0 + e >= a.m) break;
-    emit<T>(a, i0 + e, j, select_result<T, D>(v[e]));
-  }
+  for (int e = 0; e < ER; ++e) med[e] = select_result<T, D>(v[e]);
+  emit_rows<T, ER>(a, i0, j, jok, med);
 }
 
 // Ring variant for 64 KB and 32 KB sketch rows (the two-buffer form above
@@ -421,12 +463,10 @@ __device__ __forceinline__ void decompress_ring_body(const DecompressArgs& a, in
 #pragma unroll
     for (int e = 0; e < ER; ++e) select_push<T, D, t>(v[e], x[e]);
   });
-  if (!jok) return;
+  T med[ER];
 #pragma unroll
-  for (int e = 0; e < ER; ++e) {
-    if (i0 + e >= a.m) break;
-    emit<T>(a, i0 + e, j, select_result<T, D>(v[e]));
-  }
+  for (int e = 0; e < ER; ++e) med[e] = select_result<T, D>(v[e]);
+  emit_rows<T, ER>(a, i0, j, jok, med);
 }
 
 template <typename T, bool FWHT, int D, int NT, int HP>
@@ -491,11 +531,12 @@ __global__ void __launch_bounds__(NT) k_decompress_any(DecompressArgs a) {
 }
 
 // 1024 threads leave 128 registers per thread: every ring instantiation fits
-// but FP32 at d = 10, whose 12 rows x 10 values plus hash entries spill, and
-// FP32 at d = 13 with 32 KB rows (one register more)
-template <typename T, int D, int HP>
+// but FP32 at d = 10, whose 12 rows x 10 values plus hash entries spill,
+// FP32 at d = 13 with 32 KB rows (one register more), and the FP64 FWHT at
+// d = 5 (its 12 estimates live through the recovery prefix sum)
+template <typename T, bool FWHT, int D, int HP>
 constexpr bool ring_wide_fits() {
-  return !(sizeof(T) == 4 && (D == 10 || (D == 13 && HP == 1)));
+  return !(sizeof(T) == 4 && (D == 10 || (D == 13 && HP == 1))) && !(sizeof(T) == 8 && FWHT && D == 5);
 }
 
 // The ring kernel with NT threads: false when the 160 KB of LDS is refused.
@@ -510,6 +551,17 @@ bool launch_ring(const DecompressArgs& a, hipStream_t s) {
   return true;
 }
 
+// 1024 threads where the instantiation fits in 128 registers and the
+// product has at least 1024 columns, else 512; a variant that does not fit
+// is never instantiated
+template <typename T, bool FWHT, int D, int HP>
+bool launch_ring_nt(const DecompressArgs& a, hipStream_t s, bool wide) {
+  if constexpr (ring_wide_fits<T, FWHT, D, HP>()) {
+    if (wide) return launch_ring<T, FWHT, D, 1024, HP>(a, s);
+  }
+  return launch_ring<T, FWHT, D, LNT, HP>(a, s);
+}
+
 template <typename T, bool FWHT>
 hipError_t launch_t(const DecompressArgs& a, hipStream_t s) {
   const size_t row_bytes = size_t(a.b) * sizeof(T);
@@ -521,10 +573,7 @@ hipError_t launch_t(const DecompressArgs& a, hipStream_t s) {
     switch (a.d) {
 #define CM_D(N)                                                                               \
   case N:                                                                                     \
-    if (halves ? (wide && ring_wide_fits<T, N, 2>() ? launch_ring<T, FWHT, N, 1024, 2>(a, s)  \
-                                                    : launch_ring<T, FWHT, N, LNT, 2>(a, s))  \
-               : (wide && ring_wide_fits<T, N, 1>() ? launch_ring<T, FWHT, N, 1024, 1>(a, s)  \
-                                                    : launch_ring<T, FWHT, N, LNT, 1>(a, s))) \
+    if (halves ? launch_ring_nt<T, FWHT, N, 2>(a, s, wide) : launch_ring_nt<T, FWHT, N, 1>(a, s, wide)) \
       return hipGetLastError();                                                               \
     break;
       CM_D(1) CM_D(2) CM_D(3) CM_D(4) CM_D(5) CM_D(6) CM_D(7) CM_D(8)
