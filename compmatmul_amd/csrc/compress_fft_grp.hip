@@ -619,14 +619,14 @@ k_compress_fft_grp_batch(const RegJob* __restrict__ jobs, const int2* __restrict
     const int q = (home + qi) & 7;
     const int base = qstart[q];
     const int len = qstart[q + 1] - base;
-    bool probe = true;
+    int last = -1;
     for (;;) {
-      if (threadIdx.x == 0) item = int(claim_item(&counters[q], len, probe));
-      probe = false;
+      if (threadIdx.x == 0) item = int(claim_item(&counters[q], len, probe_next(qi, len, last)));
       __syncthreads();
       const int it = __builtin_amdgcn_readfirstlane(item);  // uniform: the job lives in scalar registers
       __syncthreads();  // every thread has read `item` before it is rewritten
       if (it >= len) break;
+      last = it;
       const int2 e = items[base + it];
       fft_grp_unit<LOG_G, false>(jobs[e.x], e.y, S, Rb);
       __syncthreads();

@@ -357,13 +357,13 @@ __global__ void __launch_bounds__(64) k_compress_fwht_batch(const RegJob* __rest
     const int q = (home + qi) & 7;
     const int base = qstart[q];
     const int len = qstart[q + 1] - base;
-    bool probe = true;
+    int last = -1;
     for (;;) {
       unsigned it = 0;
-      if (threadIdx.x == 0) it = claim_item(&counters[q], len, probe);
-      probe = false;
+      if (threadIdx.x == 0) it = claim_item(&counters[q], len, probe_next(qi, len, last));
       it = __builtin_amdgcn_readfirstlane(it);
       if (int(it) >= len) break;
+      last = int(it);
       const int2 e = items[base + int(it)];
       fwht_unit(jobs[e.x], e.y, S, Rb);
       __syncthreads();  // the next unit rewrites S and Rb

@@ -7,17 +7,29 @@ namespace cm {
 
 // Claims the next item of a work queue whose counter is `c` and which holds
 // `len` items: returns the item's index, or a value >= len once the queue
-// is exhausted.  With `probe` (a workgroup's first claim on a queue) a
-// relaxed load first passes over an exhausted queue without an atomic,
-// since the failed claims of every resident workgroup on one counter would
-// serialize in L2; later claims on a queue that just gave an item go
-// straight to the atomic.
+// is exhausted.  With `probe`, a relaxed load first passes over an exhausted
+// queue without an atomic: failed claims by every resident workgroup on one
+// counter serialize in L2 with the successful ones.  The load costs a round
+// trip of its own, so callers probe only where the queue may be exhausted
+// (see probe_next).
 __device__ __forceinline__ unsigned claim_item(unsigned* c, int len, bool probe) {
   if (probe) {
     const unsigned seen = __hip_atomic_load(c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (int(seen) >= len) return seen;
   }
   return atomicAdd(c, 1u);
+}
+
+// Whether the next claim on a queue of `len` items should probe: on a queue
+// taken over from another XCD (qi > 0) or one no longer than the grid's
+// share of it, and once the last item claimed (`last`, -1 before any) leaves
+// no more than that share.  Measured on MI355X: probing every claim slowed
+// the 512-product batch by 4%; probing only a workgroup's first claim on
+// each queue left one failed atomic per workgroup and up to twice the
+// compress time on small products.
+__device__ __forceinline__ bool probe_next(int qi, int len, int last) {
+  const int share = int(gridDim.x >> 3) + 1;
+  return qi > 0 || len - last <= share;
 }
 
 template <typename T>
