@@ -145,8 +145,10 @@ int time_launches(int reps, F&& launch, double* ms) {
 extern "C" int cm_probe_alu(int kind, int blocks, int iters, int reps, double* ms, double* flops) {
   using namespace cm;
   std::lock_guard<std::recursive_mutex> lk(rt::mutex());
-  if (kind < CM_PROBE_F32_ADD || kind > CM_PROBE_F64_FMA || blocks < 1 || blocks > (1 << 20) ||
-      iters < 1 || iters > (1 << 24) || reps < 1 || reps > 10000 || !ms || !flops)
+  // at most 2^26 workgroup trips per launch (about 10 ms at the FP64 FMA
+  // rate) and 100 timed launches, so no call runs for long
+  if (kind < CM_PROBE_F32_ADD || kind > CM_PROBE_F64_FMA || blocks < 1 || iters < 1 ||
+      int64_t(blocks) * iters > (int64_t(1) << 26) || reps < 1 || reps > 100 || !ms || !flops)
     return rt::error(CM_EINVAL, "cm_probe_alu: invalid argument");
   int rc = rt::init();
   if (rc) return rc;
@@ -172,7 +174,7 @@ extern "C" int cm_probe_stream(int kind, void* dst, const void* src, size_t byte
   using namespace cm;
   std::lock_guard<std::recursive_mutex> lk(rt::mutex());
   if (kind < CM_PROBE_COPY || kind > CM_PROBE_FILL || bytes == 0 || bytes % 16 || reps < 1 ||
-      reps > 10000 || !ms || !moved)
+      reps > 100 || !ms || !moved)
     return rt::error(CM_EINVAL, "cm_probe_stream: invalid argument");
   int rc = rt::init();
   if (rc) return rc;

@@ -254,7 +254,8 @@ int cm_timing_get(const char* name, double* total_ms, int64_t* launches);
  * cm_probe_alu: `blocks` workgroups of 256 threads, each thread `iters`
  * trips of 16 independent adds (x + c) or FMAs (fma(x, a, c)); *flops is the
  * work of one launch (an FMA counts 2).  Packed FP32 instructions are not
- * used.
+ * used.  blocks * iters is at most 2^26 and reps at most 100 (both probes),
+ * so no call runs for long.
  *
  * cm_probe_stream: 16-byte accesses over `bytes` (a multiple of 16) of
  * 16-byte aligned device memory: copy src to dst (non-overlapping), read

@@ -55,5 +55,10 @@ def test_stream_rejects_bad_buffers():
     assert call(0, a.ptr, a.ptr, 4096) == cma.EINVAL            # overlapping copy
     assert call(2, None, None, 4096) == cma.EINVAL
     assert L.cm_probe_alu(9, 1, 1, 1, ctypes.byref(ms), ctypes.byref(moved)) == cma.EINVAL
+    # work bounds: refused before any launch
+    assert L.cm_probe_alu(0, 1 << 14, 1 << 13, 1, ctypes.byref(ms), ctypes.byref(moved)) == cma.EINVAL
+    assert L.cm_probe_alu(0, 1, 1, 101, ctypes.byref(ms), ctypes.byref(moved)) == cma.EINVAL
+    assert L.cm_probe_stream(0, b.ptr, a.ptr, 4096, 101, ctypes.byref(ms),
+                             ctypes.byref(moved)) == cma.EINVAL
     a.free()
     b.free()
