@@ -2,7 +2,7 @@
 //
 // A plain FP64 restatement of the reference algorithm (mkarppa/compmatmul).
 // Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load
-// this library, and only as the checker (or as the timed CPU baseline).
+// this library, and only to check results (or as the timed CPU baseline).
 //
 // Pinning (all checks live in tests/test_oracle.py):
 //   orc_rng_stream     tests.cpp:90-96 known-answer vector, plus the

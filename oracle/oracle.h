@@ -4,8 +4,8 @@
  * This is a plain FP64 C++ restatement of the reference algorithm
  * (Pagh's compressed matrix multiplication as implemented in rng.hpp,
  * hashing.hpp, matmul_sketch.hpp, transform.hpp, matmul_fft.hpp,
- * matmul_fwht.hpp and array.hpp of mkarppa/compmatmul).  It is used only as
- * the checker by the tests, by __graft_entry__.smoke() and by the
+ * matmul_fwht.hpp and array.hpp of mkarppa/compmatmul).  It is used only to
+ * check results, by the tests and by __graft_entry__.smoke(), and as the
  * cpu_baseline leg of bench.py.  Nothing in the product library links,
  * loads or calls it.
  *

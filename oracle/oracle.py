@@ -2,8 +2,8 @@
 
 The oracle is a plain FP64 restatement of the reference algorithm
 (mkarppa/compmatmul).  Only the tests, ``__graft_entry__.smoke()`` and the
-``cpu_baseline`` leg of ``bench.py`` import this module, and only as the
-checker or as the timed CPU baseline; the product package never does.
+``cpu_baseline`` leg of ``bench.py`` import this module, and only to check
+results or as the timed CPU baseline; the product package never does.
 """
 import ctypes
 import os

@@ -1,4 +1,4 @@
-"""Pins the CPU oracle before it is trusted as the checker.
+"""Pins the CPU oracle before it is trusted to check results.
 
 Each test cites the reference test or the mathematical fact it uses.
 Known-answer values are the literals of the reference's tests.cpp/test.py.
