@@ -7,7 +7,7 @@
 // ALU loops: every thread runs `iters` trips of 16 independent operations
 // (x = x + c, or x = fma(x, a, c)) with a and c kernel arguments, so nothing
 // folds at compile time; 16 chains per thread and 16 waves per CU hide the
-// ALU latency.  Packed FP32 instructions are disabled for this build, so
+// ALU latency.  Packed FP32 instructions are disabled for all device code, so
 // the FP32 rates are those of one operation per lane per instruction.
 // Streams: 16-byte nontemporal vector accesses over a grid-stride loop.
 #include <hip/hip_runtime.h>
