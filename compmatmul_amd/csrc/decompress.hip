@@ -322,6 +322,7 @@ __device__ __forceinline__ uint32_t hash_entry(const uint32_t* h, int64_t i) {
 
 template <typename T, bool FWHT, int D>
 __device__ __forceinline__ void decompress_lds_body(const DecompressArgs& a, int blk) {
+  static_assert(LNT % 64 == 0, "emit_rows sums over whole waves");
   constexpr int ER = rows_per_thread<T, D>();
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int b = a.b;
@@ -406,6 +407,7 @@ __device__ __forceinline__ void stage_half_async(const T* __restrict__ P, unsign
 // refilled after the barrier of row t.
 template <typename T, bool FWHT, int D, int NT, int HP>
 __device__ __forceinline__ void decompress_ring_body(const DecompressArgs& a, int blk) {
+  static_assert(NT % 64 == 0, "emit_rows sums over whole waves");
   constexpr int ER = rows_per_thread<T, D>();
   constexpr int NH = HP * D;  // pieces to copy
   extern __shared__ __align__(16) unsigned char smem_raw[];
